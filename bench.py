@@ -1,0 +1,295 @@
+"""Benchmark of the points / line aggregation path on one MI355X.
+
+Default workload (the headline): 100M float32 points, 90% in one 0.01-sigma Gaussian and
+10% uniform, value lognormal(0, 2), Canvas 1920x1080 with the data extent as ranges,
+``summary(count=count(), mean=mean('v'))``.  A step is one ``Canvas.points`` call on the
+HBM-resident columns (plane creation, the aggregation kernel, finalize, result wrap).
+
+Prints ONE JSON line: throughput in Gpoints/s, the aggregation kernel's achieved HBM
+bandwidth (algorithmic bytes per launch / average launch time from HIP events on the
+kernel's stream) against the 8 TB/s peak, and the CPU oracle timed on the host's cores.
+
+Other workloads: --workload points800 | points_skewed | bycat | lines  (see --help).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+class KernelTimer:
+    """HIP events recorded on the current stream around each aggregation launch."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.pairs = []
+
+    def start(self):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record(self.torch.cuda.current_stream())
+        self.pairs.append([e, None])
+
+    def stop(self):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record(self.torch.cuda.current_stream())
+        self.pairs[-1][1] = e
+
+    def mean_ms(self):
+        self.torch.cuda.synchronize()
+        d = [a.elapsed_time(b) for a, b in self.pairs]
+        return sum(d) / len(d) if d else float("nan")
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(name, torch, scale=1.0):
+    """Returns (callable step, rows per step, algorithmic bytes per aggregation launch,
+    config dict, cpu_baseline callable or None)."""
+    import numpy as np
+
+    import datashader_amd as dsa
+    from datashader_amd import synthetic
+    from datashader_amd import _device as dev
+
+    if name == "points_skewed":
+        n = int(100_000_000 * scale)
+        W, H = 1920, 1080
+        x, y, v = synthetic.points_skewed_dev(n, seed=2)
+        frame = dsa.DeviceFrame({"x": x, "y": y, "v": v})
+        xr = dev.maybe_expand_bounds(dev.bounds(x))
+        yr = dev.maybe_expand_bounds(dev.bounds(y))
+        canvas = dsa.Canvas(W, H, x_range=xr, y_range=yr)
+        agg = dsa.summary(count=dsa.count(), mean=dsa.mean("v"))
+
+        def step():
+            return canvas.points(frame, "x", "y", agg)
+
+        algo = 12 * n + (4 + 8) * W * H
+        cfg = {"workload": "points 1920x1080 count+mean, 100M float32 rows (90% in a "
+                           "0.01-sigma Gaussian, 10% uniform), value lognormal(0,2)",
+               "rows": n, "canvas": [W, H], "aggs": ["count()", "mean('v')"]}
+
+        def cpu(sample_rows):
+            import oracle
+            m = min(sample_rows, n)
+            xs = x[:m].double().cpu().numpy()
+            ys = y[:m].double().cpu().numpy()
+            vs = v[:m].double().cpu().numpy()
+            sx, tx = canvas.x_axis.compute_scale_and_translate(xr, W)
+            sy, ty = canvas.y_axis.compute_scale_and_translate(yr, H)
+            view = oracle.View(sx, tx, sy, ty, xr[0], xr[1], yr[0], yr[1])
+            times = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                oracle.points(xs, ys, view, W, H, ("count", "countv", "sumz"), val=vs)
+                times.append(time.perf_counter() - t0)
+            return m, statistics.median(times)
+
+        def check(res):
+            c = res["count"].data
+            return int(c.sum().item())
+
+        return step, n, algo, cfg, cpu, check
+
+    if name == "points800":
+        n = int(10_000_000 * scale)
+        W, H = 800, 600
+        x, y, _ = synthetic.points_mixture_dev(n, seed=1, dtype=torch.float64)
+        frame = dsa.DeviceFrame({"x": x, "y": y})
+        canvas = dsa.Canvas(W, H, x_range=(-10, 10), y_range=(-10, 10))
+        agg = dsa.count()
+
+        def step():
+            return canvas.points(frame, "x", "y", agg)
+
+        algo = 16 * n + 4 * W * H
+        cfg = {"workload": "points 800x600 count, 10M float64 rows (5-Gaussian mixture, "
+                           "0.1% NaN rows)", "rows": n, "canvas": [W, H], "aggs": ["count()"]}
+
+        def cpu(sample_rows):
+            import oracle
+            m = min(sample_rows, n)
+            xs, ys = x[:m].cpu().numpy(), y[:m].cpu().numpy()
+            view = oracle.View(40.0, 400.0, 30.0, 300.0, -10, 10, -10, 10)
+            times = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                oracle.points(xs, ys, view, W, H, ("count",))
+                times.append(time.perf_counter() - t0)
+            return m, statistics.median(times)
+
+        def check(res):
+            return int(res.data.sum().item())
+
+        return step, n, algo, cfg, cpu, check
+
+    if name == "bycat":
+        n = int(500_000_000 * scale)
+        W = H = 1024
+        x, y, cat = synthetic.points_ridges_dev(n, seed=3)
+        cats = [f"c{k}" for k in range(16)]
+        frame = dsa.DeviceFrame({"x": x, "y": y, "cat": cat}, categories={"cat": cats})
+        xr = dev.maybe_expand_bounds(dev.bounds(x))
+        yr = dev.maybe_expand_bounds(dev.bounds(y))
+        canvas = dsa.Canvas(W, H, x_range=xr, y_range=yr)
+        agg = dsa.by("cat", dsa.count())
+
+        def step():
+            return canvas.points(frame, "x", "y", agg)
+
+        algo = 9 * n + 64 * W * H
+        cfg = {"workload": "points 1024x1024 by('cat', count()), 500M float32 rows "
+                           "(3 sinusoidal ridges), 16 Zipf(1.2) categories",
+               "rows": n, "canvas": [W, H], "aggs": ["by('cat', count())"]}
+
+        def cpu(sample_rows):
+            import oracle
+            m = min(sample_rows, n)
+            xs, ys = x[:m].double().cpu().numpy(), y[:m].double().cpu().numpy()
+            cs = cat[:m].long().cpu().numpy()
+            sx, tx = canvas.x_axis.compute_scale_and_translate(xr, W)
+            sy, ty = canvas.y_axis.compute_scale_and_translate(yr, H)
+            view = oracle.View(sx, tx, sy, ty, xr[0], xr[1], yr[0], yr[1])
+            times = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                oracle.points(xs, ys, view, W, H, ("count",), cat=cs, ncat=16)
+                times.append(time.perf_counter() - t0)
+            return m, statistics.median(times)
+
+        def check(res):
+            return int(res.data.sum().item())
+
+        return step, n, algo, cfg, cpu, check
+
+    if name == "lines":
+        nlines = int(1_000_000 * scale)
+        W, H = 2048, 1024
+        xs, block = synthetic.random_walks_dev(nlines, 256, seed=4)
+        names = [f"y{j}" for j in range(256)]
+        frame = dsa.DeviceFrame.from_block(names, block)
+        # autorange, as LinesAxis1XConstant computes it (line.py:328-331, 249-255)
+        xr = dev.maybe_expand_bounds((np.nanmin(xs), np.nanmax(xs)))
+        ylo, yhi = dev.bounds(block)
+        yr = dev.maybe_expand_bounds((ylo, yhi))
+        canvas = dsa.Canvas(W, H, x_range=xr, y_range=yr)
+        agg = dsa.count()
+
+        def step():
+            return canvas.line(frame, x=xs, y=names, agg=agg, axis=1)
+
+        algo = 4 * 256 * nlines + 8 * 256 + 4 * W * H
+        cfg = {"workload": "lines 2048x1024 count, 1M random walks x 256 float32 samples, "
+                           "shared x = linspace(0,1,256)", "lines": nlines, "canvas": [W, H],
+               "aggs": ["count()"]}
+
+        def cpu(sample_rows):
+            import oracle
+            m = min(sample_rows, nlines)
+            ys = block[:, :m].t().double().cpu().numpy()
+            sx, tx = canvas.x_axis.compute_scale_and_translate(xr, W)
+            sy, ty = canvas.y_axis.compute_scale_and_translate(yr, H)
+            view = oracle.View(sx, tx, sy, ty, xr[0], xr[1], yr[0], yr[1], yf32=True)
+            times = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                oracle.lines_xconst(xs, ys, view, W, H, ("count",))
+                times.append(time.perf_counter() - t0)
+            return m * 256, statistics.median(times)
+
+        def check(res):
+            return int(res.data.sum().item())
+
+        return step, nlines * 256, algo, cfg, cpu, check
+
+    raise SystemExit(f"unknown workload {name!r}")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="points_skewed",
+                    choices=["points_skewed", "points800", "bycat", "lines"])
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload size")
+    ap.add_argument("--cpu-sample", type=int, default=20_000_000,
+                    help="rows (or line samples) of the CPU oracle baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "form of this path is a design in DESIGN.md and is not built", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from datashader_amd import backend
+
+    torch.cuda.set_device(0)
+    _log(f"[bench] building workload {args.workload} (scale {args.scale})")
+    step, rows, algo, cfg, cpu, check = build_workload(args.workload, torch, args.scale)
+    torch.cuda.synchronize()
+    for i in range(args.warmup):
+        res = step()
+    torch.cuda.synchronize()
+    _log(f"[bench] warmup done ({args.warmup} steps)")
+
+    timer = KernelTimer(torch)
+    backend.set_kernel_timer(timer)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    backend.set_kernel_timer(None)
+    kern_ms = timer.mean_ms()
+    ms = (t1 - t0) * 1e3 / args.steps
+    checksum = check(res)
+    _log(f"[bench] {args.steps} steps: {ms:.3f} ms/step, kernel {kern_ms:.3f} ms, "
+         f"checksum {checksum}")
+
+    achieved = algo / (kern_ms * 1e-3) / 1e9
+    unit_rows = "Gpoints/s" if args.workload != "lines" else "Gsamples/s"
+    value = rows * args.steps / (t1 - t0) / 1e9
+
+    cpu_baseline = None
+    if cpu is not None and not args.no_cpu:
+        _log(f"[bench] CPU oracle baseline on {args.cpu_sample} rows")
+        m, sec = cpu(args.cpu_sample)
+        cpu_baseline = {"value": m / sec / 1e9, "unit": unit_rows, "cores": 1, "kind": "port",
+                        "sample": f"first {m} rows of the same workload, single-thread C "
+                                  f"oracle, median of 3"}
+
+    out = {
+        "metric": "Gpoints/s aggregated + % HBM roofline" if args.workload != "lines"
+        else "Gsamples/s aggregated (line vertices) + % HBM roofline",
+        "value": value, "unit": unit_rows, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 projection)",
+        "data": "synthetic (seeded, generated on the device)", "config": cfg,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel_ms": kern_ms, "algorithmic_bytes": algo},
+        "cpu_baseline": cpu_baseline,
+        "checksum": checksum,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,20 @@
+"""datashader_amd -- an MI355X-native (gfx950) implementation of datashader's points / line
+projection + aggregation path.
+
+Drop-in for ``datashader.Canvas.points`` / ``Canvas.line`` with the reductions
+``count, any, sum, mean, min, max, var, std, by, count_cat, summary``: the same call
+signatures and the same DataArray / Dataset results (dims, coords, attrs, dtypes).  The
+projection and aggregation run as hand-written HIP kernels in ``libdsx.so`` (C ABI in
+``include/dsx.h``), called through ctypes; columns live in HBM as torch-ROCm tensors.
+"""
+from __future__ import annotations
+
+__version__ = "0.1.0"
+
+from .core import Canvas, LinearAxis, LogAxis, bypixel  # noqa: F401
+from .frame import DeviceFrame, Schema  # noqa: F401
+from .reductions import (any, by, category_binning, category_codes,  # noqa: F401,A004
+                         category_modulo, count, count_cat, max, mean, min, std, sum,
+                         summary, var)
+from . import backend as _backend  # noqa: F401  (registers the pipelines)
+from .xrlite import DataArray, Dataset  # noqa: F401

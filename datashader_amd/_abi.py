@@ -1,0 +1,123 @@
+"""ctypes binding of libdsx.so (``include/dsx.h``).
+
+The structures below mirror the header field for field.  Every call checks the returned
+status and raises; there is no fallback: without the built library (or without a GPU)
+the aggregation path fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libdsx.so")
+
+ABI_VERSION = 1
+
+# dtype codes
+F32, F64, I8, U8, I16, I32, I64 = 1, 2, 3, 4, 5, 6, 7
+# fill modes
+FILL_ZERO32, FILL_ONES32, FILL_ZERO64, FILL_ONES64 = 1, 2, 3, 4
+# finalize modes
+FIN_ANY, FIN_SUM, FIN_MEAN, FIN_VAR, FIN_STD, FIN_MIN, FIN_MAX, FIN_SUMZERO = range(1, 9)
+
+EXPORTS = ("dsx_points_agg", "dsx_points_m2", "dsx_lines_agg", "dsx_bounds_workspace_size",
+           "dsx_bounds", "dsx_fill", "dsx_finalize", "dsx_abi_version", "dsx_strerror",
+           "dsx_last_hip_error")
+
+vp = ctypes.c_void_p
+
+
+class Accums(ctypes.Structure):
+    _fields_ = [("count", vp), ("countv", vp), ("sum_hi", vp), ("sum_lo", vp),
+                ("sum64", vp), ("min32", vp), ("max32", vp), ("min64", vp), ("max64", vp)]
+
+
+class View(ctypes.Structure):
+    _fields_ = [("sx", ctypes.c_double), ("tx", ctypes.c_double),
+                ("sy", ctypes.c_double), ("ty", ctypes.c_double),
+                ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+                ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
+                ("xxmax", ctypes.c_int64), ("yymax", ctypes.c_int64),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("xlog", ctypes.c_int32), ("ylog", ctypes.c_int32)]
+
+
+CAT_NONE, CAT_CODES, CAT_MODULO, CAT_BINNING = 0, 1, 2, 3
+
+
+class Categorizer(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("dtype", ctypes.c_int32), ("col", vp),
+                ("ncat", ctypes.c_int32), ("nbins", ctypes.c_int32),
+                ("bin_under", ctypes.c_int32), ("bin_over", ctypes.c_int32),
+                ("offset", ctypes.c_int64), ("modulo", ctypes.c_int64),
+                ("bin0", ctypes.c_double), ("binsize", ctypes.c_double)]
+
+
+class PointsDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("x", vp), ("y", vp), ("xy_dtype", ctypes.c_int32),
+                ("val_dtype", ctypes.c_int32), ("val", vp), ("cat", Categorizer)]
+
+
+class LinesDesc(ctypes.Structure):
+    _fields_ = [("nlines", ctypes.c_int64), ("nverts", ctypes.c_int64), ("x", vp), ("y", vp),
+                ("x_dtype", ctypes.c_int32), ("y_dtype", ctypes.c_int32),
+                ("x_line_stride", ctypes.c_int64), ("x_vert_stride", ctypes.c_int64),
+                ("y_line_stride", ctypes.c_int64), ("y_vert_stride", ctypes.c_int64),
+                ("axis", ctypes.c_int32), ("plot_start", ctypes.c_int32),
+                ("val", vp), ("val_dtype", ctypes.c_int32), ("cat", Categorizer)]
+
+
+class FinalizeDesc(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("val_dtype", ctypes.c_int32),
+                ("ncells", ctypes.c_int64), ("count", vp), ("countv", vp), ("hi", vp),
+                ("lo", vp), ("v64", vp), ("key", vp), ("out", vp)]
+
+
+class DsxError(RuntimeError):
+    """A libdsx call returned a non-zero status."""
+
+
+_lib = None
+
+
+def load():
+    """Load libdsx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m datashader_amd.build` "
+                "(the HIP library is required; there is no CPU fallback)")
+        lib = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        i32, i64, sz = ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t
+        lib.dsx_points_agg.argtypes = [P(PointsDesc), P(View), P(Accums), vp]
+        lib.dsx_points_m2.argtypes = [P(PointsDesc), P(View), vp, vp, vp, vp, vp]
+        lib.dsx_lines_agg.argtypes = [P(LinesDesc), P(View), P(Accums), vp]
+        lib.dsx_bounds_workspace_size.argtypes = [i64]
+        lib.dsx_bounds_workspace_size.restype = sz
+        lib.dsx_bounds.argtypes = [vp, i32, i64, vp, vp, sz, vp]
+        lib.dsx_fill.argtypes = [vp, i32, i64, vp]
+        lib.dsx_finalize.argtypes = [P(FinalizeDesc), vp]
+        lib.dsx_abi_version.argtypes = []
+        lib.dsx_strerror.argtypes = [ctypes.c_int]
+        lib.dsx_strerror.restype = ctypes.c_char_p
+        lib.dsx_last_hip_error.argtypes = []
+        lib.dsx_last_hip_error.restype = ctypes.c_char_p
+        for f in ("dsx_points_agg", "dsx_points_m2", "dsx_lines_agg", "dsx_bounds",
+                  "dsx_fill", "dsx_finalize", "dsx_abi_version"):
+            getattr(lib, f).restype = ctypes.c_int
+        if lib.dsx_abi_version() != ABI_VERSION:
+            raise RuntimeError("libdsx.so ABI version mismatch; rebuild the library")
+        _lib = lib
+    return _lib
+
+
+def check(status: int, what: str):
+    if status != 0:
+        lib = load()
+        msg = lib.dsx_strerror(status).decode()
+        if status == -5:
+            msg += ": " + lib.dsx_last_hip_error().decode()
+        raise DsxError(f"{what} failed ({status}): {msg}")

@@ -1,0 +1,349 @@
+"""The MI355X backend of ``bypixel`` (the analogue of ``data_libraries/pandas.py`` and
+``data_libraries/cudf.py``).
+
+``pipeline`` follows ``data_libraries/pandas.py:23-66`` step by step -- bounds, scale /
+translate, coordinates, create, extend, finalize -- with every step that touches the
+columns running as a libdsx kernel:
+
+* reductions are grouped into *passes*: one kernel launch per (value column,
+  categorizer) pair, producing the union of the accumulator planes its reductions need
+  (``compiler.py:36-150`` dedups bases the same way);
+* planes are created with the reference's initial values by ``dsx_fill``
+  (``reductions.py:446-472``);
+* ``dsx_points_agg`` / ``dsx_lines_agg`` is the ``extend`` call;
+* var / std add a second pass (``dsx_points_m2``) over the per-bin means;
+* ``dsx_finalize`` turns planes into the reference's output dtypes, and the result is
+  wrapped with the same dims / coords / attrs as ``pandas.py:61-66``.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pandas as pd
+
+from . import _abi
+from . import _device as dev
+from . import reductions as rd
+from .core import bypixel
+from .frame import DSX_DTYPE, DeviceFrame
+from .glyphs import Point, _LineLike
+
+try:  # pragma: no cover - xarray is not part of this image
+    import xarray as _xr
+except Exception:  # noqa: BLE001
+    _xr = None
+from . import xrlite
+
+
+# ---- instrumentation ------------------------------------------------------------------
+# An optional timer with start() / stop() methods, called around every aggregation
+# kernel launch (on the stream the kernel runs on).  None in normal use.
+_kernel_timer = None
+
+
+def set_kernel_timer(timer):
+    global _kernel_timer
+    _kernel_timer = timer
+
+
+def _timed(call):
+    t = _kernel_timer
+    if t is None:
+        return call()
+    t.start()
+    r = call()
+    t.stop()
+    return r
+
+
+# ---- planning ---------------------------------------------------------------------
+
+class Pass:
+    """One kernel launch: a value column (or none) and a categorizer (or none)."""
+
+    def __init__(self, val, cat):
+        self.val = val
+        self.cat = cat
+        self.planes = set()
+        self.m2 = False
+
+    def __repr__(self):
+        return f"Pass(val={self.val!r}, cat={self.cat!r}, planes={sorted(self.planes)}, m2={self.m2})"
+
+
+def plan(agg):
+    """Group the leaves of ``agg`` into passes; returns (passes, [(name, red, pass)])."""
+    passes = {}
+    leaves = []
+    for name, red in rd.leaves(agg):
+        cat = red.categorizer if isinstance(red, rd.by) else None
+        inner = red.reduction if isinstance(red, rd.by) else red
+        key = (inner.column, cat)
+        p = passes.get(key)
+        if p is None:
+            p = passes[key] = Pass(inner.column, cat)
+        p.planes.update(inner.planes())
+        p.m2 |= inner.needs_m2
+        leaves.append((name, red, key))
+    # a pass with no value column rides along with a value pass of the same categorizer
+    for key in [k for k in passes if k[0] is None]:
+        host = next((k for k in passes if k[0] is not None and k[1] == key[1]), None)
+        if host is not None:
+            passes[host].planes.update(passes.pop(key).planes)
+            leaves = [(n, r, host if k == key else k) for (n, r, k) in leaves]
+    return passes, leaves
+
+
+# ---- execution ----------------------------------------------------------------------
+
+def _alloc(ncells, kind, device):
+    t = dev.torch()
+    dt = {"u32": t.int32, "f32": t.float32, "f64": t.float64, "u64": t.int64}[kind]
+    return t.empty(ncells, dtype=dt, device=device)
+
+
+def _fill(lib, plane, mode):
+    _abi.check(lib.dsx_fill(dev.ptr(plane), mode, plane.numel(), dev.stream_handle()), "dsx_fill")
+
+
+def _categorizer(cat, frame, schema):
+    c = _abi.Categorizer()
+    if cat is None:
+        c.kind, c.ncat = _abi.CAT_NONE, 1
+        return c, None, 1
+    args = cat.kernel_args(schema)
+    col = frame[cat.column]
+    c.kind = args["kind"]
+    c.ncat = args["ncat"]
+    c.dtype = dev.dtype_code(col)
+    c.col = dev.ptr(col)
+    c.offset = args.get("offset", 0)
+    c.modulo = args.get("modulo", 1)
+    c.nbins = args.get("nbins", 0)
+    c.bin_under = args.get("bin_under", 0)
+    c.bin_over = args.get("bin_over", 0)
+    c.bin0 = args.get("bin0", 0.0)
+    c.binsize = args.get("binsize", 1.0)
+    return c, col, args["ncat"]
+
+
+class PassResult:
+    def __init__(self, ncat, f32, planes):
+        self.ncat = ncat
+        self.f32 = f32
+        self.planes = planes
+
+
+def run_pass(lib, frame, schema, glyph, view, p, H, W):
+    t = dev.torch()
+    device = frame.device
+    catc, catcol, ncat = _categorizer(p.cat, frame, schema)
+    ncells = H * W * ncat
+    val = dev.as_float_column(frame[p.val]) if p.val is not None else None
+    f32 = val is not None and val.dtype == t.float32
+    planes = {}
+    acc = _abi.Accums()
+    if "count" in p.planes:
+        planes["count"] = _alloc(ncells, "u32", device)
+        _fill(lib, planes["count"], _abi.FILL_ZERO32)
+        acc.count = dev.ptr(planes["count"])
+    if "countv" in p.planes:
+        planes["countv"] = _alloc(ncells, "u32", device)
+        _fill(lib, planes["countv"], _abi.FILL_ZERO32)
+        acc.countv = dev.ptr(planes["countv"])
+    if "sum" in p.planes:
+        if f32:
+            planes["sum_hi"] = _alloc(ncells, "f32", device)
+            planes["sum_lo"] = _alloc(ncells, "f32", device)
+            _fill(lib, planes["sum_hi"], _abi.FILL_ZERO32)
+            _fill(lib, planes["sum_lo"], _abi.FILL_ZERO32)
+            acc.sum_hi, acc.sum_lo = dev.ptr(planes["sum_hi"]), dev.ptr(planes["sum_lo"])
+        else:
+            planes["sum64"] = _alloc(ncells, "f64", device)
+            _fill(lib, planes["sum64"], _abi.FILL_ZERO64)
+            acc.sum64 = dev.ptr(planes["sum64"])
+    for m, empty32, empty64 in (("min", _abi.FILL_ONES32, _abi.FILL_ONES64),
+                                ("max", _abi.FILL_ZERO32, _abi.FILL_ZERO64)):
+        if m in p.planes:
+            k = _alloc(ncells, "u32" if f32 else "u64", device)
+            _fill(lib, k, empty32 if f32 else empty64)
+            planes[m] = k
+            setattr(acc, m + ("32" if f32 else "64"), dev.ptr(k))
+    vdt = (_abi.F32 if f32 else _abi.F64) if val is not None else 0
+    vw = _view_struct(view)
+    stream = dev.stream_handle()
+    if isinstance(glyph, Point):
+        x = dev.as_float_column(frame[glyph.x])
+        y = dev.as_float_column(frame[glyph.y])
+        if x.dtype != y.dtype:  # x and y share one dtype in the kernel
+            x, y = x.to(t.float64), y.to(t.float64)
+        d = _abi.PointsDesc()
+        d.n = len(frame)
+        d.x, d.y = dev.ptr(x), dev.ptr(y)
+        d.xy_dtype = dev.dtype_code(x)
+        d.val, d.val_dtype = dev.ptr(val), vdt
+        d.cat = catc
+        _abi.check(_timed(lambda: lib.dsx_points_agg(d, vw, acc, stream)), "dsx_points_agg")
+        if p.m2:
+            mean = t.empty(ncells, dtype=t.float64, device=device)
+            _finalize(lib, _abi.FIN_MEAN, f32, ncells, planes, mean)
+            if f32:
+                planes["m2_hi"] = _alloc(ncells, "f32", device)
+                planes["m2_lo"] = _alloc(ncells, "f32", device)
+                _fill(lib, planes["m2_hi"], _abi.FILL_ZERO32)
+                _fill(lib, planes["m2_lo"], _abi.FILL_ZERO32)
+                m2p = (dev.ptr(planes["m2_hi"]), dev.ptr(planes["m2_lo"]), None)
+            else:
+                planes["m2_64"] = _alloc(ncells, "f64", device)
+                _fill(lib, planes["m2_64"], _abi.FILL_ZERO64)
+                m2p = (None, None, dev.ptr(planes["m2_64"]))
+            _abi.check(lib.dsx_points_m2(d, vw, dev.ptr(mean), *m2p, stream), "dsx_points_m2")
+    elif isinstance(glyph, _LineLike):
+        if p.m2:
+            raise NotImplementedError("var / std of lines are not supported")
+        lay = glyph.vertex_layout(frame)
+        d = _abi.LinesDesc()
+        d.nlines, d.nverts = lay["nlines"], lay["nverts"]
+        d.x, d.y = dev.ptr(lay["x"]), dev.ptr(lay["y"])
+        d.x_dtype, d.y_dtype = dev.dtype_code(lay["x"]), dev.dtype_code(lay["y"])
+        d.x_line_stride, d.x_vert_stride = lay["xls"], lay["xvs"]
+        d.y_line_stride, d.y_vert_stride = lay["yls"], lay["yvs"]
+        d.axis = lay["axis"]
+        d.plot_start = 1
+        d.val, d.val_dtype = dev.ptr(val), vdt
+        d.cat = catc
+        _abi.check(_timed(lambda: lib.dsx_lines_agg(d, vw, acc, stream)), "dsx_lines_agg")
+    else:
+        raise NotImplementedError(f"glyph {type(glyph).__name__} is not supported")
+    return PassResult(ncat, f32, planes)
+
+
+def _finalize(lib, mode, f32, ncells, planes, out, key=None, counts=None):
+    f = _abi.FinalizeDesc()
+    f.mode = mode
+    f.val_dtype = _abi.F32 if f32 else _abi.F64
+    f.ncells = ncells
+    if mode == _abi.FIN_ANY:
+        f.count = dev.ptr(counts)
+    elif mode in (_abi.FIN_MIN, _abi.FIN_MAX):
+        f.key = dev.ptr(key)
+    else:
+        f.countv = dev.ptr(planes["countv"])
+        if mode in (_abi.FIN_VAR, _abi.FIN_STD):
+            hi, lo, v64 = planes.get("m2_hi"), planes.get("m2_lo"), planes.get("m2_64")
+        else:
+            hi, lo, v64 = planes.get("sum_hi"), planes.get("sum_lo"), planes.get("sum64")
+        f.hi, f.lo, f.v64 = dev.ptr(hi), dev.ptr(lo), dev.ptr(v64)
+    f.out = dev.ptr(out)
+    _abi.check(lib.dsx_finalize(f, dev.stream_handle()), "dsx_finalize")
+
+
+def finalize_leaf(lib, red, res, H, W):
+    """Device tensor holding the reference's finalized array of one reduction."""
+    t = dev.torch()
+    inner = red.reduction if isinstance(red, rd.by) else red
+    shape = (H, W, res.ncat) if isinstance(red, rd.by) else (H, W)
+    ncells = H * W * res.ncat
+    device = next(iter(res.planes.values())).device
+    if isinstance(inner, rd.count):
+        plane = res.planes["count" if inner.column is None else "countv"]
+        return plane.view(t.uint32).reshape(shape)
+    if isinstance(inner, rd.any):
+        counts = res.planes["count" if inner.column is None else "countv"]
+        out = t.empty(ncells, dtype=t.uint8, device=device)
+        _finalize(lib, _abi.FIN_ANY, res.f32, ncells, res.planes, out, counts=counts)
+        return out.view(t.bool).reshape(shape)
+    out = t.empty(ncells, dtype=t.float64, device=device)
+    mode = inner.finalize_mode
+    key = res.planes.get("min") if mode == _abi.FIN_MIN else res.planes.get("max")
+    _finalize(lib, mode, res.f32, ncells, res.planes, out, key=key)
+    return out.reshape(shape)
+
+
+# ---- the pipeline (data_libraries/pandas.py:23-66) ---------------------------------
+
+def _view_struct(view):
+    v = _abi.View()
+    for k in ("sx", "tx", "sy", "ty", "xmin", "xmax", "ymin", "ymax"):
+        setattr(v, k, float(view[k]))
+    v.xxmax, v.yymax = int(view["xxmax"]), int(view["yymax"])
+    v.width, v.height = int(view["W"]), int(view["H"])
+    v.xlog, v.ylog = int(view["xlog"]), int(view["ylog"])
+    return v
+
+
+def make_view(canvas, x_range, y_range):
+    """Scale / translate, bounds and line-snapping limits of one call."""
+    W, H = canvas.plot_width, canvas.plot_height
+    x_st = canvas.x_axis.compute_scale_and_translate(x_range, W)
+    y_st = canvas.y_axis.compute_scale_and_translate(y_range, H)
+    sx, tx = x_st
+    sy, ty = y_st
+    # map_onto_pixel_snap (line.py:716-717): round(mapper(xmax)*sx + tx), the scalar
+    # types of the operands kept (a float32 range stays float32), half to even.
+    xxmax = int(np.rint(canvas.x_axis.mapper(x_range[1]) * sx + tx))
+    yymax = int(np.rint(canvas.y_axis.mapper(y_range[1]) * sy + ty))
+    return dict(sx=sx, tx=tx, sy=sy, ty=ty, xmin=x_range[0], xmax=x_range[1],
+                ymin=y_range[0], ymax=y_range[1], xxmax=xxmax, yymax=yymax, W=W, H=H,
+                xlog=canvas.x_axis.log, ylog=canvas.y_axis.log, x_st=x_st, y_st=y_st)
+
+
+def aggregate(frame, schema, canvas, glyph, agg):
+    """Run the aggregation on the device; returns (results, view, x_range, y_range)
+    where ``results`` is ``[(name, reduction, device tensor)]``."""
+    dev.require_gpu()
+    lib = _abi.load()
+    x_range = canvas.x_range or glyph.compute_x_bounds(frame)
+    y_range = canvas.y_range or glyph.compute_y_bounds(frame)
+    canvas.validate_ranges(x_range, y_range)
+    view = make_view(canvas, x_range, y_range)
+    H, W = view["H"], view["W"]
+    passes, leaves = plan(agg)
+    results = {key: run_pass(lib, frame, schema, glyph, view, p, H, W)
+               for key, p in passes.items()}
+    out = [(name, red, finalize_leaf(lib, red, results[key], H, W))
+           for (name, red, key) in leaves]
+    return out, view, x_range, y_range
+
+
+def _wrap(red, data, canvas, glyph, view, x_range, y_range, schema, to_host, name=None):
+    x_axis = canvas.x_axis.compute_index(view["x_st"], view["W"])
+    y_axis = canvas.y_axis.compute_index(view["y_st"], view["H"])
+    coords = {glyph.x_label: x_axis, glyph.y_label: y_axis}
+    dims = [glyph.y_label, glyph.x_label]
+    if isinstance(red, rd.by):
+        dims.append(red.cat_column)
+        coords[red.cat_column] = red.categorizer.categories(schema)
+    attrs = dict(x_range=x_range, y_range=y_range)
+    if to_host:
+        data = data.cpu().numpy()
+        if _xr is not None:
+            return _xr.DataArray(data, coords=coords, dims=dims, attrs=attrs, name=name)
+    return xrlite.DataArray(data, coords=coords, dims=dims, attrs=attrs, name=name)
+
+
+def pipeline(source, schema, canvas, glyph, agg, *, antialias=False, to_host=None):
+    """``bypixel.pipeline`` implementation for pandas and device sources."""
+    if antialias:
+        raise NotImplementedError("antialiased lines are not supported by the MI355X backend")
+    if isinstance(source, pd.DataFrame):
+        dev.require_gpu()
+        frame = DeviceFrame.from_pandas(source)
+        to_host = True if to_host is None else to_host
+    else:
+        frame = source
+        to_host = False if to_host is None else to_host
+    res, view, x_range, y_range = aggregate(frame, schema, canvas, glyph, agg)
+    if isinstance(agg, rd.summary):
+        data = {name: _wrap(red, t_, canvas, glyph, view, x_range, y_range, schema, to_host,
+                            name=name) for name, red, t_ in res}
+        first = agg.keys[0]
+        attrs = {a: data[first].attrs[a] for a in ("x_range", "y_range")}
+        if to_host and _xr is not None:
+            return _xr.Dataset(data, attrs=attrs)
+        return xrlite.Dataset(data, attrs=attrs)
+    (_, red, t_), = res
+    return _wrap(red, t_, canvas, glyph, view, x_range, y_range, schema, to_host)
+
+
+bypixel.pipeline.register(pd.DataFrame)(pipeline)
+bypixel.pipeline.register(DeviceFrame)(pipeline)

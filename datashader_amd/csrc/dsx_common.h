@@ -1,0 +1,186 @@
+// dsx_common.h -- shared device helpers of libdsx (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/dsx.h"
+
+namespace dsx {
+
+// ---- status plumbing ---------------------------------------------------------------
+void record_hip_error(hipError_t e);
+int launch_status();  // DSX_OK or DSX_ERR_HIP after a launch
+
+static inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---- dtype loads -------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ double load_f(const void *p, int64_t i) {
+    return (double)(reinterpret_cast<const T *>(p)[i]);
+}
+
+// Category code of row i for a runtime dtype (the branch is wave-uniform).
+__device__ __forceinline__ int64_t load_code(const void *p, int32_t dt, int64_t i) {
+    switch (dt) {
+    case DSX_I8: return reinterpret_cast<const int8_t *>(p)[i];
+    case DSX_U8: return reinterpret_cast<const uint8_t *>(p)[i];
+    case DSX_I16: return reinterpret_cast<const int16_t *>(p)[i];
+    case DSX_I32: return reinterpret_cast<const int32_t *>(p)[i];
+    default: return reinterpret_cast<const int64_t *>(p)[i];
+    }
+}
+
+// by()'s categorizer (reductions.py:143-259).  Returns the category of row i in
+// [0, ncat), or -1 when the row has no category inside the aggregate.
+struct Cat {
+    int32_t kind, dtype, ncat, nbins, bin_under, bin_over;
+    const void *col;
+    int64_t offset, modulo;
+    double bin0, binsize;
+};
+
+inline Cat cat_of(const dsx_categorizer &c) {
+    Cat k;
+    k.kind = c.kind; k.dtype = c.dtype; k.ncat = c.kind == DSX_CAT_NONE ? 1 : c.ncat;
+    k.nbins = c.nbins; k.bin_under = c.bin_under; k.bin_over = c.bin_over;
+    k.col = c.col; k.offset = c.offset; k.modulo = c.modulo;
+    k.bin0 = c.bin0; k.binsize = c.binsize;
+    return k;
+}
+
+__device__ __forceinline__ int64_t category(const Cat &c, int64_t i) {
+    if (c.kind == DSX_CAT_NONE) return 0;
+    int64_t k;
+    if (c.kind == DSX_CAT_BINNING) {
+        // category_binning.apply (reductions.py:242-258): numpy evaluates
+        // (values - bin0) / binsize in the column's float type (float32 stays float32).
+        bool nan_v;
+        int64_t idx = 0;
+        if (c.dtype == DSX_F32) {
+            const float v = reinterpret_cast<const float *>(c.col)[i];
+            nan_v = v != v;
+            if (!nan_v) idx = (int64_t)((v - (float)c.bin0) / (float)c.binsize);
+        } else {
+            const double v = c.dtype == DSX_F64 ? reinterpret_cast<const double *>(c.col)[i]
+                                                : (double)load_code(c.col, c.dtype, i);
+            nan_v = v != v;
+            if (!nan_v) idx = (int64_t)((v - c.bin0) / c.binsize);
+        }
+        if (nan_v) k = c.nbins;
+        else if (idx < 0) k = c.bin_under;
+        else if (idx >= c.nbins) k = c.bin_over;
+        else k = idx;
+    } else {
+        const int64_t raw = load_code(c.col, c.dtype, i);
+        if (c.kind == DSX_CAT_MODULO) {
+            int64_t r = (raw - c.offset) % c.modulo;  // Python's floor modulo
+            if (r != 0 && ((r < 0) != (c.modulo < 0))) r += c.modulo;
+            k = r;
+        } else {
+            k = raw;
+        }
+    }
+    if (k < 0) k += c.ncat;  // numba's negative-index wraparound (code -1 = NaN category)
+    if (k < 0 || k >= c.ncat) return -1;
+    return k;
+}
+
+static inline int check_cat(const dsx_categorizer &c) {
+    if (c.kind == DSX_CAT_NONE) return c.ncat == 1 || c.ncat == 0 ? DSX_OK : DSX_ERR_SHAPE;
+    if (c.kind < DSX_CAT_CODES || c.kind > DSX_CAT_BINNING) return DSX_ERR_MODE;
+    if (!c.col) return DSX_ERR_NULL;
+    if (c.ncat < 1) return DSX_ERR_SHAPE;
+    if (c.dtype < DSX_F32 || c.dtype > DSX_I64) return DSX_ERR_DTYPE;
+    if (c.kind != DSX_CAT_BINNING && (c.dtype == DSX_F32 || c.dtype == DSX_F64))
+        return DSX_ERR_DTYPE;
+    if (c.kind == DSX_CAT_MODULO && c.modulo == 0) return DSX_ERR_SHAPE;
+    if (c.kind == DSX_CAT_BINNING && (c.nbins < 1 || c.binsize == 0.0)) return DSX_ERR_SHAPE;
+    return DSX_OK;
+}
+
+// Order-preserving unsigned keys of IEEE floats: key(a) < key(b) <=> a < b for non-NaN
+// a, b.  -0.0 orders before +0.0.  NaN payloads map to the two extremes, which are the
+// "empty" values of the min (all ones) and max (zero) planes; NaN is never inserted.
+__device__ __forceinline__ uint32_t fkey32(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unkey32(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(u);
+}
+__device__ __forceinline__ uint64_t fkey64(double d) {
+    uint64_t u = (uint64_t)__double_as_longlong(d);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double unkey64(uint64_t k) {
+    uint64_t u = (k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+
+// ---- exact float64 projection (glyphs/points.py:199-200, line.py:701-702) -----------
+// int(x*sx + tx): the product and the sum are rounded separately (no FMA), then the
+// result is truncated toward zero.  The bounds test ahead of it keeps the result within
+// [-1, W + 1], so a 32-bit conversion is exact.
+__device__ __forceinline__ int32_t proj(double x, double s, double t) {
+    return (int32_t)__dadd_rn(__dmul_rn(x, s), t);
+}
+
+// Axis mapper: identity (LinearAxis) or log10 (LogAxis, core.py:127-145).
+__device__ __forceinline__ double axis_map(double v, int32_t log) { return log ? log10(v) : v; }
+
+// ---- accumulation -------------------------------------------------------------------
+// Compensated float32 accumulation into (hi, lo): the hardware add returns the old hi,
+// so the rounding error of hi + v is recovered exactly (Knuth's two-sum) and added to
+// lo.  hi + lo then equals the exact sum up to the rounding of lo's own additions.
+__device__ __forceinline__ void add_comp(float *hi, float *lo, float v) {
+    const float old = unsafeAtomicAdd(hi, v);
+    const float s = old + v;
+    const float bp = s - old;
+    const float err = (old - (s - bp)) + (v - bp);
+    if (err != 0.0f) unsafeAtomicAdd(lo, err);
+}
+
+struct Planes {
+    uint32_t *count, *countv;
+    float *sum_hi, *sum_lo;
+    double *sum64;
+    uint32_t *min32, *max32;
+    uint64_t *min64, *max64;
+};
+
+// Apply every requested base to cell `cell` for a value v (TV = float / double), or for
+// a row with no value column (HAS_VAL = false).
+template <typename TV, bool HAS_VAL>
+__device__ __forceinline__ void append(const Planes &p, int64_t cell, TV v) {
+    if (p.count) atomicAdd(p.count + cell, 1u);
+    if constexpr (HAS_VAL) {
+        if (v != v) return;  // NaN: skipped by every value-reading base
+        if (p.countv) atomicAdd(p.countv + cell, 1u);
+        if constexpr (sizeof(TV) == 4) {
+            if (p.sum_hi) add_comp(p.sum_hi + cell, p.sum_lo + cell, (float)v);
+            if (p.min32) atomicMin(p.min32 + cell, fkey32((float)v));
+            if (p.max32) atomicMax(p.max32 + cell, fkey32((float)v));
+        } else {
+            if (p.sum64) unsafeAtomicAdd(p.sum64 + cell, (double)v);
+            if (p.min64) atomicMin((unsigned long long *)(p.min64 + cell),
+                                   (unsigned long long)fkey64((double)v));
+            if (p.max64) atomicMax((unsigned long long *)(p.max64 + cell),
+                                   (unsigned long long)fkey64((double)v));
+        }
+    }
+}
+
+inline Planes planes_of(const dsx_accums *a) {
+    Planes p;
+    p.count = a->count; p.countv = a->countv;
+    p.sum_hi = a->sum_hi; p.sum_lo = a->sum_lo; p.sum64 = a->sum64;
+    p.min32 = a->min32; p.max32 = a->max32; p.min64 = a->min64; p.max64 = a->max64;
+    return p;
+}
+
+// Number of workgroups for a grid-stride launch over n items.
+int grid_for(int64_t n, int block, int per_cu = 8);
+
+}  // namespace dsx

@@ -1,0 +1,228 @@
+// dsx_misc.hip -- autorange bounds, plane fill, finalize, status strings.
+#include <math.h>
+#include <string.h>
+
+#include "dsx_common.h"
+
+namespace dsx {
+
+static thread_local char g_last_hip[256] = "";
+
+void record_hip_error(hipError_t e) {
+    strncpy(g_last_hip, hipGetErrorString(e), sizeof(g_last_hip) - 1);
+    g_last_hip[sizeof(g_last_hip) - 1] = 0;
+}
+
+int launch_status() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        record_hip_error(e);
+        return DSX_ERR_HIP;
+    }
+    return DSX_OK;
+}
+
+int grid_for(int64_t n, int block, int per_cu) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            c > 0)
+            cus = c;
+    }
+    const int64_t want = (n + block - 1) / block;
+    const int64_t cap = (int64_t)cus * per_cu;
+    return (int)(want < 1 ? 1 : (want < cap ? want : cap));
+}
+
+// ---- bounds: glyph.py:66-78 ------------------------------------------------------------
+constexpr int kBoundsBlock = 256;
+
+__device__ __forceinline__ void minmax_reduce_block(double &lo, double &hi, double *sh) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double olo = __shfl_xor(lo, off, 64);
+        const double ohi = __shfl_xor(hi, off, 64);
+        lo = olo < lo ? olo : lo;
+        hi = ohi > hi ? ohi : hi;
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        sh[2 * wave] = lo;
+        sh[2 * wave + 1] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            lo = sh[2 * w] < lo ? sh[2 * w] : lo;
+            hi = sh[2 * w + 1] > hi ? sh[2 * w + 1] : hi;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBoundsBlock) void bounds_partial(const T *a, int64_t n,
+                                                              double *part) {
+    __shared__ double sh[2 * (kBoundsBlock / 64)];
+    T lo = (T)INFINITY, hi = (T)-INFINITY;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const T x = a[i];
+        if (x == x) {
+            lo = x < lo ? x : lo;
+            hi = x > hi ? x : hi;
+        }
+    }
+    double dlo = (double)lo, dhi = (double)hi;
+    minmax_reduce_block(dlo, dhi, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = dlo;
+        part[2 * blockIdx.x + 1] = dhi;
+    }
+}
+
+__global__ __launch_bounds__(kBoundsBlock) void bounds_final(const double *part, int nblk,
+                                                            double *out2) {
+    __shared__ double sh[2 * (kBoundsBlock / 64)];
+    double lo = INFINITY, hi = -INFINITY;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        lo = part[2 * b] < lo ? part[2 * b] : lo;
+        hi = part[2 * b + 1] > hi ? part[2 * b + 1] : hi;
+    }
+    minmax_reduce_block(lo, hi, sh);
+    if (threadIdx.x == 0) {
+        out2[0] = lo;
+        out2[1] = hi;
+    }
+}
+
+// ---- fill ---------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void fill_kernel(T *p, T value, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = value;
+}
+
+// ---- finalize -----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void finalize_kernel(dsx_finalize_desc f) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const bool f32 = f.val_dtype == DSX_F32;
+    double *out = reinterpret_cast<double *>(f.out);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.ncells; i += stride) {
+        switch (f.mode) {
+        case DSX_FIN_ANY:
+            reinterpret_cast<uint8_t *>(f.out)[i] = f.count[i] > 0u;
+            break;
+        case DSX_FIN_MIN:
+        case DSX_FIN_MAX: {
+            double r;
+            if (f32) {
+                const uint32_t k = reinterpret_cast<const uint32_t *>(f.key)[i];
+                const bool empty = f.mode == DSX_FIN_MIN ? k == DSX_MIN_EMPTY32 : k == 0u;
+                r = empty ? (double)NAN : (double)unkey32(k);
+            } else {
+                const uint64_t k = reinterpret_cast<const uint64_t *>(f.key)[i];
+                const bool empty = f.mode == DSX_FIN_MIN ? k == DSX_MIN_EMPTY64 : k == 0ull;
+                r = empty ? (double)NAN : unkey64(k);
+            }
+            out[i] = r;
+            break;
+        }
+        default: {
+            const uint32_t c = f.countv[i];
+            const double s = f32 ? (double)f.hi[i] + (double)f.lo[i] : f.v64[i];
+            double r;
+            switch (f.mode) {
+            case DSX_FIN_SUM: r = c > 0u ? s : (double)NAN; break;
+            case DSX_FIN_SUMZERO: r = s; break;
+            case DSX_FIN_MEAN: r = c > 0u ? s / (double)c : (double)NAN; break;
+            case DSX_FIN_VAR: r = c > 0u ? s / (double)c : (double)NAN; break;
+            default: r = c > 0u ? sqrt(s / (double)c) : (double)NAN; break;
+            }
+            out[i] = r;
+        }
+        }
+    }
+}
+
+}  // namespace dsx
+
+using namespace dsx;
+
+extern "C" size_t dsx_bounds_workspace_size(int64_t n) {
+    return (size_t)grid_for(n < 1 ? 1 : n, kBoundsBlock, 4) * 2 * sizeof(double);
+}
+
+extern "C" int dsx_bounds(const void *col, int32_t dtype, int64_t n, double *out2,
+                          void *workspace, size_t workspace_bytes, void *stream) {
+    if (!out2 || (n > 0 && !col)) return DSX_ERR_NULL;
+    if (n < 0) return DSX_ERR_SHAPE;
+    if (dtype != DSX_F32 && dtype != DSX_F64) return DSX_ERR_DTYPE;
+    const int grid = grid_for(n < 1 ? 1 : n, kBoundsBlock, 4);
+    if (!workspace || workspace_bytes < (size_t)grid * 2 * sizeof(double)) return DSX_ERR_WORKSPACE;
+    hipStream_t s = as_stream(stream);
+    double *part = reinterpret_cast<double *>(workspace);
+    if (dtype == DSX_F32)
+        bounds_partial<float><<<grid, kBoundsBlock, 0, s>>>((const float *)col, n, part);
+    else
+        bounds_partial<double><<<grid, kBoundsBlock, 0, s>>>((const double *)col, n, part);
+    int st = launch_status();
+    if (st) return st;
+    bounds_final<<<1, kBoundsBlock, 0, s>>>(part, grid, out2);
+    return launch_status();
+}
+
+extern "C" int dsx_fill(void *plane, int32_t mode, int64_t ncells, void *stream) {
+    if (ncells < 0) return DSX_ERR_SHAPE;
+    if (ncells == 0) return DSX_OK;
+    if (!plane) return DSX_ERR_NULL;
+    const int grid = grid_for(ncells, 256, 4);
+    hipStream_t s = as_stream(stream);
+    switch (mode) {
+    case DSX_FILL_ZERO32: fill_kernel<uint32_t><<<grid, 256, 0, s>>>((uint32_t *)plane, 0u, ncells); break;
+    case DSX_FILL_ONES32: fill_kernel<uint32_t><<<grid, 256, 0, s>>>((uint32_t *)plane, 0xFFFFFFFFu, ncells); break;
+    case DSX_FILL_ZERO64: fill_kernel<uint64_t><<<grid, 256, 0, s>>>((uint64_t *)plane, 0ull, ncells); break;
+    case DSX_FILL_ONES64: fill_kernel<uint64_t><<<grid, 256, 0, s>>>((uint64_t *)plane, ~0ull, ncells); break;
+    default: return DSX_ERR_MODE;
+    }
+    return launch_status();
+}
+
+extern "C" int dsx_finalize(const dsx_finalize_desc *f, void *stream) {
+    if (!f || !f->out) return DSX_ERR_NULL;
+    if (f->ncells < 0) return DSX_ERR_SHAPE;
+    if (f->mode < DSX_FIN_ANY || f->mode > DSX_FIN_SUMZERO) return DSX_ERR_MODE;
+    if (f->mode == DSX_FIN_ANY) {
+        if (!f->count) return DSX_ERR_NULL;
+    } else if (f->mode == DSX_FIN_MIN || f->mode == DSX_FIN_MAX) {
+        if (!f->key) return DSX_ERR_NULL;
+        if (f->val_dtype != DSX_F32 && f->val_dtype != DSX_F64) return DSX_ERR_DTYPE;
+    } else {
+        if (f->val_dtype != DSX_F32 && f->val_dtype != DSX_F64) return DSX_ERR_DTYPE;
+        if (!f->countv) return DSX_ERR_NULL;
+        if (f->val_dtype == DSX_F32 && (!f->hi || !f->lo)) return DSX_ERR_NULL;
+        if (f->val_dtype == DSX_F64 && !f->v64) return DSX_ERR_NULL;
+    }
+    if (f->ncells == 0) return DSX_OK;
+    const int grid = grid_for(f->ncells, 256, 4);
+    finalize_kernel<<<grid, 256, 0, as_stream(stream)>>>(*f);
+    return launch_status();
+}
+
+extern "C" int dsx_abi_version(void) { return DSX_ABI_VERSION; }
+
+extern "C" const char *dsx_strerror(int status) {
+    switch (status) {
+    case DSX_OK: return "ok";
+    case DSX_ERR_NULL: return "a required pointer is NULL";
+    case DSX_ERR_DTYPE: return "unsupported dtype";
+    case DSX_ERR_SHAPE: return "invalid shape, length or stride";
+    case DSX_ERR_WORKSPACE: return "workspace missing or too small";
+    case DSX_ERR_HIP: return "HIP runtime error";
+    case DSX_ERR_ACCUM: return "accumulator planes inconsistent with the value column";
+    case DSX_ERR_MODE: return "unknown mode";
+    default: return "unknown status";
+    }
+}
+
+extern "C" const char *dsx_last_hip_error(void) { return g_last_hip; }

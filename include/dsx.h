@@ -1,0 +1,216 @@
+/*
+ * dsx.h -- C ABI of libdsx.so, the MI355X (gfx950) points / line aggregation library.
+ *
+ * Every pointer argument named "device" is a caller-owned device allocation (the Python
+ * host passes torch-ROCm tensor data pointers); the library never allocates or frees
+ * caller memory.  Every call is asynchronous on the given HIP stream (a hipStream_t
+ * passed as void*; NULL = the default stream) unless stated otherwise, returns DSX_OK
+ * (0) or a negative dsx_status, and never throws.  The library keeps no mutable global
+ * state: calls on disjoint buffers are reentrant, and calls that share accumulator
+ * planes on one stream are ordered by that stream.
+ *
+ * Correspondence with the reference (datashader 0.19.1) -- this ABI replaces the numba
+ * kernels that the reference's backends call:
+ *   dsx_points_agg     glyphs/points.py:188-221 (_perform_extend_points / extend_cpu /
+ *                      extend_cuda) with the appends that compiler.py:321-475 generates
+ *                      from reductions.py (count 554-584, _sum_zero 958-962, sum
+ *                      1056-1063, min 1178-1182, max 1223-1227, any 842-861, by 789-798)
+ *   dsx_points_m2      the m2 append of reductions.py:1128-1137 (second pass, see below)
+ *   dsx_lines_agg      glyphs/line.py:1033-1097 draw_segment (+ _liang_barsky 734-800,
+ *                      map_onto_pixel_snap 688-720, _bresenham 986-1031) driven by the
+ *                      extend loops of line.py:1099-1136 (axis 0), 1152-1193 (axis 0,
+ *                      several columns), 1244-1290 (axis 1), 1340-1386 (axis 1, x
+ *                      constant) and the y-constant variant of the same builder
+ *   dsx_bounds         glyphs/glyph.py:66-78 (_compute_bounds_numba)
+ *   dsx_finalize       reductions.py:491-493, 1092-1096, 1294-1297, 1312-1317, 1332-1337
+ *                      (OptionalFieldReduction / sum / mean / var / std finalizers)
+ *   dsx_fill           reductions.py:446-472 (the create functions' initial values)
+ */
+#ifndef DSX_H
+#define DSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSX_ABI_VERSION 1
+
+/* ---- status codes --------------------------------------------------------------- */
+typedef enum {
+    DSX_OK = 0,
+    DSX_ERR_NULL = -1,        /* a required pointer is NULL */
+    DSX_ERR_DTYPE = -2,       /* unsupported dtype code */
+    DSX_ERR_SHAPE = -3,       /* non-positive canvas, negative length, bad stride */
+    DSX_ERR_WORKSPACE = -4,   /* workspace missing or smaller than dsx_*_workspace_size */
+    DSX_ERR_HIP = -5,         /* a HIP runtime call failed (launch or memory) */
+    DSX_ERR_ACCUM = -6,       /* accumulator set inconsistent with the value column */
+    DSX_ERR_MODE = -7         /* unknown finalize / fill mode */
+} dsx_status;
+
+/* ---- dtype codes ---------------------------------------------------------------- */
+typedef enum {
+    DSX_F32 = 1, DSX_F64 = 2,
+    DSX_I8 = 3, DSX_U8 = 4, DSX_I16 = 5, DSX_I32 = 6, DSX_I64 = 7
+} dsx_dtype;
+
+/* ---- accumulator planes --------------------------------------------------------- *
+ * Each non-NULL plane is a dense device array of H*W*ncat cells, cell index
+ * (y*W + x)*ncat + category (the reference's agg[y, x, cat] layout, C order).
+ * Initial values (dsx_fill): counts 0, sums 0, min keys DSX_MIN_EMPTY*, max keys 0.
+ *
+ * Float accumulation: a float32 value column accumulates its sum (and the squared
+ * deviations of dsx_points_m2) in float32 with an exact two-sum compensation term:
+ * sum = (double)hi + (double)lo.  A float64 value column accumulates in float64.
+ * min / max are kept as order-preserving unsigned keys of the value's own width
+ * (key(v) = bits(v) ^ (sign ? all-ones : sign-bit)), so min/max are exact.          */
+#define DSX_MIN_EMPTY32 0xFFFFFFFFu
+#define DSX_MIN_EMPTY64 0xFFFFFFFFFFFFFFFFull
+
+typedef struct {
+    uint32_t *count;      /* count()                      (rows in range)             */
+    uint32_t *countv;     /* count(v): rows whose value is not NaN                      */
+    float *sum_hi;        /* float32 value column: compensated sum, high part          */
+    float *sum_lo;        /*                                        low (error) part   */
+    double *sum64;        /* float64 value column: sum                                 */
+    uint32_t *min32;      /* float32 value column: min key                             */
+    uint32_t *max32;      /*                        max key                            */
+    uint64_t *min64;      /* float64 value column: min key                             */
+    uint64_t *max64;      /*                        max key                            */
+} dsx_accums;
+
+/* ---- view: the extend() arguments vt + bounds (points.py:223, line.py:362) --------- */
+typedef struct {
+    double sx, tx, sy, ty;        /* axis scale / translate (core.py:62-81)               */
+    double xmin, xmax, ymin, ymax;/* inclusive data-space bounds                          */
+    int64_t xxmax, yymax;         /* lines: round(m(xmax)*sx+tx), round(m(ymax)*sy+ty)    */
+    int32_t width, height;        /* canvas W, H                                          */
+    int32_t xlog, ylog;           /* 1: the axis maps v -> log10(v) before scaling        */
+} dsx_view;                       /*    (LogAxis, core.py:127-145); bounds stay raw       */
+
+/* ---- points --------------------------------------------------------------------- */
+/* ---- categorizer: by(...)'s category of a row (reductions.py:143-259) ------------- *
+ * kind DSX_CAT_NONE    no categories (ncat must be 1)
+ *      DSX_CAT_CODES   category = col[i] (pandas category codes; -1 wraps to ncat-1)
+ *      DSX_CAT_MODULO  category = (col[i] - offset) mod modulo (Python sign rules)
+ *      DSX_CAT_BINNING category = trunc((col[i] - bin0) / binsize), computed in float32
+ *                      for a float32 column and float64 otherwise; < 0 -> bin_under,
+ *                      >= nbins -> bin_over, NaN -> nbins
+ * A category outside [0, ncat) after the wrap drops the row.                         */
+typedef enum { DSX_CAT_NONE = 0, DSX_CAT_CODES = 1, DSX_CAT_MODULO = 2, DSX_CAT_BINNING = 3 }
+    dsx_cat_kind;
+
+typedef struct {
+    int32_t kind;                 /* dsx_cat_kind                                         */
+    int32_t dtype;                /* dtype of col                                         */
+    const void *col;              /* device column (one element per row / line)           */
+    int32_t ncat;                 /* categories (1 for DSX_CAT_NONE)                      */
+    int32_t nbins, bin_under, bin_over;
+    int64_t offset, modulo;
+    double bin0, binsize;
+} dsx_categorizer;
+
+typedef struct {
+    int64_t n;                    /* rows                                                 */
+    const void *x, *y;            /* device, n elements each                              */
+    int32_t xy_dtype;             /* DSX_F32 | DSX_F64 (x and y share it)                 */
+    int32_t val_dtype;            /* DSX_F32 | DSX_F64, 0 when val == NULL                */
+    const void *val;              /* device value column or NULL                          */
+    dsx_categorizer cat;
+} dsx_points_desc;
+
+/* Project rows to bins and update every non-NULL plane of acc (one pass over the
+ * columns).  A row is dropped unless xmin <= x <= xmax and ymin <= y <= ymax (NaN
+ * fails); its bin is trunc(x*sx + tx) (float64, two separate roundings), clamped to
+ * W-1 at the upper edge.  Planes that need the value column (countv, sums, min/max)
+ * require val. */
+int dsx_points_agg(const dsx_points_desc *d, const dsx_view *v, const dsx_accums *acc,
+                   void *stream);
+
+/* Second pass for var / std: for every row that dsx_points_agg counted in countv,
+ * accumulate (v - mean[bin])^2 into m2 (float32 value column: m2_hi/m2_lo compensated;
+ * float64: m2_64).  mean is a device float64 plane of H*W*ncat cells.  The reference's
+ * streaming update (reductions.py:1128-1137) computes the same sum of squared
+ * deviations; the two-pass form is its exact algebraic equivalent. */
+int dsx_points_m2(const dsx_points_desc *d, const dsx_view *v, const double *mean,
+                  float *m2_hi, float *m2_lo, double *m2_64, void *stream);
+
+/* ---- lines ---------------------------------------------------------------------- *
+ * nlines lines of nverts vertices each.  Vertex k of line l is
+ *   x = X[l*x_line_stride + k*x_vert_stride],  y = Y[l*y_line_stride + k*y_vert_stride]
+ * (element strides; a stride of 0 broadcasts), which expresses every layout of the
+ * reference: axis 0 one line (1 line, vertex stride 1), axis 0 several columns, axis 1
+ * (row-major or column-major), axis 1 with constant x or constant y.
+ * Segment (l, k) -> (l, k+1) starts a new run when k == 0 (then segment_start =
+ * plot_start for axis 0 and 1 for axis 1) or when vertex k-1 has a NaN coordinate.
+ * The value / category index of a segment is k for axis 0 and l for axis 1.          */
+typedef struct {
+    int64_t nlines, nverts;
+    const void *x, *y;
+    int32_t x_dtype, y_dtype;     /* DSX_F32 | DSX_F64 each                               */
+    int64_t x_line_stride, x_vert_stride, y_line_stride, y_vert_stride;
+    int32_t axis;                 /* 0 or 1                                               */
+    int32_t plot_start;           /* axis 0: draw the first vertex                        */
+    const void *val; int32_t val_dtype;
+    dsx_categorizer cat;          /* indexed like val                                     */
+} dsx_lines_desc;
+
+int dsx_lines_agg(const dsx_lines_desc *d, const dsx_view *v, const dsx_accums *acc,
+                  void *stream);
+
+/* ---- bounds (autorange) ----------------------------------------------------------- *
+ * NaN-skipping min and max of n elements of a device column; writes {min, max} as two
+ * doubles to the device pointer out2 (+inf/-inf when every element is NaN).  Needs a
+ * device workspace of dsx_bounds_workspace_size(n) bytes. */
+size_t dsx_bounds_workspace_size(int64_t n);
+int dsx_bounds(const void *col, int32_t dtype, int64_t n, double *out2, void *workspace,
+               size_t workspace_bytes, void *stream);
+
+/* ---- fill / finalize -------------------------------------------------------------- */
+typedef enum {
+    DSX_FILL_ZERO32 = 1,      /* u32 / f32 planes := 0                                    */
+    DSX_FILL_ONES32 = 2,      /* u32 plane := 0xFFFFFFFF (min key, empty)                 */
+    DSX_FILL_ZERO64 = 3,      /* u64 / f64 planes := 0                                    */
+    DSX_FILL_ONES64 = 4       /* u64 plane := all ones (min key, empty)                   */
+} dsx_fill_mode;
+
+/* Fill ncells elements with a kernel (no hipMemset, so it is safe under graph capture). */
+int dsx_fill(void *plane, int32_t mode, int64_t ncells, void *stream);
+
+typedef enum {
+    DSX_FIN_ANY = 1,          /* out u8  = count > 0                                      */
+    DSX_FIN_SUM = 2,          /* out f64 = countv > 0 ? sum : NaN                         */
+    DSX_FIN_MEAN = 3,         /* out f64 = countv > 0 ? sum / countv : NaN                */
+    DSX_FIN_VAR = 4,          /* out f64 = countv > 0 ? m2 / countv : NaN                 */
+    DSX_FIN_STD = 5,          /* out f64 = countv > 0 ? sqrt(m2 / countv) : NaN           */
+    DSX_FIN_MIN = 6,          /* out f64 = decoded min key, NaN when empty                */
+    DSX_FIN_MAX = 7,          /* out f64 = decoded max key, NaN when empty                */
+    DSX_FIN_SUMZERO = 8       /* out f64 = sum (0 when empty)                             */
+} dsx_finalize_mode;
+
+typedef struct {
+    int32_t mode;             /* dsx_finalize_mode                                        */
+    int32_t val_dtype;        /* DSX_F32 | DSX_F64: which sum / key planes are read       */
+    int64_t ncells;
+    const uint32_t *count;    /* DSX_FIN_ANY: count or countv                             */
+    const uint32_t *countv;
+    const float *hi, *lo;     /* f32: sum or m2 (hi + lo)                                 */
+    const double *v64;        /* f64: sum or m2                                           */
+    const void *key;          /* min / max key plane                                      */
+    void *out;                /* device output (u8 or f64)                                */
+} dsx_finalize_desc;
+
+int dsx_finalize(const dsx_finalize_desc *f, void *stream);
+
+/* ---- misc --------------------------------------------------------------------------- */
+int dsx_abi_version(void);
+const char *dsx_strerror(int status);
+/* Last HIP error string recorded by a call that returned DSX_ERR_HIP on this thread. */
+const char *dsx_last_hip_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSX_H */

@@ -1,0 +1,259 @@
+"""Helpers shared by the tests: load the reference's golden cases and evaluate them with
+the CPU oracle (the oracle is the checker; nothing here is product code)."""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+import pandas as pd
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_CASES = None
+
+
+def cases():
+    global _CASES
+    if _CASES is None:
+        with open(os.path.join(HERE, "golden", "reference_cases.json")) as f:
+            _CASES = json.load(f)
+    return _CASES
+
+
+def dec(a, dtype=float):
+    """Inverse of the generator's encoder: None -> NaN."""
+    def fix(v):
+        if isinstance(v, list):
+            return [fix(u) for u in v]
+        return np.nan if v is None else v
+    return np.array(fix(a), dtype=dtype)
+
+
+def frame(name):
+    """The named golden frame as a pandas DataFrame (categoricals as ``category``)."""
+    spec = cases()["frames"][name]
+    cols = {}
+    for k, c in spec.items():
+        if c["dtype"] == "category":
+            cols[k] = pd.Categorical(c["data"])
+        else:
+            cols[k] = dec(c["data"], float).astype(c["dtype"])
+    return pd.DataFrame(cols)
+
+
+def line_frame(spec):
+    cols = {k: dec(v, float) for k, v in spec.items()}
+    return pd.DataFrame(cols)
+
+
+def scale_translate(rng, n):
+    """``Axis.compute_scale_and_translate`` for a linear axis (core.py:62-81)."""
+    start, end = rng
+    s = n / (end - start)
+    t = -start * s
+    return s, t
+
+
+def maybe_expand(bounds):
+    """``Glyph.maybe_expand_bounds`` (glyph.py:42-49)."""
+    lo, hi = bounds
+    if not (np.isfinite(lo) and np.isfinite(hi)):
+        return -1.0, 1.0
+    if lo == hi:
+        return lo - 1, lo + 1
+    return lo, hi
+
+
+def view_for(W, H, x_range, y_range, xf32=False, yf32=False):
+    sx, tx = scale_translate(x_range, W)
+    sy, ty = scale_translate(y_range, H)
+    return oracle.View(sx, tx, sy, ty, x_range[0], x_range[1], y_range[0], y_range[1],
+                       xf32=xf32, yf32=yf32)
+
+
+# ---- evaluating a reduction spec with the oracle ------------------------------------
+
+def _cat_codes(df, cat):
+    """Category codes and categories for ``by``'s categorizer (reductions.py:143-259)."""
+    if isinstance(cat, str):
+        c = df[cat]
+        return np.asarray(c.cat.codes, np.int64), list(c.cat.categories)
+    if cat["kind"] == "category_modulo":
+        v = (df[cat["column"]] - cat.get("offset", 0)) % cat["modulo"]
+        return np.asarray(v, np.int64), list(range(cat["modulo"]))
+    if cat["kind"] == "category_binning":
+        nb = cat["nbins"]
+        bin0 = cat["lower"]
+        binsize = (cat["upper"] - cat["lower"]) / float(nb)
+        values = df[cat["column"]].to_numpy()
+        nanv = np.isnan(values)
+        idx = (values - bin0) / binsize
+        idx[nanv] = 0
+        idx = idx.astype(int)
+        idx[idx < 0] = 0
+        idx[idx >= nb] = nb - 1
+        idx[nanv] = nb
+        return idx.astype(np.int64), list(range(nb + 1))
+    raise ValueError(cat)
+
+
+def _leaf(spec, run):
+    """Finalised array of one non-summary reduction, given ``run(bases, val, cat, ncat)``."""
+    kind, col = spec["kind"], spec.get("column")
+    if kind == "count_cat":
+        spec = {"kind": "by", "cat": col, "reduction": {"kind": "count", "column": None}}
+        kind = "by"
+    if kind == "by":
+        codes, cats = run.cat_codes(spec["cat"])
+        sub = spec["reduction"]
+        return _leaf_inner(sub, run, codes, len(cats))
+    return _leaf_inner(spec, run, None, 1)
+
+
+def _leaf_inner(spec, run, codes, ncat):
+    kind, col = spec["kind"], spec.get("column")
+    val = None if col is None else run.column(col)
+    squeeze = codes is None
+    if kind == "count":
+        r = run(("count",) if col is None else ("countv",), val, codes, ncat, squeeze)
+        return r["count" if col is None else "countv"]
+    if kind == "any":
+        r = run(("any",) if col is None else ("anyv",), val, codes, ncat, squeeze)
+        return r["any" if col is None else "anyv"]
+    if kind in ("sum", "min", "max"):
+        return run((kind,), val, codes, ncat, squeeze)[kind]
+    if kind == "mean":
+        r = run(("sumz", "countv"), val, codes, ncat, squeeze)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return np.where(r["countv"] > 0, r["sumz"] / r["countv"], np.nan)
+    if kind in ("var", "std"):
+        r = run(("countv", "m2"), val, codes, ncat, squeeze)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            v = np.where(r["countv"] > 0, r["m2"] / r["countv"], np.nan)
+        return np.sqrt(v) if kind == "std" else v
+    raise ValueError(kind)
+
+
+def evaluate(spec, run):
+    if spec["kind"] == "summary":
+        return {k: _leaf(v, run) for k, v in spec["items"].items()}
+    return _leaf(spec, run)
+
+
+class PointRun:
+    def __init__(self, df, x, y, view, W, H):
+        self.df, self.x, self.y, self.view, self.W, self.H = df, x, y, view, W, H
+
+    def column(self, name):
+        return self.df[name].to_numpy(dtype=np.float64)
+
+    def cat_codes(self, cat):
+        return _cat_codes(self.df, cat)
+
+    def __call__(self, bases, val, codes, ncat, squeeze):
+        return oracle.points(self.df[self.x].to_numpy(np.float64),
+                             self.df[self.y].to_numpy(np.float64), self.view, self.W, self.H,
+                             bases, val=val, cat=codes, ncat=ncat, squeeze=squeeze)
+
+
+def point_case_oracle(case):
+    """Evaluate one golden points case with the oracle (autorange as the reference)."""
+    df = frame(case["frame"])
+    cv = case["canvas"]
+    W, H = cv["W"], cv["H"]
+    xr_ = cv["x_range"] or maybe_expand(oracle.bounds(df[case["x"]].to_numpy(np.float64)))
+    yr_ = cv["y_range"] or maybe_expand(oracle.bounds(df[case["y"]].to_numpy(np.float64)))
+    view = view_for(W, H, xr_, yr_)
+    res = evaluate(case["agg"], PointRun(df, case["x"], case["y"], view, W, H))
+    if isinstance(res, dict):
+        res = res[case["pick"]]
+    return res, (xr_, yr_)
+
+
+def expected_array(case):
+    exp = case["expected"]
+    return dec(exp, float)
+
+
+def assert_same(got, exp):
+    """Exact equality with NaN == NaN (xarray.testing.assert_equal semantics)."""
+    got = np.asarray(got, dtype=float)
+    exp = np.asarray(exp, dtype=float)
+    assert got.shape == exp.shape, (got.shape, exp.shape)
+    both_nan = np.isnan(got) & np.isnan(exp)
+    bad = ~(both_nan | (got == exp))
+    assert not bad.any(), f"mismatch at {np.argwhere(bad)[:5].tolist()}: got {got[bad][:5]}, exp {exp[bad][:5]}"
+
+
+# ---- lines ---------------------------------------------------------------------------
+
+def line_polylines(case_kwargs, df):
+    """Expand a Canvas.line call of the golden cases into (xs_2d, ys_2d, layout) where
+    row k of xs_2d/ys_2d is one line.  ``layout`` is 'axis0' (value index = vertex) or
+    'axis1' (value index = line)."""
+    x, y, axis = case_kwargs["x"], case_kwargs["y"], case_kwargs.get("axis", 0)
+    if isinstance(x, list) and isinstance(y, str):
+        y = [y] * len(x)
+    if isinstance(y, list) and isinstance(x, str):
+        x = [x] * len(y)
+    if axis == 0:
+        if isinstance(x, str):
+            return [df[x].to_numpy(float)], [df[y].to_numpy(float)], "axis0"
+        return ([df[c].to_numpy(float) for c in x], [df[c].to_numpy(float) for c in y], "axis0")
+    if isinstance(x, dict):
+        xs = np.asarray(x["ndarray"], float)
+        ys = df[list(y)].to_numpy(float)
+        return [xs] * ys.shape[0], list(ys), "axis1"
+    if isinstance(y, dict):
+        ys = np.asarray(y["ndarray"], float)
+        xs = df[list(x)].to_numpy(float)
+        return list(xs), [ys] * xs.shape[0], "axis1"
+    return list(df[list(x)].to_numpy(float)), list(df[list(y)].to_numpy(float)), "axis1"
+
+
+def line_case_oracle(case):
+    df = line_frame(case["frame"])
+    kw = case["kwargs"]
+    xs, ys, layout = line_polylines(kw, df)
+    cv = case["canvas"]
+    W, H = cv["W"], cv["H"]
+
+    def bnd(arrs):
+        mins, maxs = zip(*[oracle.bounds(a) for a in arrs])
+        return maybe_expand((min(mins), max(maxs)))
+
+    xr_ = cv["x_range"] or (maybe_expand((np.nanmin(kw["x"]["ndarray"]),
+                                         np.nanmax(kw["x"]["ndarray"])))
+                            if isinstance(kw["x"], dict) else bnd(xs))
+    yr_ = cv["y_range"] or (maybe_expand((np.nanmin(kw["y"]["ndarray"]),
+                                         np.nanmax(kw["y"]["ndarray"])))
+                            if isinstance(kw["y"], dict) else bnd(ys))
+    view = view_for(W, H, xr_, yr_)
+    agg = kw["agg"]
+    col = agg.get("column")
+    kind = agg["kind"]
+    base = {"count": "count" if col is None else "countv", "sum": "sum", "max": "max",
+            "min": "min", "any": "any" if col is None else "anyv"}[kind]
+    val = None if col is None else df[col].to_numpy(float)
+    if layout == "axis0":
+        total = None
+        for xl, yl in zip(xs, ys):
+            r = oracle.line_axis0(xl, yl, view, W, H, (base,), plot_start=True, val=val)[base]
+            total = r.astype(float) if total is None else _merge(kind, total, r)
+        return total, (xr_, yr_)
+    r = oracle.lines_axis1(np.array(xs), np.array(ys), view, W, H, (base,), val=val)[base]
+    return r, (xr_, yr_)
+
+
+def _merge(kind, a, b):
+    b = b.astype(float)
+    if kind in ("count", "any"):
+        return a + b
+    if kind == "sum":
+        return np.where(np.isnan(a), b, np.where(np.isnan(b), a, a + b))
+    if kind == "max":
+        return np.fmax(a, b)
+    return np.fmin(a, b)

@@ -1,0 +1,149 @@
+"""The C ABI: libdsx.so loads, exports every function include/dsx.h declares, the ctypes
+structures match the header's layout, and descriptor validation fails with the
+documented status codes before any kernel is launched.  CPU only."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from datashader_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dsx.h")
+
+
+def header_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(dsx_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_function():
+    lib = _abi.load()
+    declared = header_functions()
+    assert set(declared) == set(_abi.EXPORTS)
+    out = subprocess.run(["nm", "-D", "--defined-only", _abi.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (dsx_[a-z0-9_]+)", out))
+    missing = set(declared) - exported
+    assert not missing, missing
+    for f in declared:
+        assert hasattr(lib, f)
+    assert lib.dsx_abi_version() == _abi.ABI_VERSION
+
+
+STRUCTS = {
+    "dsx_accums": _abi.Accums, "dsx_view": _abi.View, "dsx_categorizer": _abi.Categorizer,
+    "dsx_points_desc": _abi.PointsDesc, "dsx_lines_desc": _abi.LinesDesc,
+    "dsx_finalize_desc": _abi.FinalizeDesc,
+}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"',
+             "int main(void) {"]
+    for cname, py in STRUCTS.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(c)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    got = {}
+    for line in out.splitlines():
+        s, f, v = line.split()
+        got[(s, f)] = int(v)
+    for cname, py in STRUCTS.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_status_strings():
+    lib = _abi.load()
+    for code in range(-7, 1):
+        assert lib.dsx_strerror(code)
+    with pytest.raises(_abi.DsxError, match="unsupported dtype"):
+        _abi.check(-2, "x")
+
+
+def _view(W=4, H=4):
+    v = _abi.View()
+    v.sx, v.tx, v.sy, v.ty = 1.0, 0.0, 1.0, 0.0
+    v.xmin, v.xmax, v.ymin, v.ymax = 0.0, 4.0, 0.0, 4.0
+    v.width, v.height = W, H
+    return v
+
+
+def test_points_descriptor_validation():
+    """Invalid descriptors are rejected on the host, before any HIP call."""
+    lib = _abi.load()
+    acc = _abi.Accums()
+    d = _abi.PointsDesc()
+    d.n = 10
+    assert lib.dsx_points_agg(None, _view(), acc, None) == -1
+    assert lib.dsx_points_agg(d, _view(), acc, None) == -1          # x / y NULL
+    d.x = d.y = ctypes.c_void_p(0x1000)
+    d.xy_dtype = 7
+    assert lib.dsx_points_agg(d, _view(), acc, None) == -2          # coordinates must be float
+    d.xy_dtype = _abi.F32
+    assert lib.dsx_points_agg(d, _view(0, 4), acc, None) == -3      # empty canvas
+    d.n = -1
+    assert lib.dsx_points_agg(d, _view(), acc, None) == -3
+    d.n = 10
+    acc.countv = ctypes.c_void_p(0x2000)
+    assert lib.dsx_points_agg(d, _view(), acc, None) == -6          # countv without a value
+    acc = _abi.Accums()
+    acc.sum_hi = ctypes.c_void_p(0x2000)
+    d.val, d.val_dtype = ctypes.c_void_p(0x3000), _abi.F32
+    assert lib.dsx_points_agg(d, _view(), acc, None) == -6          # hi without lo
+    acc = _abi.Accums()
+    acc.sum64 = ctypes.c_void_p(0x2000)
+    assert lib.dsx_points_agg(d, _view(), acc, None) == -6          # f64 plane, f32 value
+    acc = _abi.Accums()
+    d.cat.kind = _abi.CAT_CODES
+    assert lib.dsx_points_agg(d, _view(), acc, None) == -1          # categorizer column NULL
+    d.cat.col, d.cat.dtype, d.cat.ncat = ctypes.c_void_p(0x4000), _abi.F32, 3
+    assert lib.dsx_points_agg(d, _view(), acc, None) == -2          # codes must be integers
+    d.cat.kind, d.cat.dtype, d.cat.modulo = _abi.CAT_MODULO, _abi.I64, 0
+    assert lib.dsx_points_agg(d, _view(), acc, None) == -3          # modulo 0
+    d.cat.kind = 9
+    assert lib.dsx_points_agg(d, _view(), acc, None) == -7
+    d.cat = _abi.Categorizer()
+    d.n = 0
+    assert lib.dsx_points_agg(d, _view(), acc, None) == 0           # nothing to do
+
+
+def test_lines_and_misc_validation():
+    lib = _abi.load()
+    acc = _abi.Accums()
+    d = _abi.LinesDesc()
+    d.nlines, d.nverts, d.x_dtype, d.y_dtype = 1, 3, _abi.F64, _abi.F64
+    d.axis = 2
+    assert lib.dsx_lines_agg(d, _view(), acc, None) == -3
+    d.axis = 0
+    d.x_dtype = 5
+    assert lib.dsx_lines_agg(d, _view(), acc, None) == -2
+    d.x_dtype = _abi.F64
+    assert lib.dsx_lines_agg(d, _view(), acc, None) == -1           # x / y NULL
+    d.nverts = 1
+    assert lib.dsx_lines_agg(d, _view(), acc, None) == 0            # no segment
+    assert lib.dsx_fill(None, _abi.FILL_ZERO32, 5, None) == -1
+    assert lib.dsx_fill(ctypes.c_void_p(0x1000), 99, 5, None) == -7
+    assert lib.dsx_fill(None, _abi.FILL_ZERO32, 0, None) == 0
+    f = _abi.FinalizeDesc()
+    f.mode, f.ncells, f.out = 42, 4, ctypes.c_void_p(0x1000)
+    assert lib.dsx_finalize(f, None) == -7
+    f.mode = _abi.FIN_MEAN
+    f.val_dtype = _abi.F32
+    assert lib.dsx_finalize(f, None) == -1                          # countv NULL
+    assert lib.dsx_bounds(None, _abi.F32, 4, ctypes.c_void_p(0x1000), None, 0, None) == -1
+    assert lib.dsx_bounds(ctypes.c_void_p(0x1000), 9, 4, ctypes.c_void_p(0x1000), None, 0,
+                          None) == -2

@@ -1,0 +1,185 @@
+"""GPU parity of the line path: libdsx's Bresenham rasteriser (through the C ABI and
+through Canvas.line) vs the reference's expectations, the touched-pixel masks of the
+reference's line images, and the CPU oracle.  Every line result is integer: bit-exact."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pandas as pd
+import pytest
+
+import oracle
+
+import golden_utils as G
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import datashader_amd as dsa  # noqa: E402
+from datashader_amd import _abi, _device  # noqa: E402
+
+GC = G.cases()
+
+
+def abi_lines(xs, ys, view, W, H, axis=0, plot_start=True, x_ls=0, x_vs=1, y_ls=0, y_vs=1,
+              nlines=1, nverts=None):
+    """One dsx_lines_agg call (count plane) on explicit device buffers."""
+    lib = _abi.load()
+    xt = torch.as_tensor(np.ascontiguousarray(xs, np.float64), device="cuda")
+    yt = torch.as_tensor(np.ascontiguousarray(ys, np.float64), device="cuda")
+    cnt = torch.zeros(H * W, dtype=torch.int32, device="cuda")
+    d = _abi.LinesDesc()
+    d.nlines, d.nverts = nlines, (len(xs) if nverts is None else nverts)
+    d.x, d.y = _device.ptr(xt), _device.ptr(yt)
+    d.x_dtype = d.y_dtype = _abi.F64
+    d.x_line_stride, d.x_vert_stride, d.y_line_stride, d.y_vert_stride = x_ls, x_vs, y_ls, y_vs
+    d.axis, d.plot_start = axis, int(plot_start)
+    v = _abi.View()
+    v.sx, v.tx, v.sy, v.ty = view.sx, view.tx, view.sy, view.ty
+    v.xmin, v.xmax, v.ymin, v.ymax = view.xmin, view.xmax, view.ymin, view.ymax
+    v.xxmax, v.yymax, v.width, v.height = view.xxmax, view.yymax, W, H
+    acc = _abi.Accums()
+    acc.count = _device.ptr(cnt)
+    _abi.check(lib.dsx_lines_agg(d, v, acc, _device.stream_handle()), "dsx_lines_agg")
+    return cnt.view(torch.uint32).cpu().numpy().reshape(H, W)
+
+
+@pytest.mark.parametrize("case", GC["glyph_lines"], ids=[c["name"] for c in GC["glyph_lines"]])
+def test_glyph_lines_golden_abi(case):
+    vv = case["view"]
+    view = oracle.View(vv["sx"], vv["tx"], vv["sy"], vv["ty"], vv["xmin"], vv["xmax"],
+                       vv["ymin"], vv["ymax"])
+    W, H = case["W"], case["H"]
+    if case["kind"] == "segment":
+        (x0, y0), (x1, y1) = case["p0"], case["p1"]
+        agg = abi_lines([x0, x1], [y0, y1], view, W, H, plot_start=case["segment_start"])
+    else:
+        agg = abi_lines(G.dec(case["xs"]), G.dec(case["ys"]), view, W, H,
+                        plot_start=case["plot_start"])
+    if case["expected"] is not None:
+        np.testing.assert_array_equal(agg, np.array(case["expected"]))
+    if case.get("total") is not None:
+        assert agg.sum() == case["total"]
+    if case.get("at"):
+        r, c_, val = case["at"]
+        assert agg[r, c_] == val
+
+
+def _line_kwargs(kw):
+    kw = dict(kw)
+    spec = kw.pop("agg")
+    col = spec.get("column")
+    agg = getattr(dsa, spec["kind"])() if col is None else getattr(dsa, spec["kind"])(col)
+    for k in ("x", "y"):
+        if isinstance(kw[k], dict):
+            kw[k] = np.asarray(kw[k]["ndarray"])
+    return kw, agg
+
+
+@pytest.mark.parametrize("source", ["pandas", "device"])
+@pytest.mark.parametrize("case", GC["lines"], ids=[c["name"] for c in GC["lines"]])
+def test_lines_golden(case, source):
+    df = G.line_frame(case["frame"])
+    src = df if source == "pandas" else dsa.DeviceFrame.from_pandas(df)
+    kw, agg_ = _line_kwargs(case["kwargs"])
+    cv = case["canvas"]
+    agg = dsa.Canvas(cv["W"], cv["H"], x_range=cv["x_range"], y_range=cv["y_range"]).line(
+        src, agg=agg_, **kw)
+    got = agg.values
+    if case.get("ones_at") is not None:
+        assert np.argwhere(got == 1).tolist() == case["ones_at"]
+        assert got.sum() == len(case["ones_at"])
+    else:
+        G.assert_same(got, G.expected_array(case))
+    if "x_range" in case:
+        assert np.allclose(agg.attrs["x_range"], case["x_range"])
+        assert np.allclose(agg.attrs["y_range"], case["y_range"])
+
+
+@pytest.mark.parametrize("case", GC["noaa_masks"], ids=[c["name"] for c in GC["noaa_masks"]])
+def test_noaa_masks(case):
+    """The reference's line images (test_antialias.py:302-370, max('val') lines): the
+    touched pixels of each Canvas.line call, unioned, equal the image's non-white mask."""
+    cv = case["canvas"]
+    c = dsa.Canvas(cv["W"], cv["H"], x_range=cv["x_range"], y_range=cv["y_range"])
+    touched = np.zeros((cv["H"], cv["W"]), bool)
+    for pl in case["polylines"]:
+        pl = np.asarray(pl, float)
+        df = pd.DataFrame({"x": pl[:, 0], "y": pl[:, 1], "val": 5.0})
+        agg = c.line(df, "x", "y", agg=dsa.max("val"))
+        touched |= ~np.isnan(agg.values)
+    np.testing.assert_array_equal(touched, np.array(case["expected"], bool))
+
+
+# ---- random lines vs the oracle --------------------------------------------------------
+
+@pytest.mark.parametrize("ydtype", [np.float32, np.float64])
+@pytest.mark.parametrize("layout", ["pandas", "row_major", "block"])
+def test_random_xconst_lines_vs_oracle(ydtype, layout):
+    """Random walks with NaN gaps, partly outside the view (clipped), x-constant axis 1."""
+    rng = np.random.default_rng(5)
+    nlines, nverts, W, H = 3000, 64, 300, 200
+    xs = np.linspace(-0.2, 1.1, nverts)
+    ys = (rng.uniform(-1.2, 1.2, (nlines, 1)) +
+          np.cumsum(rng.normal(0, 0.08, (nlines, nverts)), axis=1)).astype(ydtype)
+    ys[rng.random(ys.shape) < 0.01] = np.nan
+    names = [f"y{j}" for j in range(nverts)]
+    df = pd.DataFrame(ys, columns=names)
+    if layout == "block":
+        src = dsa.DeviceFrame.from_pandas(df)
+    elif layout == "row_major":  # columns are strided views of one [nlines, nverts] tensor
+        t = torch.as_tensor(ys, device="cuda")
+        src = dsa.DeviceFrame({n: t[:, j] for j, n in enumerate(names)})
+    else:
+        src = df
+    xr_, yr_ = (0.0, 1.0), (-1.0, 1.0)
+    agg = dsa.Canvas(W, H, x_range=xr_, y_range=yr_).line(src, x=xs, y=names,
+                                                         agg=dsa.count(), axis=1)
+    view = G.view_for(W, H, xr_, yr_, yf32=(ydtype == np.float32))
+    exp = oracle.lines_xconst(xs, ys.astype(np.float64), view, W, H)["count"]
+    np.testing.assert_array_equal(agg.values, exp)
+    assert exp.sum() > 0
+
+
+def test_random_axis0_and_axis1_vs_oracle():
+    rng = np.random.default_rng(6)
+    n, W, H = 20000, 128, 96
+    x = np.cumsum(rng.normal(0, 0.05, n))
+    y = np.cumsum(rng.normal(0, 0.05, n))
+    x[rng.random(n) < 0.002] = np.nan
+    df = pd.DataFrame({"x": x, "y": y, "v": rng.uniform(0, 1, n)})
+    xr_, yr_ = (float(np.nanpercentile(x, 10)), float(np.nanpercentile(x, 90))), \
+        (float(np.nanpercentile(y, 10)), float(np.nanpercentile(y, 90)))
+    c = dsa.Canvas(W, H, x_range=xr_, y_range=yr_)
+    view = G.view_for(W, H, xr_, yr_)
+    agg = c.line(df, "x", "y", agg=dsa.count())
+    exp = oracle.line_axis0(x, y, view, W, H)["count"]
+    np.testing.assert_array_equal(agg.values, exp)
+    # value reductions along a line take the value of the segment's first row
+    agg = c.line(df, "x", "y", agg=dsa.max("v"))
+    expm = oracle.line_axis0(x, y, view, W, H, ("max",), val=df["v"].to_numpy())["max"]
+    G.assert_same(agg.values, expm)
+    # axis 1: each row of a 2-D block is one line
+    X = rng.uniform(xr_[0] - 1, xr_[1] + 1, (500, 7))
+    Y = rng.uniform(yr_[0] - 1, yr_[1] + 1, (500, 7))
+    cols = {f"x{j}": X[:, j] for j in range(7)} | {f"y{j}": Y[:, j] for j in range(7)}
+    df1 = pd.DataFrame(cols)
+    agg = c.line(df1, x=[f"x{j}" for j in range(7)], y=[f"y{j}" for j in range(7)],
+                 agg=dsa.count(), axis=1)
+    np.testing.assert_array_equal(agg.values, oracle.lines_axis1(X, Y, view, W, H)["count"])
+
+
+def test_random_walks_2048x1024_vs_oracle():
+    """The 2048x1024 random-walk workload at 20k lines, bit-exact vs the oracle."""
+    from datashader_amd import synthetic
+    xs, ys = synthetic.random_walks_np(20000, 256, seed=4)
+    names = [f"y{j}" for j in range(256)]
+    f = dsa.DeviceFrame.from_pandas(pd.DataFrame(ys, columns=names))
+    agg = dsa.Canvas(2048, 1024).line(f, x=xs, y=names, agg=dsa.count(), axis=1)
+    xr_, yr_ = agg.x_range, agg.y_range
+    assert yr_ == G.maybe_expand(oracle.bounds(ys))
+    view = G.view_for(2048, 1024, xr_, yr_, yf32=True)
+    exp = oracle.lines_xconst(xs, ys.astype(np.float64), view, 2048, 1024)["count"]
+    np.testing.assert_array_equal(agg.values, exp)

@@ -1,0 +1,271 @@
+"""GPU parity of the points path: libdsx (through Canvas.points and through the C ABI) vs
+the reference's expectations and the CPU oracle.
+
+Tolerances (float32 value columns, against the float64 oracle):
+  count, count(v), any, by(...) counts, min, max   bit-exact
+  sum, mean                                         max per-bin relative error <= 1e-6
+  var, std                                          max per-bin relative error <= 1e-5
+float64 value columns accumulate in float64 and are held to 1e-12.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pandas as pd
+import pytest
+
+import oracle
+from oracle import closed_forms as cf
+
+import golden_utils as G
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import datashader_amd as dsa  # noqa: E402
+
+GC = G.cases()
+
+
+def _red(spec):
+    """Build a datashader_amd reduction from a golden-case spec."""
+    k = spec["kind"]
+    if k == "summary":
+        return dsa.summary(**{n: _red(v) for n, v in spec["items"].items()})
+    if k == "count_cat":
+        return dsa.count_cat(spec["column"])
+    if k == "by":
+        cat = spec["cat"]
+        if isinstance(cat, dict):
+            kind = cat.pop("kind") if "kind" in cat else None
+            cat = dict(cat)
+            if kind == "category_modulo":
+                cat = dsa.category_modulo(cat["column"], cat["modulo"], cat.get("offset", 0))
+            else:
+                cat = dsa.category_binning(cat["column"], cat["lower"], cat["upper"],
+                                           cat["nbins"])
+        return dsa.by(cat, _red(spec["reduction"]))
+    cls = getattr(dsa, k)
+    col = spec.get("column")
+    return cls() if col is None and k in ("count", "any") else cls(col)
+
+
+def _copy(spec):
+    import copy
+    return copy.deepcopy(spec)
+
+
+def _canvas(cv):
+    return dsa.Canvas(plot_width=cv["W"], plot_height=cv["H"], x_range=cv["x_range"],
+                      y_range=cv["y_range"])
+
+
+@pytest.mark.parametrize("source", ["pandas", "device"])
+@pytest.mark.parametrize("case", GC["points"], ids=[c["name"] for c in GC["points"]])
+def test_points_golden(case, source):
+    df = G.frame(case["frame"])
+    src = df if source == "pandas" else dsa.DeviceFrame.from_pandas(df)
+    agg = _canvas(case["canvas"]).points(src, case["x"], case["y"], _red(_copy(case["agg"])))
+    if "pick" in case:
+        agg = agg[case["pick"]]
+    G.assert_same(agg.values, G.expected_array(case))
+    if "x_range" in case:
+        assert np.allclose(agg.attrs["x_range"], case["x_range"])
+        assert np.allclose(agg.attrs["y_range"], case["y_range"])
+    if "cat_dim" in case:
+        assert list(agg.dims) == [case["y"], case["x"], case["cat_dim"]]
+        assert list(agg.coords[case["cat_dim"]].values) == case["cat_coords"]
+    else:
+        assert list(agg.dims) == [case["y"], case["x"]]
+    if "x_coords" in case:
+        assert agg.coords["x"].values.tolist() == case["x_coords"]
+        assert agg.coords["y"].values.tolist() == case["y_coords"]
+    k = case["agg"]["kind"] if "pick" not in case else case["agg"]["items"][case["pick"]]["kind"]
+    exp_dtype = {"count": np.uint32, "any": np.bool_, "count_cat": np.uint32,
+                 "by": np.uint32}.get(k, np.float64)
+    assert agg.values.dtype == exp_dtype
+
+
+def test_coords_match_reference_formula():
+    """core.py:83-99 coordinates, compared with the reference's own lincoords."""
+    df = G.frame("_pandas")
+    agg = dsa.Canvas(2, 2, x_range=(0, 1), y_range=(0, 1)).points(df, "x", "y")
+    ax = dsa.LinearAxis()
+    lin = ax.compute_index(ax.compute_scale_and_translate((0, 1), 2), 2)
+    assert agg.coords["x"].values.tolist() == lin.tolist() == [0.25, 0.75]
+    assert agg.x_range == (0, 1)
+
+
+def test_log_axis_points():
+    """test_pandas.py:1171-1190 (log_x / log_y columns on a 2x2 canvas)."""
+    df = G.frame("_pandas")
+    df["log_x"] = np.array([1.0] * 10 + [10] * 10)
+    df["log_y"] = np.array([1.0] * 5 + [10] * 5 + [1] * 5 + [10] * 5)
+    sol = np.array([[5, 5], [5, 5]])
+    c = dsa.Canvas(2, 2, x_range=(1, 10), y_range=(0, 1), x_axis_type="log")
+    agg = c.points(df, "log_x", "y", dsa.count("i32"))
+    np.testing.assert_array_equal(agg.values, sol)
+    ax = dsa.LogAxis()
+    logc = ax.compute_index(ax.compute_scale_and_translate((1, 10), 2), 2)
+    np.testing.assert_array_equal(agg.coords["log_x"].values, logc)
+    c = dsa.Canvas(2, 2, x_range=(1, 10), y_range=(1, 10), x_axis_type="log",
+                   y_axis_type="log")
+    agg = c.points(df, "log_x", "log_y", dsa.count("i32"))
+    np.testing.assert_array_equal(agg.values, sol)
+
+
+@pytest.mark.parametrize("high", [9, 10, 99, 100])
+def test_uniform_diagonal_points(high):
+    """test_pandas.py:1147-1168."""
+    n = high * high
+    df = pd.DataFrame({"time": np.ones(n, dtype="i4"),
+                       "x": np.array([np.arange(0, high, dtype="f8")] * high).flatten(),
+                       "y": np.array([np.arange(0, high, dtype="f8")] * high).flatten()})
+    agg = dsa.Canvas(2, 2, x_range=(0, high), y_range=(0, high)).points(df, "x", "y",
+                                                                        dsa.count("time"))
+    diagonal = agg.values.diagonal(0)
+    assert sum(diagonal) == n
+    assert abs(high) % 2 == abs(diagonal[1] / high - diagonal[0] / high)
+
+
+# ---- random inputs vs the oracle ------------------------------------------------------
+
+def _rel_err(got, exp):
+    got = np.asarray(got, np.float64)
+    exp = np.asarray(exp, np.float64)
+    assert np.array_equal(np.isnan(got), np.isnan(exp))
+    m = ~np.isnan(exp)
+    denom = np.maximum(np.abs(exp[m]), 1e-300)
+    return float(np.max(np.abs(got[m] - exp[m]) / denom)) if m.any() else 0.0
+
+
+def _all_aggs():
+    return dsa.summary(cnt=dsa.count(), cntv=dsa.count("v"), any=dsa.any(), s=dsa.sum("v"),
+                       mean=dsa.mean("v"), mn=dsa.min("v"), mx=dsa.max("v"),
+                       var=dsa.var("v"), std=dsa.std("v"))
+
+
+@pytest.mark.parametrize("dtype", ["f4", "f8"])
+@pytest.mark.parametrize("shape", [(97, 61), (800, 600)])
+def test_random_points_vs_oracle(dtype, shape):
+    W, H = shape
+    rng = np.random.default_rng(7)
+    n = 1_000_003
+    x = rng.normal(0.0, 3.0, n).astype(dtype)
+    y = rng.normal(0.5, 2.0, n).astype(dtype)
+    v = rng.lognormal(0.0, 2.0, n).astype(dtype)
+    for a in (x, y, v):
+        a[rng.choice(n, n // 200, replace=False)] = np.nan
+    df = pd.DataFrame({"x": x, "y": y, "v": v})
+    xr_, yr_ = (-6.0, 7.5), (-3.0, 4.0)
+    ds = dsa.Canvas(W, H, x_range=xr_, y_range=yr_).points(df, "x", "y", _all_aggs())
+    view = G.view_for(W, H, xr_, yr_)
+    r = oracle.points(x, y, view, W, H, ("count", "countv", "sumz", "min", "max", "m2"),
+                      val=v)
+    np.testing.assert_array_equal(ds["cnt"].values, r["count"])
+    np.testing.assert_array_equal(ds["cntv"].values, r["countv"])
+    np.testing.assert_array_equal(ds["any"].values, r["count"] > 0)
+    G.assert_same(ds["mn"].values, np.where(r["countv"] > 0, r["min"], np.nan))
+    G.assert_same(ds["mx"].values, np.where(r["countv"] > 0, r["max"], np.nan))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        s_exp = np.where(r["countv"] > 0, r["sumz"], np.nan)
+        m_exp = np.where(r["countv"] > 0, r["sumz"] / r["countv"], np.nan)
+        v_exp = np.where(r["countv"] > 0, r["m2"] / r["countv"], np.nan)
+    tol = 1e-6 if dtype == "f4" else 1e-12
+    assert _rel_err(ds["s"].values, s_exp) <= tol
+    assert _rel_err(ds["mean"].values, m_exp) <= tol
+    vt = 1e-5 if dtype == "f4" else 1e-9
+    big = r["countv"] >= 2
+    ve = _rel_err(np.where(big, ds["var"].values, np.nan), np.where(big, v_exp, np.nan))
+    assert ve <= vt, ve
+    se = _rel_err(np.where(big, ds["std"].values, np.nan), np.where(big, np.sqrt(v_exp), np.nan))
+    assert se <= vt, se
+
+
+def test_skewed_hot_bins_compensated_sum():
+    """90% of the rows in a 0.01-sigma blob: thousands of rows per hot bin.  The float32
+    compensated sum stays within 1e-6 of the float64 row-ordered sum."""
+    from datashader_amd import synthetic
+    n = 4_000_000
+    x, y, v = synthetic.points_skewed_np(n, seed=2)
+    W, H = 1920, 1080
+    xr_ = (float(np.nanmin(x)), float(np.nanmax(x)))
+    yr_ = (float(np.nanmin(y)), float(np.nanmax(y)))
+    df = pd.DataFrame({"x": x, "y": y, "v": v})
+    ds = dsa.Canvas(W, H, x_range=xr_, y_range=yr_).points(
+        df, "x", "y", dsa.summary(c=dsa.count(), m=dsa.mean("v"), s=dsa.sum("v")))
+    view = G.view_for(W, H, xr_, yr_)
+    bins = cf.point_bins(x, y, view, W, H)
+    np.testing.assert_array_equal(ds["c"].values, cf.count(bins, W, H))
+    assert ds["c"].values.max() > 1000
+    assert _rel_err(ds["m"].values, cf.mean(bins, v, W, H)) <= 1e-6
+    assert _rel_err(ds["s"].values, cf.sum_nan(bins, v, W, H)) <= 1e-6
+
+
+def test_by_category_vs_oracle():
+    from datashader_amd import synthetic
+    n = 2_000_000
+    x, y, cat = synthetic.points_ridges_np(n, seed=3)
+    cats = [f"c{k}" for k in range(16)]
+    df = pd.DataFrame({"x": x, "y": y, "cat": pd.Categorical.from_codes(cat, cats)})
+    W = H = 256
+    agg = dsa.Canvas(W, H, x_range=(0, 1), y_range=(0, 1)).points(
+        df, "x", "y", dsa.by("cat", dsa.count()))
+    view = G.view_for(W, H, (0, 1), (0, 1))
+    exp = oracle.points(x, y, view, W, H, ("count",), cat=cat.astype(np.int64), ncat=16,
+                        squeeze=False)["count"]
+    np.testing.assert_array_equal(agg.values, exp)
+    assert list(agg.coords["cat"].values) == cats
+    plain = dsa.Canvas(W, H, x_range=(0, 1), y_range=(0, 1)).points(df, "x", "y")
+    np.testing.assert_array_equal(agg.values.sum(axis=2), plain.values)
+
+
+def test_full_size_count_800x600_f64_bitexact():
+    """10M float64 rows of the 5-Gaussian mixture into 800x600, bit-exact vs the oracle."""
+    from datashader_amd import synthetic
+    x, y = synthetic.points_mixture_np(10_000_000, seed=1)
+    W, H = 800, 600
+    df = pd.DataFrame({"x": x, "y": y})
+    agg = dsa.Canvas(W, H, x_range=(-10, 10), y_range=(-10, 10)).points(df, "x", "y",
+                                                                        dsa.count())
+    view = G.view_for(W, H, (-10, 10), (-10, 10))
+    exp = oracle.points(x, y, view, W, H, ("count",))["count"]
+    np.testing.assert_array_equal(agg.values, exp)
+    assert agg.values.dtype == np.uint32
+
+
+def test_autorange_on_device_matches_reference_bounds():
+    rng = np.random.default_rng(11)
+    x = rng.normal(size=100_001).astype(np.float32)
+    y = rng.normal(size=100_001)
+    x[5] = np.nan
+    df = pd.DataFrame({"x": x, "y": y})
+    agg = dsa.Canvas(50, 40).points(df, "x", "y")
+    assert agg.x_range == G.maybe_expand(oracle.bounds(x))
+    assert agg.y_range == G.maybe_expand(oracle.bounds(y))
+    # all-NaN and constant columns (glyph.py:42-49)
+    df = pd.DataFrame({"x": np.full(10, np.nan), "y": np.full(10, 3.0)})
+    agg = dsa.Canvas(4, 4).points(df, "x", "y")
+    assert agg.x_range == (-1.0, 1.0) and agg.y_range == (2.0, 4.0)
+    assert agg.values.sum() == 0
+
+
+def test_empty_and_tiny_inputs():
+    df = pd.DataFrame({"x": np.zeros(0), "y": np.zeros(0), "v": np.zeros(0)})
+    ds = dsa.Canvas(3, 2, x_range=(0, 1), y_range=(0, 1)).points(
+        df, "x", "y", dsa.summary(c=dsa.count(), m=dsa.mean("v"), mx=dsa.max("v")))
+    assert ds["c"].values.sum() == 0
+    assert np.isnan(ds["m"].values).all() and np.isnan(ds["mx"].values).all()
+    df = pd.DataFrame({"x": [0.5], "y": [0.5], "v": [np.inf]})
+    ds = dsa.Canvas(1, 1, x_range=(0, 1), y_range=(0, 1)).points(
+        df, "x", "y", dsa.summary(s=dsa.sum("v"), mn=dsa.min("v")))
+    assert ds["s"].values[0, 0] == np.inf and ds["mn"].values[0, 0] == np.inf
+
+
+def test_device_frame_result_stays_on_device():
+    x = torch.tensor([0.1, 0.9, 0.9], dtype=torch.float32, device="cuda")
+    y = torch.tensor([0.1, 0.9, 0.9], dtype=torch.float32, device="cuda")
+    f = dsa.DeviceFrame({"x": x, "y": y})
+    agg = dsa.Canvas(2, 2, x_range=(0, 1), y_range=(0, 1)).points(f, "x", "y")
+    assert agg.data.is_cuda and agg.data.dtype == torch.uint32
+    np.testing.assert_array_equal(agg.values, [[1, 0], [0, 2]])

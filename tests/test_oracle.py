@@ -1,0 +1,234 @@
+"""Pin the CPU oracle before trusting it: against the reference's own test expectations
+(golden fixtures), the touched-pixel masks of the reference's line images, numpy closed
+forms and brute force on tiny inputs.  CPU only."""
+from __future__ import annotations
+
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import closed_forms as cf
+
+import golden_utils as G
+
+GC = G.cases()
+
+
+# ---- points: the reference's expectations ------------------------------------------
+
+@pytest.mark.parametrize("case", GC["points"], ids=[c["name"] for c in GC["points"]])
+def test_points_golden(case):
+    got, (xr_, yr_) = G.point_case_oracle(case)
+    G.assert_same(got, G.expected_array(case))
+    if "x_range" in case:
+        assert np.allclose(xr_, case["x_range"]) and np.allclose(yr_, case["y_range"])
+
+
+@pytest.mark.parametrize("high", [9, 10, 99, 100])
+def test_uniform_diagonal_points(high):
+    """test_pandas.py:1147-1168: a square grid of integer points, 2x2 canvas."""
+    low = 0
+    width = height = high - low
+    n = width * height
+    x = np.array([np.arange(low, high, dtype="f8")] * width).flatten()
+    y = np.array([np.arange(low, high, dtype="f8")] * height).flatten()
+    view = G.view_for(2, 2, (low, high), (low, high))
+    agg = oracle.points(x, y, view, 2, 2, ("count",))["count"]
+    diagonal = agg.diagonal(0)
+    assert sum(diagonal) == n
+    assert abs(high - low) % 2 == abs(diagonal[1] / high - diagonal[0] / high)
+
+
+# ---- points: closed forms on random inputs ----------------------------------------
+
+def _random_points(rng, n, nan_frac=0.01, f32=False):
+    x = rng.normal(0.0, 3.0, n)
+    y = rng.normal(0.5, 2.0, n)
+    v = rng.lognormal(0.0, 2.0, n)
+    if f32:
+        x, y, v = x.astype(np.float32), y.astype(np.float32), v.astype(np.float32)
+    k = int(n * nan_frac)
+    x[rng.choice(n, k, replace=False)] = np.nan
+    y[rng.choice(n, k, replace=False)] = np.nan
+    v[rng.choice(n, k, replace=False)] = np.nan
+    return x, y, v
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_points_closed_forms(rng, f32):
+    W, H = 97, 61
+    x, y, v = _random_points(rng, 200_000, f32=f32)
+    view = G.view_for(W, H, (-6.0, 7.5), (-3.0, 4.0))
+    bins_c = oracle.point_bins(x, y, view, W, H)
+    bins_np = cf.point_bins(x, y, view, W, H)
+    np.testing.assert_array_equal(bins_c, bins_np)
+    r = oracle.points(x, y, view, W, H, BASES_ALL, val=v)
+    np.testing.assert_array_equal(r["count"], cf.count(bins_np, W, H))
+    np.testing.assert_array_equal(r["countv"], cf.count(bins_np, W, H, v))
+    # bincount adds sequentially in row order: bit-identical to the row-ordered f64 sum
+    np.testing.assert_array_equal(r["sumz"], cf.sum_zero(bins_np, v, W, H))
+    G.assert_same(r["sum"], cf.sum_nan(bins_np, v, W, H))
+    mn, mx = cf.minmax(bins_np, v, W, H)
+    G.assert_same(r["min"], mn)
+    G.assert_same(r["max"], mx)
+    np.testing.assert_array_equal(r["any"], cf.count(bins_np, W, H) > 0)
+    np.testing.assert_array_equal(r["anyv"], cf.count(bins_np, W, H, v) > 0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        var = np.where(r["countv"] > 0, r["m2"] / r["countv"], np.nan)
+    ref = cf.var2pass(bins_np, v, W, H)
+    ok = ~np.isnan(ref)
+    assert np.array_equal(np.isnan(var), np.isnan(ref))
+    # Welford-on-sums vs two-pass: same quantity, different rounding
+    assert np.allclose(var[ok], ref[ok], rtol=1e-9, atol=0)
+
+
+BASES_ALL = ("count", "countv", "any", "anyv", "sum", "sumz", "min", "max", "m2")
+
+
+def test_points_edges_and_wraparound():
+    """Upper bound maps into the last bin; values below xmin are dropped; a tiny
+    negative product truncates to bin 0."""
+    view = G.view_for(4, 4, (0.0, 1.0), (0.0, 1.0))
+    x = np.array([0.0, 1.0, -1e-300, 0.999999, 1.0000001, np.nan, 0.5])
+    y = np.array([0.0, 1.0, 0.0, 0.999999, 0.5, 0.5, np.inf])
+    bins = oracle.point_bins(x, y, view, 4, 4)
+    assert bins.tolist() == [0, 15, -1, 15, -1, -1, -1]
+
+
+def test_points_brute_force_tiny(rng):
+    """Exhaustive small inputs: every row's bin computed by hand in Python floats."""
+    W, H = 3, 2
+    xr_, yr_ = (-1.0, 2.0), (0.0, 1.0)
+    view = G.view_for(W, H, xr_, yr_)
+    vals = [-1.0, -0.5, 0.0, 0.3333333333333333, 1.0, 1.9999999, 2.0, 2.5, math.nan]
+    ys = [-0.1, 0.0, 0.49999, 0.5, 1.0, math.nan]
+    pts = list(itertools.product(vals, ys))
+    x = np.array([p[0] for p in pts])
+    y = np.array([p[1] for p in pts])
+    exp = np.zeros((H, W), np.uint32)
+    for a, b in pts:
+        if xr_[0] <= a <= xr_[1] and yr_[0] <= b <= yr_[1]:
+            xx = int(a * view.sx + view.tx)
+            yy = int(b * view.sy + view.ty)
+            exp[min(yy, H - 1), min(xx, W - 1)] += 1
+    got = oracle.points(x, y, view, W, H, ("count",))["count"]
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_by_category_wraparound():
+    """A code of -1 (pandas' NaN category) lands in the last category."""
+    view = G.view_for(2, 1, (0.0, 1.0), (0.0, 1.0))
+    x = np.array([0.1, 0.1, 0.9, 0.9])
+    y = np.zeros(4)
+    cat = np.array([0, -1, 1, 2])
+    r = oracle.points(x, y, view, 2, 1, ("count",), cat=cat, ncat=3, squeeze=False)["count"]
+    assert r[0, 0].tolist() == [1, 0, 1] and r[0, 1].tolist() == [0, 1, 1]
+    # sum over categories equals the plain count
+    plain = oracle.points(x, y, view, 2, 1, ("count",))["count"]
+    np.testing.assert_array_equal(r.sum(axis=2), plain)
+
+
+def test_bounds():
+    a = np.array([np.nan, 3.0, -2.5, np.inf, np.nan])
+    assert oracle.bounds(a) == (-2.5, np.inf)
+    assert oracle.bounds(np.array([np.nan])) == (np.inf, -np.inf)
+    r = np.random.default_rng(0).normal(size=1000)
+    assert oracle.bounds(r) == (np.nanmin(r), np.nanmax(r))
+
+
+# ---- lines: the reference's expectations --------------------------------------------
+
+@pytest.mark.parametrize("case", GC["glyph_lines"], ids=[c["name"] for c in GC["glyph_lines"]])
+def test_glyph_lines_golden(case):
+    v = case["view"]
+    view = oracle.View(v["sx"], v["tx"], v["sy"], v["ty"], v["xmin"], v["xmax"], v["ymin"],
+                       v["ymax"])
+    W, H = case["W"], case["H"]
+    if case["kind"] == "segment":
+        (x0, y0), (x1, y1) = case["p0"], case["p1"]
+        agg = oracle.draw_segment(view, W, H, case["segment_start"], x0, y0, x1, y1)["count"]
+    else:
+        agg = oracle.line_axis0(G.dec(case["xs"]), G.dec(case["ys"]), view, W, H,
+                                plot_start=case["plot_start"])["count"]
+    if case["expected"] is not None:
+        np.testing.assert_array_equal(agg, np.array(case["expected"]))
+    if case.get("total") is not None:
+        assert agg.sum() == case["total"]
+    if case.get("at"):
+        r, c_, val = case["at"]
+        assert agg[r, c_] == val
+
+
+@pytest.mark.parametrize("case", GC["lines"], ids=[c["name"] for c in GC["lines"]])
+def test_lines_golden(case):
+    got, (xr_, yr_) = G.line_case_oracle(case)
+    if case.get("ones_at") is not None:
+        ones = np.argwhere(got == 1).tolist()
+        assert ones == case["ones_at"]
+        assert got.sum() == len(ones)
+    else:
+        G.assert_same(got, G.expected_array(case))
+    if "x_range" in case:
+        assert np.allclose(xr_, case["x_range"]) and np.allclose(yr_, case["y_range"])
+
+
+@pytest.mark.parametrize("case", GC["noaa_masks"], ids=[c["name"] for c in GC["noaa_masks"]])
+def test_noaa_masks(case):
+    """Union of the pixels the oracle's Bresenham touches == the non-white pixels of
+    the reference's rendered image."""
+    cv = case["canvas"]
+    W, H = cv["W"], cv["H"]
+    view = G.view_for(W, H, cv["x_range"], cv["y_range"])
+    touched = np.zeros((H, W), bool)
+    for pl in case["polylines"]:
+        pl = np.asarray(pl, float)
+        touched |= oracle.line_axis0(pl[:, 0], pl[:, 1], view, W, H, ("count",))["count"] > 0
+    np.testing.assert_array_equal(touched, np.array(case["expected"], bool))
+
+
+# ---- lines: brute-force properties ---------------------------------------------------
+
+def test_lines_xconst_equals_axis1_and_axis0(rng):
+    """The three layouts draw the same segments: x-constant == axis1 with broadcast x ==
+    one axis0 call per row."""
+    nrows, ncols, W, H = 40, 17, 31, 23
+    xs = np.sort(rng.uniform(-1, 1, ncols))
+    ys = rng.normal(0, 1, (nrows, ncols)).cumsum(axis=1) * 0.3
+    ys[3, 5] = np.nan
+    view = G.view_for(W, H, (-0.8, 0.9), (-1.5, 1.2))
+    a = oracle.lines_xconst(xs, ys, view, W, H)["count"]
+    b = oracle.lines_axis1(np.broadcast_to(xs, ys.shape), ys, view, W, H)["count"]
+    np.testing.assert_array_equal(a, b)
+    c = sum(oracle.line_axis0(xs, ys[i], view, W, H, plot_start=True)["count"].astype(np.int64)
+            for i in range(nrows))
+    np.testing.assert_array_equal(a, c)
+
+
+def test_line_unclipped_pixel_count(rng):
+    """An unclipped segment drawn with segment_start touches max(|dx|,|dy|)+1 pixels
+    (Bresenham steps once per major-axis pixel), its two snapped ends included."""
+    W = H = 64
+    view = G.view_for(W, H, (0.0, 1.0), (0.0, 1.0))
+    for _ in range(200):
+        p = rng.uniform(0.0, 0.999, 4)
+        agg = oracle.draw_segment(view, W, H, True, p[0], p[1], p[2], p[3])["count"]
+        x0, y0 = int(p[0] * W), int(p[1] * H)
+        x1, y1 = int(p[2] * W), int(p[3] * H)
+        n = max(abs(x1 - x0), abs(y1 - y0)) + 1
+        assert agg.sum() == n
+        assert agg[y0, x0] >= 1 and agg[y1, x1] >= 1
+
+
+def test_line_f32_delta_flag():
+    """The float32 segment delta only changes results when a segment is clipped."""
+    W = H = 16
+    v64 = G.view_for(W, H, (0.0, 1.0), (0.0, 1.0))
+    v32 = G.view_for(W, H, (0.0, 1.0), (0.0, 1.0), xf32=True, yf32=True)
+    xs = np.array([0.1, 0.7, 0.3], dtype=np.float32).astype(float)
+    ys = np.array([0.2, 0.9, 0.4], dtype=np.float32).astype(float)
+    a = oracle.line_axis0(xs, ys, v64, W, H)["count"]
+    b = oracle.line_axis0(xs, ys, v32, W, H)["count"]
+    np.testing.assert_array_equal(a, b)
