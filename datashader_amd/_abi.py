@@ -12,7 +12,7 @@ import os
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libdsx.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # dtype codes
 F32, F64, I8, U8, I16, I32, I64 = 1, 2, 3, 4, 5, 6, 7
@@ -21,16 +21,15 @@ FILL_ZERO32, FILL_ONES32, FILL_ZERO64, FILL_ONES64 = 1, 2, 3, 4
 # finalize modes
 FIN_ANY, FIN_SUM, FIN_MEAN, FIN_VAR, FIN_STD, FIN_MIN, FIN_MAX, FIN_SUMZERO = range(1, 9)
 
-EXPORTS = ("dsx_points_agg", "dsx_points_m2", "dsx_lines_agg", "dsx_bounds_workspace_size",
-           "dsx_bounds", "dsx_fill", "dsx_finalize", "dsx_abi_version", "dsx_strerror",
-           "dsx_last_hip_error")
+EXPORTS = ("dsx_points_workspace_size", "dsx_points_agg", "dsx_lines_agg",
+           "dsx_bounds_workspace_size", "dsx_bounds", "dsx_fill", "dsx_finalize",
+           "dsx_abi_version", "dsx_strerror", "dsx_last_hip_error")
 
 vp = ctypes.c_void_p
 
 
-class Accums(ctypes.Structure):
-    _fields_ = [("count", vp), ("countv", vp), ("sum_hi", vp), ("sum_lo", vp),
-                ("sum64", vp), ("min32", vp), ("max32", vp), ("min64", vp), ("max64", vp)]
+class Planes(ctypes.Structure):
+    _fields_ = [("count", vp), ("countv", vp), ("sum", vp), ("min", vp), ("max", vp)]
 
 
 class View(ctypes.Structure):
@@ -56,7 +55,8 @@ class Categorizer(ctypes.Structure):
 
 class PointsDesc(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("x", vp), ("y", vp), ("xy_dtype", ctypes.c_int32),
-                ("val_dtype", ctypes.c_int32), ("val", vp), ("cat", Categorizer)]
+                ("val_dtype", ctypes.c_int32), ("val", vp), ("cat", Categorizer),
+                ("window", vp), ("reuse_window", ctypes.c_int32)]
 
 
 class LinesDesc(ctypes.Structure):
@@ -69,9 +69,9 @@ class LinesDesc(ctypes.Structure):
 
 
 class FinalizeDesc(ctypes.Structure):
-    _fields_ = [("mode", ctypes.c_int32), ("val_dtype", ctypes.c_int32),
-                ("ncells", ctypes.c_int64), ("count", vp), ("countv", vp), ("hi", vp),
-                ("lo", vp), ("v64", vp), ("key", vp), ("out", vp)]
+    _fields_ = [("mode", ctypes.c_int32), ("key_bytes", ctypes.c_int32),
+                ("ncells", ctypes.c_int64), ("count", vp), ("countv", vp), ("sum", vp),
+                ("key", vp), ("out", vp)]
 
 
 class DsxError(RuntimeError):
@@ -92,9 +92,10 @@ def load():
         lib = ctypes.CDLL(LIB_PATH)
         P = ctypes.POINTER
         i32, i64, sz = ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t
-        lib.dsx_points_agg.argtypes = [P(PointsDesc), P(View), P(Accums), vp]
-        lib.dsx_points_m2.argtypes = [P(PointsDesc), P(View), vp, vp, vp, vp, vp]
-        lib.dsx_lines_agg.argtypes = [P(LinesDesc), P(View), P(Accums), vp]
+        lib.dsx_points_workspace_size.argtypes = [P(PointsDesc), P(View), P(Planes)]
+        lib.dsx_points_workspace_size.restype = sz
+        lib.dsx_points_agg.argtypes = [P(PointsDesc), P(View), P(Planes), vp, vp, sz, vp]
+        lib.dsx_lines_agg.argtypes = [P(LinesDesc), P(View), P(Planes), vp]
         lib.dsx_bounds_workspace_size.argtypes = [i64]
         lib.dsx_bounds_workspace_size.restype = sz
         lib.dsx_bounds.argtypes = [vp, i32, i64, vp, vp, sz, vp]
@@ -105,7 +106,7 @@ def load():
         lib.dsx_strerror.restype = ctypes.c_char_p
         lib.dsx_last_hip_error.argtypes = []
         lib.dsx_last_hip_error.restype = ctypes.c_char_p
-        for f in ("dsx_points_agg", "dsx_points_m2", "dsx_lines_agg", "dsx_bounds",
+        for f in ("dsx_points_agg", "dsx_lines_agg", "dsx_bounds",
                   "dsx_fill", "dsx_finalize", "dsx_abi_version"):
             getattr(lib, f).restype = ctypes.c_int
         if lib.dsx_abi_version() != ABI_VERSION:

@@ -127,10 +127,36 @@ def _categorizer(cat, frame, schema):
 
 
 class PassResult:
-    def __init__(self, ncat, f32, planes):
+    def __init__(self, ncat, keyb, planes):
         self.ncat = ncat
-        self.f32 = f32
+        self.keyb = keyb          # min / max key width in bytes (4: float32 values)
         self.planes = planes
+
+
+def _window_slot(frame, key, device):
+    """(device int32[4] window, reuse flag) of the hot-window cache kept on a
+    DeviceFrame.  The window only steers performance (which canvas cells are kept in
+    LDS during the scan); any stored window gives the same result."""
+    t = dev.torch()
+    cache = frame.__dict__.setdefault("_window_cache", {})
+    slot = cache.get(key)
+    if slot is None:
+        slot = cache[key] = t.zeros(4, dtype=t.int32, device=device)
+        return slot, 0
+    return slot, 1
+
+
+def _points_call(lib, d, vw, planes_struct, mean, device, stream):
+    """dsx_points_agg with a workspace from torch's caching allocator."""
+    t = dev.torch()
+    ws = int(lib.dsx_points_workspace_size(d, vw, planes_struct))
+    if ws == 0:
+        raise _abi.DsxError("dsx_points_workspace_size rejected the call (canvas too large "
+                            "for the tile partition, or invalid descriptor)")
+    work = t.empty(ws, dtype=t.uint8, device=device)
+    _abi.check(_timed(lambda: lib.dsx_points_agg(d, vw, planes_struct, dev.ptr(mean),
+                                                 dev.ptr(work), ws, stream)),
+               "dsx_points_agg")
 
 
 def run_pass(lib, frame, schema, glyph, view, p, H, W):
@@ -140,38 +166,29 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W):
     ncells = H * W * ncat
     val = dev.as_float_column(frame[p.val]) if p.val is not None else None
     f32 = val is not None and val.dtype == t.float32
-    planes = {}
-    acc = _abi.Accums()
-    if "count" in p.planes:
-        planes["count"] = _alloc(ncells, "u32", device)
-        _fill(lib, planes["count"], _abi.FILL_ZERO32)
-        acc.count = dev.ptr(planes["count"])
-    if "countv" in p.planes:
-        planes["countv"] = _alloc(ncells, "u32", device)
-        _fill(lib, planes["countv"], _abi.FILL_ZERO32)
-        acc.countv = dev.ptr(planes["countv"])
-    if "sum" in p.planes:
-        if f32:
-            planes["sum_hi"] = _alloc(ncells, "f32", device)
-            planes["sum_lo"] = _alloc(ncells, "f32", device)
-            _fill(lib, planes["sum_hi"], _abi.FILL_ZERO32)
-            _fill(lib, planes["sum_lo"], _abi.FILL_ZERO32)
-            acc.sum_hi, acc.sum_lo = dev.ptr(planes["sum_hi"]), dev.ptr(planes["sum_lo"])
-        else:
-            planes["sum64"] = _alloc(ncells, "f64", device)
-            _fill(lib, planes["sum64"], _abi.FILL_ZERO64)
-            acc.sum64 = dev.ptr(planes["sum64"])
-    for m, empty32, empty64 in (("min", _abi.FILL_ONES32, _abi.FILL_ONES64),
-                                ("max", _abi.FILL_ZERO32, _abi.FILL_ZERO64)):
-        if m in p.planes:
-            k = _alloc(ncells, "u32" if f32 else "u64", device)
-            _fill(lib, k, empty32 if f32 else empty64)
-            planes[m] = k
-            setattr(acc, m + ("32" if f32 else "64"), dev.ptr(k))
+    keyb = 4 if f32 else 8
     vdt = (_abi.F32 if f32 else _abi.F64) if val is not None else 0
     vw = _view_struct(view)
     stream = dev.stream_handle()
-    if isinstance(glyph, Point):
+    points = isinstance(glyph, Point)
+    if not points and not isinstance(glyph, _LineLike):
+        raise NotImplementedError(f"glyph {type(glyph).__name__} is not supported")
+    planes = {}
+    ps = _abi.Planes()
+    # points write every cell; lines accumulate into planes filled with the initial values
+    for name, kind, init in (("count", "u32", _abi.FILL_ZERO32),
+                             ("countv", "u32", _abi.FILL_ZERO32),
+                             ("sum", "f64", _abi.FILL_ZERO64),
+                             ("min", "u32" if f32 else "u64",
+                              _abi.FILL_ONES32 if f32 else _abi.FILL_ONES64),
+                             ("max", "u32" if f32 else "u64",
+                              _abi.FILL_ZERO32 if f32 else _abi.FILL_ZERO64)):
+        if name in p.planes:
+            planes[name] = _alloc(ncells, kind, device)
+            if not points:
+                _fill(lib, planes[name], init)
+            setattr(ps, name, dev.ptr(planes[name]))
+    if points:
         x = dev.as_float_column(frame[glyph.x])
         y = dev.as_float_column(frame[glyph.y])
         if x.dtype != y.dtype:  # x and y share one dtype in the kernel
@@ -182,22 +199,21 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W):
         d.xy_dtype = dev.dtype_code(x)
         d.val, d.val_dtype = dev.ptr(val), vdt
         d.cat = catc
-        _abi.check(_timed(lambda: lib.dsx_points_agg(d, vw, acc, stream)), "dsx_points_agg")
+        if getattr(frame, "persistent", False):
+            key = (glyph.x, glyph.y, p.val, None if p.cat is None else p.cat._key(),
+                   tuple(sorted(p.planes)),
+                   W, H, tuple(view[k] for k in ("xmin", "xmax", "ymin", "ymax")))
+            win, reuse = _window_slot(frame, key, device)
+            d.window, d.reuse_window = dev.ptr(win), reuse
+        _points_call(lib, d, vw, ps, None, device, stream)
         if p.m2:
             mean = t.empty(ncells, dtype=t.float64, device=device)
-            _finalize(lib, _abi.FIN_MEAN, f32, ncells, planes, mean)
-            if f32:
-                planes["m2_hi"] = _alloc(ncells, "f32", device)
-                planes["m2_lo"] = _alloc(ncells, "f32", device)
-                _fill(lib, planes["m2_hi"], _abi.FILL_ZERO32)
-                _fill(lib, planes["m2_lo"], _abi.FILL_ZERO32)
-                m2p = (dev.ptr(planes["m2_hi"]), dev.ptr(planes["m2_lo"]), None)
-            else:
-                planes["m2_64"] = _alloc(ncells, "f64", device)
-                _fill(lib, planes["m2_64"], _abi.FILL_ZERO64)
-                m2p = (None, None, dev.ptr(planes["m2_64"]))
-            _abi.check(lib.dsx_points_m2(d, vw, dev.ptr(mean), *m2p, stream), "dsx_points_m2")
-    elif isinstance(glyph, _LineLike):
+            _finalize(lib, _abi.FIN_MEAN, keyb, ncells, planes, mean)
+            planes["m2"] = _alloc(ncells, "f64", device)
+            m2s = _abi.Planes()
+            m2s.sum = dev.ptr(planes["m2"])
+            _points_call(lib, d, vw, m2s, mean, device, stream)
+    else:
         if p.m2:
             raise NotImplementedError("var / std of lines are not supported")
         lay = glyph.vertex_layout(frame)
@@ -211,16 +227,14 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W):
         d.plot_start = 1
         d.val, d.val_dtype = dev.ptr(val), vdt
         d.cat = catc
-        _abi.check(_timed(lambda: lib.dsx_lines_agg(d, vw, acc, stream)), "dsx_lines_agg")
-    else:
-        raise NotImplementedError(f"glyph {type(glyph).__name__} is not supported")
-    return PassResult(ncat, f32, planes)
+        _abi.check(_timed(lambda: lib.dsx_lines_agg(d, vw, ps, stream)), "dsx_lines_agg")
+    return PassResult(ncat, keyb, planes)
 
 
-def _finalize(lib, mode, f32, ncells, planes, out, key=None, counts=None):
+def _finalize(lib, mode, keyb, ncells, planes, out, key=None, counts=None):
     f = _abi.FinalizeDesc()
     f.mode = mode
-    f.val_dtype = _abi.F32 if f32 else _abi.F64
+    f.key_bytes = keyb
     f.ncells = ncells
     if mode == _abi.FIN_ANY:
         f.count = dev.ptr(counts)
@@ -228,11 +242,7 @@ def _finalize(lib, mode, f32, ncells, planes, out, key=None, counts=None):
         f.key = dev.ptr(key)
     else:
         f.countv = dev.ptr(planes["countv"])
-        if mode in (_abi.FIN_VAR, _abi.FIN_STD):
-            hi, lo, v64 = planes.get("m2_hi"), planes.get("m2_lo"), planes.get("m2_64")
-        else:
-            hi, lo, v64 = planes.get("sum_hi"), planes.get("sum_lo"), planes.get("sum64")
-        f.hi, f.lo, f.v64 = dev.ptr(hi), dev.ptr(lo), dev.ptr(v64)
+        f.sum = dev.ptr(planes["m2"] if mode in (_abi.FIN_VAR, _abi.FIN_STD) else planes["sum"])
     f.out = dev.ptr(out)
     _abi.check(lib.dsx_finalize(f, dev.stream_handle()), "dsx_finalize")
 
@@ -250,12 +260,12 @@ def finalize_leaf(lib, red, res, H, W):
     if isinstance(inner, rd.any):
         counts = res.planes["count" if inner.column is None else "countv"]
         out = t.empty(ncells, dtype=t.uint8, device=device)
-        _finalize(lib, _abi.FIN_ANY, res.f32, ncells, res.planes, out, counts=counts)
+        _finalize(lib, _abi.FIN_ANY, res.keyb, ncells, res.planes, out, counts=counts)
         return out.view(t.bool).reshape(shape)
     out = t.empty(ncells, dtype=t.float64, device=device)
     mode = inner.finalize_mode
     key = res.planes.get("min") if mode == _abi.FIN_MIN else res.planes.get("max")
-    _finalize(lib, mode, res.f32, ncells, res.planes, out, key=key)
+    _finalize(lib, mode, res.keyb, ncells, res.planes, out, key=key)
     return out.reshape(shape)
 
 
@@ -328,6 +338,7 @@ def pipeline(source, schema, canvas, glyph, agg, *, antialias=False, to_host=Non
     if isinstance(source, pd.DataFrame):
         dev.require_gpu()
         frame = DeviceFrame.from_pandas(source)
+        frame.persistent = False  # uploaded for this call only: keep no tuning state
         to_host = True if to_host is None else to_host
     else:
         frame = source

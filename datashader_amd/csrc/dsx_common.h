@@ -130,55 +130,58 @@ __device__ __forceinline__ int32_t proj(double x, double s, double t) {
 // Axis mapper: identity (LinearAxis) or log10 (LogAxis, core.py:127-145).
 __device__ __forceinline__ double axis_map(double v, int32_t log) { return log ? log10(v) : v; }
 
-// ---- accumulation -------------------------------------------------------------------
-// Compensated float32 accumulation into (hi, lo): the hardware add returns the old hi,
-// so the rounding error of hi + v is recovered exactly (Knuth's two-sum) and added to
-// lo.  hi + lo then equals the exact sum up to the rounding of lo's own additions.
-__device__ __forceinline__ void add_comp(float *hi, float *lo, float v) {
-    const float old = unsafeAtomicAdd(hi, v);
-    const float s = old + v;
-    const float bp = s - old;
-    const float err = (old - (s - bp)) + (v - bp);
-    if (err != 0.0f) unsafeAtomicAdd(lo, err);
+// ---- aggregate planes ----------------------------------------------------------------
+template <int KEYB> struct KeyOf;
+template <> struct KeyOf<4> {
+    using T = uint32_t;
+    static constexpr T kMinEmpty = 0xFFFFFFFFu;
+    __device__ static T key(double v) { return fkey32((float)v); }
+    __device__ static double val(T k) { return (double)unkey32(k); }
+};
+template <> struct KeyOf<8> {
+    using T = unsigned long long;
+    static constexpr T kMinEmpty = ~0ull;
+    __device__ static T key(double v) { return (T)fkey64(v); }
+    __device__ static double val(T k) { return unkey64((uint64_t)k); }
+};
+
+// Which planes a call produces.
+enum : uint32_t { PL_CNT = 1, PL_CNTV = 2, PL_SUM = 4, PL_MIN = 8, PL_MAX = 16 };
+
+inline uint32_t plane_flags(const dsx_planes *p) {
+    return (p->count ? PL_CNT : 0) | (p->countv ? PL_CNTV : 0) | (p->sum ? PL_SUM : 0) |
+           (p->min ? PL_MIN : 0) | (p->max ? PL_MAX : 0);
 }
 
 struct Planes {
     uint32_t *count, *countv;
-    float *sum_hi, *sum_lo;
-    double *sum64;
-    uint32_t *min32, *max32;
-    uint64_t *min64, *max64;
+    double *sum;
+    void *min, *max;
 };
 
-// Apply every requested base to cell `cell` for a value v (TV = float / double), or for
-// a row with no value column (HAS_VAL = false).
+inline Planes planes_of(const dsx_planes *a) {
+    Planes p;
+    p.count = a->count; p.countv = a->countv; p.sum = a->sum; p.min = a->min; p.max = a->max;
+    return p;
+}
+
+// Global-atomic append of one row (or line pixel) at `cell` (used by the line kernel,
+// whose planes the caller initialised).
 template <typename TV, bool HAS_VAL>
 __device__ __forceinline__ void append(const Planes &p, int64_t cell, TV v) {
     if (p.count) atomicAdd(p.count + cell, 1u);
     if constexpr (HAS_VAL) {
         if (v != v) return;  // NaN: skipped by every value-reading base
         if (p.countv) atomicAdd(p.countv + cell, 1u);
-        if constexpr (sizeof(TV) == 4) {
-            if (p.sum_hi) add_comp(p.sum_hi + cell, p.sum_lo + cell, (float)v);
-            if (p.min32) atomicMin(p.min32 + cell, fkey32((float)v));
-            if (p.max32) atomicMax(p.max32 + cell, fkey32((float)v));
-        } else {
-            if (p.sum64) unsafeAtomicAdd(p.sum64 + cell, (double)v);
-            if (p.min64) atomicMin((unsigned long long *)(p.min64 + cell),
-                                   (unsigned long long)fkey64((double)v));
-            if (p.max64) atomicMax((unsigned long long *)(p.max64 + cell),
-                                   (unsigned long long)fkey64((double)v));
-        }
+        if (p.sum) unsafeAtomicAdd(p.sum + cell, (double)v);
+        using K = KeyOf<sizeof(TV)>;
+        if (p.min) atomicMin(reinterpret_cast<typename K::T *>(p.min) + cell, K::key((double)v));
+        if (p.max) atomicMax(reinterpret_cast<typename K::T *>(p.max) + cell, K::key((double)v));
     }
 }
 
-inline Planes planes_of(const dsx_accums *a) {
-    Planes p;
-    p.count = a->count; p.countv = a->countv;
-    p.sum_hi = a->sum_hi; p.sum_lo = a->sum_lo; p.sum64 = a->sum64;
-    p.min32 = a->min32; p.max32 = a->max32; p.min64 = a->min64; p.max64 = a->max64;
-    return p;
-}
+// Fill n elements of a plane with a kernel (dsx_fill's modes); status as dsx_fill.
+int fill_async(void *plane, int32_t mode, int64_t n, hipStream_t stream);
 
 // Number of workgroups for a grid-stride launch over n items.
 int grid_for(int64_t n, int block, int per_cu = 8);

@@ -180,7 +180,7 @@ static void launch_lines(const LineArgs &a, const Planes &p, int32_t vdt, int gr
 
 using namespace dsx;
 
-extern "C" int dsx_lines_agg(const dsx_lines_desc *d, const dsx_view *v, const dsx_accums *acc,
+extern "C" int dsx_lines_agg(const dsx_lines_desc *d, const dsx_view *v, const dsx_planes *acc,
                              void *stream) {
     if (!d || !v || !acc) return DSX_ERR_NULL;
     if (d->nlines < 0 || d->nverts < 0 || v->width <= 0 || v->height <= 0) return DSX_ERR_SHAPE;
@@ -191,10 +191,8 @@ extern "C" int dsx_lines_agg(const dsx_lines_desc *d, const dsx_view *v, const d
     if (d->val && d->val_dtype != DSX_F32 && d->val_dtype != DSX_F64) return DSX_ERR_DTYPE;
     const int cst = check_cat(d->cat);
     if (cst) return cst;
-    const bool needs_val = acc->countv || acc->sum_hi || acc->sum64 || acc->min32 ||
-                           acc->max32 || acc->min64 || acc->max64;
+    const bool needs_val = acc->countv || acc->sum || acc->min || acc->max;
     if (needs_val && !d->val) return DSX_ERR_ACCUM;
-    if ((acc->sum_hi != nullptr) != (acc->sum_lo != nullptr)) return DSX_ERR_ACCUM;
     if (d->nlines == 0 || d->nverts < 2) return DSX_OK;
     if (!d->x || !d->y) return DSX_ERR_NULL;
     LineArgs a;

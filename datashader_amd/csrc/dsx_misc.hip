@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void fill_kernel(T *p, T value, int64_t n) {
 // ---- finalize -----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void finalize_kernel(dsx_finalize_desc f) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const bool f32 = f.val_dtype == DSX_F32;
+    const bool f32 = f.key_bytes == 4;
     double *out = reinterpret_cast<double *>(f.out);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.ncells; i += stride) {
         switch (f.mode) {
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(dsx_finalize_desc f) {
         }
         default: {
             const uint32_t c = f.countv[i];
-            const double s = f32 ? (double)f.hi[i] + (double)f.lo[i] : f.v64[i];
+            const double s = f.sum[i];
             double r;
             switch (f.mode) {
             case DSX_FIN_SUM: r = c > 0u ? s : (double)NAN; break;
@@ -172,12 +172,12 @@ extern "C" int dsx_bounds(const void *col, int32_t dtype, int64_t n, double *out
     return launch_status();
 }
 
-extern "C" int dsx_fill(void *plane, int32_t mode, int64_t ncells, void *stream) {
+namespace dsx {
+int fill_async(void *plane, int32_t mode, int64_t ncells, hipStream_t s) {
     if (ncells < 0) return DSX_ERR_SHAPE;
     if (ncells == 0) return DSX_OK;
     if (!plane) return DSX_ERR_NULL;
     const int grid = grid_for(ncells, 256, 4);
-    hipStream_t s = as_stream(stream);
     switch (mode) {
     case DSX_FILL_ZERO32: fill_kernel<uint32_t><<<grid, 256, 0, s>>>((uint32_t *)plane, 0u, ncells); break;
     case DSX_FILL_ONES32: fill_kernel<uint32_t><<<grid, 256, 0, s>>>((uint32_t *)plane, 0xFFFFFFFFu, ncells); break;
@@ -186,6 +186,11 @@ extern "C" int dsx_fill(void *plane, int32_t mode, int64_t ncells, void *stream)
     default: return DSX_ERR_MODE;
     }
     return launch_status();
+}
+}  // namespace dsx
+
+extern "C" int dsx_fill(void *plane, int32_t mode, int64_t ncells, void *stream) {
+    return fill_async(plane, mode, ncells, as_stream(stream));
 }
 
 extern "C" int dsx_finalize(const dsx_finalize_desc *f, void *stream) {
@@ -196,12 +201,9 @@ extern "C" int dsx_finalize(const dsx_finalize_desc *f, void *stream) {
         if (!f->count) return DSX_ERR_NULL;
     } else if (f->mode == DSX_FIN_MIN || f->mode == DSX_FIN_MAX) {
         if (!f->key) return DSX_ERR_NULL;
-        if (f->val_dtype != DSX_F32 && f->val_dtype != DSX_F64) return DSX_ERR_DTYPE;
+        if (f->key_bytes != 4 && f->key_bytes != 8) return DSX_ERR_DTYPE;
     } else {
-        if (f->val_dtype != DSX_F32 && f->val_dtype != DSX_F64) return DSX_ERR_DTYPE;
-        if (!f->countv) return DSX_ERR_NULL;
-        if (f->val_dtype == DSX_F32 && (!f->hi || !f->lo)) return DSX_ERR_NULL;
-        if (f->val_dtype == DSX_F64 && !f->v64) return DSX_ERR_NULL;
+        if (!f->countv || !f->sum) return DSX_ERR_NULL;
     }
     if (f->ncells == 0) return DSX_OK;
     const int grid = grid_for(f->ncells, 256, 4);

@@ -1,12 +1,45 @@
-// dsx_points.hip -- point projection + per-bin aggregation (glyphs/points.py:188-221).
+// dsx_points.hip -- point projection + per-bin aggregation (glyphs/points.py:188-221)
+// as a tiled LDS engine for gfx950.
 //
-// One pass streams the x / y (/ value / category) columns once.  Each row is tested
-// against the inclusive bounds, projected with the reference's exact float64
-// arithmetic, and every requested accumulator plane is updated with a device-scope
-// atomic at its bin.
+// Device-scope global atomics run at ~27 G/s on this part while LDS atomics run at
+// ~780 G/s (u32 and f64 alike), so no per-row update touches a global atomic:
+//
+//   select   a sampled coarse density histogram picks the hot rectangle ("window")
+//            of the canvas whose aggregate planes fit the 160 KB LDS of a workgroup
+//   scan     one streaming pass over the columns: rows that land in the window update
+//            the workgroup's LDS copy of the window; every other kept row is compacted
+//            (wave ballots into an LDS staging area, one global reservation per 4096-row
+//            chunk) into a spill list of (cell, value)
+//   merge    the per-workgroup window partials are summed cell by cell in a fixed order
+//            and written to the output planes (plain stores)
+//   hist / offsets / scatter
+//            the spill list is counting-sorted by canvas tile (a tile is a run of cells
+//            whose planes fit LDS); each partition workgroup owns a fixed slice of the
+//            list, so every entry's destination is known without atomics
+//   tiles    one workgroup per tile accumulates its spill entries in LDS and writes
+//            every cell of the tile outside the window (plain stores)
+//
+// Every output cell is written exactly once; counts, min and max are exact; sums are
+// accumulated in float64 (LDS ds_add_f64), so only their last-bit rounding depends on
+// the order of the LDS atomics.
+#include <math.h>
+
 #include "dsx_common.h"
 
 namespace dsx {
+
+constexpr int kScanThreads = 1024;
+constexpr int kItems = 4;
+constexpr int kChunk = kScanThreads * kItems;  // rows per workgroup iteration
+constexpr int kLdsBytes = 160 * 1024;           // LDS of one gfx950 workgroup
+constexpr int kCtrlBytes = 64;
+constexpr int kPartThreads = 1024;
+constexpr int kPartGroups = 512;                // workgroups of the partition passes
+constexpr int kSubChunk = 4096;                 // spill entries sorted per LDS round
+constexpr int kMaxTiles = 4096;
+constexpr int kSelBins = 8192;                  // coarse density histogram bins
+constexpr int kSampleRuns = 256;
+constexpr int kSampleRun = 4096;
 
 struct PointArgs {
     int64_t n;
@@ -16,59 +49,744 @@ struct PointArgs {
     double sx, tx, sy, ty, xmin, xmax, ymin, ymax;
 };
 
-// Bin (cell index) of row i, or -1 when the row is dropped.
+// A value column is present when its pointer is set; an empty input (whose column has
+// no storage) is identified by its dtype alone.
+static bool has_value(const dsx_points_desc *d) {
+    return d->val != nullptr ||
+           (d->n == 0 && (d->val_dtype == DSX_F32 || d->val_dtype == DSX_F64));
+}
+
+// Internal plane: rows whose value is NaN.  When both count() and count(v) are asked
+// for, the kernels keep (count, nan) instead of (count, countv): a row with a value
+// then costs one counter update instead of two, and countv = count - nan on output.
+constexpr uint32_t PL_NAN = 32;
+
+static uint32_t internal_flags(uint32_t requested) {
+    if ((requested & PL_CNT) && (requested & PL_CNTV)) return (requested & ~PL_CNTV) | PL_NAN;
+    return requested;
+}
+
+// Byte layout of the planes of `cap` cells in one LDS region (or one partial buffer):
+// f64 sum first, then keys, then u32 counts, so every plane is naturally aligned.
+struct Lay {
+    uint32_t flags, cap, o_sum, o_min, o_max, o_cnt, o_cntv, o_nan, bytes;
+};
+
+static uint32_t cell_bytes(uint32_t flags, int keyb) {
+    return ((flags & PL_SUM) ? 8u : 0u) + ((flags & PL_MIN) ? keyb : 0) +
+           ((flags & PL_MAX) ? keyb : 0) + ((flags & PL_CNT) ? 4u : 0u) +
+           ((flags & PL_CNTV) ? 4u : 0u) + ((flags & PL_NAN) ? 4u : 0u);
+}
+
+static Lay make_lay(uint32_t flags, int keyb, uint32_t cap) {
+    Lay L;
+    L.flags = flags;
+    L.cap = cap;
+    uint32_t o = 0;
+    L.o_sum = o; o += (flags & PL_SUM) ? 8u * cap : 0u;
+    L.o_min = o; o += (flags & PL_MIN) ? (uint32_t)keyb * cap : 0u;
+    L.o_max = o; o += (flags & PL_MAX) ? (uint32_t)keyb * cap : 0u;
+    L.o_cnt = o; o += (flags & PL_CNT) ? 4u * cap : 0u;
+    L.o_cntv = o; o += (flags & PL_CNTV) ? 4u * cap : 0u;
+    L.o_nan = o; o += (flags & PL_NAN) ? 4u * cap : 0u;
+    L.bytes = (o + 15u) & ~15u;
+    return L;
+}
+
+// ---- projection -----------------------------------------------------------------------
+// Pixel and category of row i; false when the row is dropped.
 template <typename TC>
-__device__ __forceinline__ int64_t point_cell(const PointArgs &a, int64_t i) {
-    const double x = (double)reinterpret_cast<const TC *>(a.x)[i];
-    const double y = (double)reinterpret_cast<const TC *>(a.y)[i];
-    if (!((a.xmin <= x && x <= a.xmax) && (a.ymin <= y && y <= a.ymax))) return -1;
+__device__ __forceinline__ bool locate(const PointArgs &a, int64_t i, TC xc, TC yc,
+                                       int32_t &px, int32_t &py, int32_t &pc) {
+    const double x = (double)xc, y = (double)yc;
+    if (!((a.xmin <= x && x <= a.xmax) && (a.ymin <= y && y <= a.ymax))) return false;
     int32_t xx = proj(axis_map(x, a.xlog), a.sx, a.tx);
     int32_t yy = proj(axis_map(y, a.ylog), a.sy, a.ty);
     xx = xx >= a.W ? a.W - 1 : xx;
     yy = yy >= a.H ? a.H - 1 : yy;
     if (xx < 0) xx += a.W;  // numba's negative-index wraparound
     if (yy < 0) yy += a.H;
-    if ((uint32_t)xx >= (uint32_t)a.W || (uint32_t)yy >= (uint32_t)a.H) return -1;
+    if ((uint32_t)xx >= (uint32_t)a.W || (uint32_t)yy >= (uint32_t)a.H) return false;
     const int64_t c = category(a.cat, i);
-    if (c < 0) return -1;
-    return ((int64_t)yy * a.W + xx) * a.cat.ncat + c;
+    if (c < 0) return false;
+    px = xx; py = yy; pc = (int32_t)c;
+    return true;
 }
 
-template <typename TC, typename TV, bool HAS_VAL>
-__global__ __launch_bounds__(256) void points_agg_kernel(PointArgs a, Planes p) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
-        const int64_t cell = point_cell<TC>(a, i);
-        if (cell < 0) continue;
-        TV v = TV(0);
-        if constexpr (HAS_VAL) v = reinterpret_cast<const TV *>(a.val)[i];
-        append<TV, HAS_VAL>(p, cell, v);
+// ---- plane regions (LDS or global) --------------------------------------------------------
+template <int KEYB>
+__device__ __forceinline__ void region_init(uint8_t *r, const Lay &L, int tid, int nt) {
+    using K = KeyOf<KEYB>;
+    for (uint32_t c = tid; c < L.cap; c += nt) {
+        if (L.flags & PL_SUM) reinterpret_cast<double *>(r + L.o_sum)[c] = 0.0;
+        if (L.flags & PL_MIN) reinterpret_cast<typename K::T *>(r + L.o_min)[c] = K::kMinEmpty;
+        if (L.flags & PL_MAX) reinterpret_cast<typename K::T *>(r + L.o_max)[c] = 0;
+        if (L.flags & PL_CNT) reinterpret_cast<uint32_t *>(r + L.o_cnt)[c] = 0u;
+        if (L.flags & PL_CNTV) reinterpret_cast<uint32_t *>(r + L.o_cntv)[c] = 0u;
+        if (L.flags & PL_NAN) reinterpret_cast<uint32_t *>(r + L.o_nan)[c] = 0u;
     }
 }
 
-// Squared deviations from the per-bin mean, for var / std.
-template <typename TC, typename TV>
-__global__ __launch_bounds__(256) void points_m2_kernel(PointArgs a, const double *mean,
-                                                        float *hi, float *lo, double *m64) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
-        const int64_t cell = point_cell<TC>(a, i);
-        if (cell < 0) continue;
-        const TV v = reinterpret_cast<const TV *>(a.val)[i];
-        if (v != v) continue;
-        const double d = (double)v - mean[cell];
-        if constexpr (sizeof(TV) == 4) {
-            const float df = (float)d;
-            add_comp(hi + cell, lo + cell, df * df);
-        } else {
-            unsafeAtomicAdd(m64 + cell, d * d);
+// LDS atomic update of cell c by one row with value v (vnan: the value is NaN / absent
+// for value-reading planes).  `add` is the quantity summed (v, or its squared deviation).
+template <int KEYB>
+__device__ __forceinline__ void region_add(uint8_t *r, const Lay &L, uint32_t c, bool vnan,
+                                           double v, double add) {
+    using K = KeyOf<KEYB>;
+    if (L.flags & PL_CNT) atomicAdd(reinterpret_cast<uint32_t *>(r + L.o_cnt) + c, 1u);
+    if (vnan) {
+        if (L.flags & PL_NAN) atomicAdd(reinterpret_cast<uint32_t *>(r + L.o_nan) + c, 1u);
+        return;
+    }
+    if (L.flags & PL_CNTV) atomicAdd(reinterpret_cast<uint32_t *>(r + L.o_cntv) + c, 1u);
+    if (L.flags & PL_SUM) atomicAdd(reinterpret_cast<double *>(r + L.o_sum) + c, add);
+    if (L.flags & PL_MIN) atomicMin(reinterpret_cast<typename K::T *>(r + L.o_min) + c, K::key(v));
+    if (L.flags & PL_MAX) atomicMax(reinterpret_cast<typename K::T *>(r + L.o_max) + c, K::key(v));
+}
+
+// Copy LDS region cell c into the output planes at global cell g.
+template <int KEYB>
+__device__ __forceinline__ void region_store(const uint8_t *r, const Lay &L, uint32_t c,
+                                             const Planes &out, int64_t g) {
+    using K = KeyOf<KEYB>;
+    if (L.flags & PL_SUM) out.sum[g] = reinterpret_cast<const double *>(r + L.o_sum)[c];
+    if (L.flags & PL_MIN)
+        reinterpret_cast<typename K::T *>(out.min)[g] = reinterpret_cast<const typename K::T *>(r + L.o_min)[c];
+    if (L.flags & PL_MAX)
+        reinterpret_cast<typename K::T *>(out.max)[g] = reinterpret_cast<const typename K::T *>(r + L.o_max)[c];
+    if (L.flags & PL_CNT) {
+        const uint32_t n = reinterpret_cast<const uint32_t *>(r + L.o_cnt)[c];
+        if (out.count) out.count[g] = n;
+        if (L.flags & PL_NAN) out.countv[g] = n - reinterpret_cast<const uint32_t *>(r + L.o_nan)[c];
+    }
+    if (L.flags & PL_CNTV) out.countv[g] = reinterpret_cast<const uint32_t *>(r + L.o_cntv)[c];
+}
+
+// ---- window -------------------------------------------------------------------------------
+struct Window {
+    int32_t x0, y0, w, h;
+};
+
+// The window of a call, sanitised: anything outside the canvas or larger than the LDS
+// capacity (a stale or foreign hint) becomes the empty window.
+__device__ __forceinline__ Window load_window(const int32_t *p, int32_t W, int32_t H,
+                                              int64_t cap_px) {
+    Window w{p[0], p[1], p[2], p[3]};
+    if (w.x0 < 0 || w.y0 < 0 || w.w < 0 || w.h < 0 || (int64_t)w.x0 + w.w > W ||
+        (int64_t)w.y0 + w.h > H || (int64_t)w.w * w.h > cap_px)
+        w = Window{0, 0, 0, 0};
+    return w;
+}
+
+__device__ __forceinline__ bool in_window(const Window &w, int32_t px, int32_t py) {
+    return (uint32_t)(px - w.x0) < (uint32_t)w.w && (uint32_t)(py - w.y0) < (uint32_t)w.h;
+}
+
+__global__ void set_window_kernel(int32_t *w, int32_t x0, int32_t y0, int32_t ww, int32_t wh) {
+    if (threadIdx.x == 0) { w[0] = x0; w[1] = y0; w[2] = ww; w[3] = wh; }
+}
+
+// Coarse density histogram of a sample of rows: kSampleRuns runs of kSampleRun
+// consecutive rows spread evenly over the input; bins of (1 << bshift)^2 pixels.
+template <typename TC>
+__global__ __launch_bounds__(1024) void select_hist_kernel(PointArgs a, int32_t bshift,
+                                                           int32_t cw, int32_t ch,
+                                                           uint32_t *hist) {
+    __shared__ uint32_t h[kSelBins];
+    for (int b = threadIdx.x; b < kSelBins; b += blockDim.x) h[b] = 0u;
+    __syncthreads();
+    const TC *X = reinterpret_cast<const TC *>(a.x);
+    const TC *Y = reinterpret_cast<const TC *>(a.y);
+    for (int run = blockIdx.x; run < kSampleRuns; run += gridDim.x) {
+        const int64_t start = (int64_t)((double)run * (double)a.n / kSampleRuns);
+        for (int k = threadIdx.x; k < kSampleRun; k += blockDim.x) {
+            const int64_t i = start + k;
+            if (i >= a.n) break;
+            int32_t px, py, pc;
+            if (!locate<TC>(a, i, X[i], Y[i], px, py, pc)) continue;
+            const int32_t bx = px >> bshift, by = py >> bshift;
+            if (bx < cw && by < ch) atomicAdd(&h[by * cw + bx], 1u);
         }
     }
+    __syncthreads();
+    for (int b = threadIdx.x; b < cw * ch; b += blockDim.x)
+        if (h[b]) atomicAdd(hist + b, h[b]);
+}
+
+// Pick the window: for a few aspect ratios, the position (in coarse bins) of the
+// rectangle of at most cap_px pixels covering the most sampled rows.
+__global__ __launch_bounds__(1024) void select_pick_kernel(const uint32_t *hist, int32_t bshift,
+                                                           int32_t cw, int32_t ch, int32_t W,
+                                                           int32_t H, int64_t cap_px,
+                                                           int32_t *win) {
+    // (cw+1) x (ch+1) inclusive prefix sums; cw*ch <= kSelBins, so cw + ch <= kSelBins + 1
+    __shared__ uint32_t ps[2 * kSelBins + 2];
+    __shared__ unsigned long long best[1024 / 64];
+    const int pw = cw + 1;
+    for (int i = threadIdx.x; i < pw * (ch + 1); i += blockDim.x) ps[i] = 0u;
+    __syncthreads();
+    for (int i = threadIdx.x; i < cw * ch; i += blockDim.x)
+        ps[(i / cw + 1) * pw + (i % cw) + 1] = hist[i];
+    __syncthreads();
+    for (int y = threadIdx.x + 1; y <= ch; y += blockDim.x)  // rows, in parallel
+        for (int x = 1; x <= cw; ++x) ps[y * pw + x] += ps[y * pw + x - 1];
+    __syncthreads();
+    for (int x = threadIdx.x + 1; x <= cw; x += blockDim.x)  // then columns
+        for (int y = 1; y <= ch; ++y) ps[y * pw + x] += ps[(y - 1) * pw + x];
+    __syncthreads();
+    const int64_t b = 1ll << bshift;
+    unsigned long long mybest = 0ull;  // (count << 32) | (0xFFFFFFFF - encoded choice)
+    const float aspects[5] = {0.25f, 0.5f, 1.0f, 2.0f, 4.0f};
+    for (int s = 0; s < 5; ++s) {
+        int kh = (int)sqrtf((float)cap_px / (aspects[s] * (float)(b * b)));
+        if (kh < 1) kh = 1;
+        if (kh > ch) kh = ch;
+        int kw = (int)(cap_px / ((int64_t)kh * b * b));
+        if (kw < 1) continue;
+        if (kw > cw) kw = cw;
+        const int nx = cw - kw + 1, ny = ch - kh + 1;
+        for (int p = threadIdx.x; p < nx * ny; p += blockDim.x) {
+            const int x0 = p % nx, y0 = p / nx;
+            const uint32_t cnt = ps[(y0 + kh) * pw + (x0 + kw)] - ps[y0 * pw + (x0 + kw)] -
+                                 ps[(y0 + kh) * pw + x0] + ps[y0 * pw + x0];
+            const uint32_t code = ((uint32_t)s << 28) | ((uint32_t)y0 << 14) | (uint32_t)x0;
+            const unsigned long long key = ((unsigned long long)cnt << 32) | (0xFFFFFFFFu - code);
+            if (key > mybest) mybest = key;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(mybest, off, 64);
+        mybest = o > mybest ? o : mybest;
+    }
+    if ((threadIdx.x & 63) == 0) best[threadIdx.x >> 6] = mybest;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long k = 0ull;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) k = best[w] > k ? best[w] : k;
+        if (k == 0ull) {  // no shape fits a coarse bin: no window, every row spills
+            win[0] = 0; win[1] = 0; win[2] = 0; win[3] = 0;
+            return;
+        }
+        const uint32_t code = 0xFFFFFFFFu - (uint32_t)(k & 0xFFFFFFFFu);
+        const int s = (int)(code >> 28), y0 = (int)((code >> 14) & 0x3FFF), x0 = (int)(code & 0x3FFF);
+        int kh = (int)sqrtf((float)cap_px / (aspects[s] * (float)(b * b)));
+        if (kh < 1) kh = 1;
+        if (kh > ch) kh = ch;
+        int kw = (int)(cap_px / ((int64_t)kh * b * b));
+        if (kw > cw) kw = cw;
+        if (kw < 1) kw = 1;
+        const int px0 = (int)(x0 * b), py0 = (int)(y0 * b);
+        int ww = (int)(kw * b), wh = (int)(kh * b);
+        if (px0 + ww > W) ww = W - px0;
+        if (py0 + wh > H) wh = H - py0;
+        if ((int64_t)ww * wh > cap_px || ww < 1 || wh < 1) { ww = 0; wh = 0; }
+        win[0] = px0; win[1] = py0; win[2] = ww; win[3] = wh;
+    }
+}
+
+// ---- scan ------------------------------------------------------------------------------------
+struct ScanArgs {
+    Lay wl;                          // window planes (cap = window capacity in cells)
+    const int32_t *window;           // device [x0, y0, w, h]
+    uint8_t *partials;               // gridDim.x regions of wl.bytes
+    uint32_t *spill_cell;            // spill list
+    void *spill_val;
+    unsigned long long *spill_count; // entries appended (device, starts at 0)
+    int64_t spill_cap;
+    const double *mean;              // squared-deviation mode (m2 pass)
+};
+
+template <typename TC, typename TV, bool HAS_VAL>
+__device__ __forceinline__ void load_chunk(const PointArgs &a, int64_t ck, int tid, TC *xs,
+                                           TC *ys, TV *vs) {
+    const TC *X = reinterpret_cast<const TC *>(a.x);
+    const TC *Y = reinterpret_cast<const TC *>(a.y);
+    const TV *V = reinterpret_cast<const TV *>(a.val);
+    const int64_t base = ck * kChunk;
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        const int64_t i = base + it * kScanThreads + tid;
+        xs[it] = TC(0);
+        ys[it] = TC(0);
+        vs[it] = TV(0);
+        if (i < a.n) {
+            xs[it] = X[i];
+            ys[it] = Y[i];
+            if constexpr (HAS_VAL) vs[it] = V[i];
+        }
+    }
+}
+
+template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
+__global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArgs s) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    uint8_t *win = smem;
+    uint32_t *stg_cell = reinterpret_cast<uint32_t *>(smem + s.wl.bytes);
+    TV *stg_val = reinterpret_cast<TV *>(stg_cell + kChunk);
+    uint32_t *ctrl = reinterpret_cast<uint32_t *>(
+        reinterpret_cast<uint8_t *>(stg_val) + (HAS_VAL ? kChunk * sizeof(TV) : 0));
+    unsigned long long *gbase = reinterpret_cast<unsigned long long *>(ctrl + 2);
+    const int tid = threadIdx.x;
+    const int ncat = a.cat.ncat;
+    const Window w = load_window(s.window, a.W, a.H, (int64_t)(s.wl.cap / (uint32_t)ncat));
+    region_init<KEYB>(win, s.wl, tid, kScanThreads);
+    if (tid == 0) ctrl[0] = 0u;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
+    // software pipeline: the next chunk's loads are in flight while this one is binned
+    TC xa[kItems], ya[kItems];
+    TV va[kItems];
+    load_chunk<TC, TV, HAS_VAL>(a, blockIdx.x, tid, xa, ya, va);
+    for (int64_t ck = blockIdx.x; ck < nchunks; ck += gridDim.x) {
+        TC xb[kItems], yb[kItems];
+        TV vb[kItems];
+        load_chunk<TC, TV, HAS_VAL>(a, ck + gridDim.x, tid, xb, yb, vb);
+        const int64_t base = ck * kChunk;
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            const int64_t i = base + it * kScanThreads + tid;
+            int32_t px = 0, py = 0, pc = 0;
+            const bool keep = i < a.n && locate<TC>(a, i, xa[it], ya[it], px, py, pc);
+            const bool inw = keep && in_window(w, px, py);
+            const double v = HAS_VAL ? (double)va[it] : 0.0;
+            const bool vnan = HAS_VAL ? (v != v) : true;
+            if (inw) {
+                const uint32_t lc = (uint32_t)(((py - w.y0) * w.w + (px - w.x0)) * ncat + pc);
+                double add = v;
+                if constexpr (SQDEV) {
+                    const int64_t gc = ((int64_t)py * a.W + px) * ncat + pc;
+                    const double d = v - s.mean[gc];
+                    add = d * d;
+                }
+                region_add<KEYB>(win, s.wl, lc, vnan, v, add);
+            }
+            const bool spill = keep && !inw;
+            const unsigned long long m = __ballot(spill);
+            if (m) {
+                const uint32_t nw = (uint32_t)__popcll(m);
+                const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                uint32_t b0 = 0;
+                if (lane == 0) b0 = atomicAdd(&ctrl[0], nw);
+                b0 = __shfl(b0, 0, 64);
+                if (spill) {
+                    stg_cell[b0 + rank] = (uint32_t)(((int64_t)py * a.W + px) * ncat + pc);
+                    if constexpr (HAS_VAL) stg_val[b0 + rank] = va[it];
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t staged = ctrl[0];
+        if (staged) {
+            if (tid == 0) gbase[0] = atomicAdd(s.spill_count, (unsigned long long)staged);
+            __syncthreads();
+            const unsigned long long g0 = gbase[0];
+            for (uint32_t j = tid; j < staged; j += kScanThreads) {
+                const unsigned long long g = g0 + j;
+                if (g < (unsigned long long)s.spill_cap) {
+                    s.spill_cell[g] = stg_cell[j];
+                    if constexpr (HAS_VAL) reinterpret_cast<TV *>(s.spill_val)[g] = stg_val[j];
+                }
+            }
+            __syncthreads();
+            if (tid == 0) ctrl[0] = 0u;
+            __syncthreads();
+        }
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            xa[it] = xb[it];
+            ya[it] = yb[it];
+            va[it] = vb[it];
+        }
+    }
+    __syncthreads();
+    // this workgroup's window partial
+    uint4 *dst = reinterpret_cast<uint4 *>(s.partials + (size_t)blockIdx.x * s.wl.bytes);
+    const uint4 *src = reinterpret_cast<const uint4 *>(win);
+    for (uint32_t j = tid; j < s.wl.bytes / 16u; j += kScanThreads) dst[j] = src[j];
+}
+
+// ---- merge the window partials --------------------------------------------------------------
+// A block covers kMergeCells cells; its kMergeWays thread rows each sum a strided subset
+// of the partials, then the rows are combined in a fixed tree order (deterministic).
+constexpr int kMergeCells = 64;
+constexpr int kMergeWays = 16;
+
+template <int KEYB>
+__global__ __launch_bounds__(kMergeCells * kMergeWays) void merge_window_kernel(
+    Lay L, const uint8_t *partials, int nparts, const int32_t *window, int32_t W, int32_t H,
+    int32_t ncat, Planes out) {
+    using K = KeyOf<KEYB>;
+    __shared__ double s_sum[kMergeWays][kMergeCells];
+    __shared__ typename K::T s_min[kMergeWays][kMergeCells];
+    __shared__ typename K::T s_max[kMergeWays][kMergeCells];
+    __shared__ uint32_t s_cnt[kMergeWays][kMergeCells];
+    __shared__ uint32_t s_cntv[kMergeWays][kMergeCells];
+    __shared__ uint32_t s_nan[kMergeWays][kMergeCells];
+    const Window w = load_window(window, W, H, (int64_t)(L.cap / (uint32_t)ncat));
+    const uint32_t ncells = (uint32_t)(w.w * w.h * ncat);
+    const int cl = threadIdx.x % kMergeCells, way = threadIdx.x / kMergeCells;
+    const uint32_t c = blockIdx.x * kMergeCells + cl;
+    const bool live = c < ncells && c < L.cap;
+    double sum = 0.0;
+    typename K::T mn = K::kMinEmpty, mx = 0;
+    uint32_t cnt = 0u, cntv = 0u, nan = 0u;
+    if (live) {
+        for (int g = way; g < nparts; g += kMergeWays) {
+            const uint8_t *r = partials + (size_t)g * L.bytes;
+            if (L.flags & PL_SUM) sum += reinterpret_cast<const double *>(r + L.o_sum)[c];
+            if (L.flags & PL_MIN) {
+                const typename K::T k = reinterpret_cast<const typename K::T *>(r + L.o_min)[c];
+                mn = k < mn ? k : mn;
+            }
+            if (L.flags & PL_MAX) {
+                const typename K::T k = reinterpret_cast<const typename K::T *>(r + L.o_max)[c];
+                mx = k > mx ? k : mx;
+            }
+            if (L.flags & PL_CNT) cnt += reinterpret_cast<const uint32_t *>(r + L.o_cnt)[c];
+            if (L.flags & PL_CNTV) cntv += reinterpret_cast<const uint32_t *>(r + L.o_cntv)[c];
+            if (L.flags & PL_NAN) nan += reinterpret_cast<const uint32_t *>(r + L.o_nan)[c];
+        }
+    }
+    s_sum[way][cl] = sum; s_min[way][cl] = mn; s_max[way][cl] = mx;
+    s_cnt[way][cl] = cnt; s_cntv[way][cl] = cntv; s_nan[way][cl] = nan;
+    __syncthreads();
+    for (int half = kMergeWays / 2; half > 0; half >>= 1) {
+        if (way < half) {
+            s_sum[way][cl] += s_sum[way + half][cl];
+            s_min[way][cl] = s_min[way + half][cl] < s_min[way][cl] ? s_min[way + half][cl] : s_min[way][cl];
+            s_max[way][cl] = s_max[way + half][cl] > s_max[way][cl] ? s_max[way + half][cl] : s_max[way][cl];
+            s_cnt[way][cl] += s_cnt[way + half][cl];
+            s_cntv[way][cl] += s_cntv[way + half][cl];
+            s_nan[way][cl] += s_nan[way + half][cl];
+        }
+        __syncthreads();
+    }
+    if (way != 0 || !live) return;
+    const uint32_t pix = c / (uint32_t)ncat, cat = c - pix * (uint32_t)ncat;
+    const int32_t ly = (int32_t)(pix / (uint32_t)w.w), lx = (int32_t)(pix - (uint32_t)ly * w.w);
+    const int64_t g = ((int64_t)(w.y0 + ly) * W + (w.x0 + lx)) * ncat + cat;
+    if (L.flags & PL_SUM) out.sum[g] = s_sum[0][cl];
+    if (L.flags & PL_MIN) reinterpret_cast<typename K::T *>(out.min)[g] = s_min[0][cl];
+    if (L.flags & PL_MAX) reinterpret_cast<typename K::T *>(out.max)[g] = s_max[0][cl];
+    if (L.flags & PL_CNT) {
+        if (out.count) out.count[g] = s_cnt[0][cl];
+        if (L.flags & PL_NAN) out.countv[g] = s_cnt[0][cl] - s_nan[0][cl];
+    }
+    if (L.flags & PL_CNTV) out.countv[g] = s_cntv[0][cl];
+}
+
+// ---- partition the spill list by tile --------------------------------------------------------
+struct PartArgs {
+    const uint32_t *cell;
+    const void *val;
+    const unsigned long long *count;  // spill entries
+    uint32_t tile_cells;
+    int32_t ntiles;
+    uint32_t *hist;                   // [kPartGroups][ntiles]
+    unsigned long long *tile_start;   // [ntiles + 1]
+    uint32_t *sorted_cell;
+    void *sorted_val;
+};
+
+__device__ __forceinline__ void slice_of(unsigned long long m, int g, unsigned long long &b,
+                                         unsigned long long &e) {
+    const unsigned long long per = (m + kPartGroups - 1) / kPartGroups;
+    b = per * (unsigned long long)g;
+    e = b + per;
+    if (b > m) b = m;
+    if (e > m) e = m;
+}
+
+__global__ __launch_bounds__(kPartThreads) void spill_hist_kernel(PartArgs p) {
+    __shared__ uint32_t h[kMaxTiles];
+    for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x) h[t] = 0u;
+    __syncthreads();
+    unsigned long long b, e;
+    slice_of(*p.count, blockIdx.x, b, e);
+    for (unsigned long long j = b + threadIdx.x; j < e; j += blockDim.x)
+        atomicAdd(&h[p.cell[j] / p.tile_cells], 1u);
+    __syncthreads();
+    for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x)
+        p.hist[(size_t)blockIdx.x * p.ntiles + t] = h[t];
+}
+
+// Block-wide exclusive scan of one value per thread (1024 threads).
+__device__ __forceinline__ unsigned long long block_exscan(unsigned long long v,
+                                                           unsigned long long *sh,
+                                                           unsigned long long &total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned long long x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+            const unsigned long long t = sh[k];
+            sh[k] = run;
+            run += t;
+        }
+        sh[blockDim.x >> 6] = run;
+    }
+    __syncthreads();
+    total = sh[blockDim.x >> 6];
+    const unsigned long long r = sh[wv] + x - v;
+    __syncthreads();
+    return r;
+}
+
+// Per tile (one workgroup each): exclusive prefix of the partition groups' counts, in
+// place, and the tile total.
+__global__ __launch_bounds__(kPartThreads) void spill_colscan_kernel(PartArgs p,
+                                                                    unsigned long long *totals) {
+    __shared__ unsigned long long sh[kPartThreads / 64 + 1];
+    const int t = blockIdx.x;
+    const int g = threadIdx.x;  // kPartGroups <= kPartThreads
+    const unsigned long long v = g < kPartGroups ? p.hist[(size_t)g * p.ntiles + t] : 0ull;
+    unsigned long long tot;
+    const unsigned long long ex = block_exscan(v, sh, tot);
+    if (g < kPartGroups) p.hist[(size_t)g * p.ntiles + t] = (uint32_t)ex;
+    if (g == 0) totals[t] = tot;
+}
+
+__global__ __launch_bounds__(kPartThreads) void spill_tilescan_kernel(PartArgs p,
+                                                                     const unsigned long long *totals) {
+    __shared__ unsigned long long sh[kPartThreads / 64 + 1];
+    unsigned long long carry = 0;
+    for (int t0 = 0; t0 < p.ntiles; t0 += blockDim.x) {
+        const int t = t0 + threadIdx.x;
+        const unsigned long long v = t < p.ntiles ? totals[t] : 0ull;
+        unsigned long long tot;
+        const unsigned long long ex = block_exscan(v, sh, tot);
+        if (t < p.ntiles) p.tile_start[t] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) p.tile_start[p.ntiles] = carry;
+}
+
+// Scatter: each partition group walks its slice in sub-chunks, counting-sorts each
+// sub-chunk by tile in LDS and writes it out run by run (coalesced within runs).
+template <typename TV, bool HAS_VAL>
+__global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p) {
+    __shared__ uint32_t cur[kMaxTiles];    // next slot of this group in each tile
+    __shared__ uint32_t cnt[kMaxTiles];    // sub-chunk counts, then sub-chunk starts
+    __shared__ uint32_t s_cell[kSubChunk];
+    __shared__ uint16_t s_tile[kSubChunk];
+    __shared__ TV s_val[HAS_VAL ? kSubChunk : 1];
+    __shared__ unsigned long long sh[kPartThreads / 64 + 1];
+    const int g = blockIdx.x;
+    for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x) {
+        cur[t] = p.hist[(size_t)g * p.ntiles + t];
+        cnt[t] = 0u;
+    }
+    __syncthreads();
+    unsigned long long b, e;
+    slice_of(*p.count, g, b, e);
+    const TV *val = reinterpret_cast<const TV *>(p.val);
+    constexpr int kPer = kSubChunk / kPartThreads;
+    for (unsigned long long c0 = b; c0 < e; c0 += kSubChunk) {
+        uint32_t rc[kPer], rt[kPer], rr[kPer];
+        TV rv[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const unsigned long long j = c0 + (unsigned long long)(k * kPartThreads + threadIdx.x);
+            rt[k] = 0xFFFFFFFFu;
+            rc[k] = 0u;
+            rr[k] = 0u;
+            rv[k] = TV(0);
+            if (j < e) {
+                rc[k] = p.cell[j];
+                if constexpr (HAS_VAL) rv[k] = val[j];
+                rt[k] = rc[k] / p.tile_cells;
+                rr[k] = atomicAdd(&cnt[rt[k]], 1u);
+            }
+        }
+        __syncthreads();
+        // exclusive scan of the sub-chunk counts -> sub-chunk start of each tile
+        unsigned long long carry = 0;
+        for (int t0 = 0; t0 < p.ntiles; t0 += blockDim.x) {
+            const int t = t0 + threadIdx.x;
+            const unsigned long long v = t < p.ntiles ? cnt[t] : 0ull;
+            unsigned long long tot;
+            const unsigned long long ex = block_exscan(v, sh, tot);
+            if (t < p.ntiles) cnt[t] = (uint32_t)(carry + ex);
+            carry += tot;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (rt[k] != 0xFFFFFFFFu) {
+                const uint32_t slot = cnt[rt[k]] + rr[k];
+                s_cell[slot] = rc[k];
+                s_tile[slot] = (uint16_t)rt[k];
+                if constexpr (HAS_VAL) s_val[slot] = rv[k];
+            }
+        }
+        __syncthreads();
+        const uint32_t nsub = (uint32_t)((e - c0) < (unsigned long long)kSubChunk ? (e - c0) : kSubChunk);
+        for (uint32_t j = threadIdx.x; j < nsub; j += blockDim.x) {
+            const uint32_t t = s_tile[j];
+            const unsigned long long dst = p.tile_start[t] + cur[t] + (j - cnt[t]);
+            p.sorted_cell[dst] = s_cell[j];
+            if constexpr (HAS_VAL) reinterpret_cast<TV *>(p.sorted_val)[dst] = s_val[j];
+        }
+        __syncthreads();
+        // advance the group's cursors by this sub-chunk's counts; reset the counts
+        for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x) {
+            const uint32_t next = (t + 1 < p.ntiles) ? cnt[t + 1] : nsub;
+            cur[t] += next - cnt[t];
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x) cnt[t] = 0u;
+        __syncthreads();
+    }
+}
+
+// ---- per-tile aggregation of the sorted spill ------------------------------------------------
+template <typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
+__global__ __launch_bounds__(kPartThreads) void tile_agg_kernel(PartArgs p, Lay L,
+                                                               const int32_t *window, int32_t W,
+                                                               int32_t H, int64_t wcap_px,
+                                                               int32_t ncat, int64_t ncells,
+                                                               const double *mean, Planes out) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    constexpr int kU = 4;  // entries in flight per thread
+    const int t = blockIdx.x;
+    region_init<KEYB>(smem, L, threadIdx.x, blockDim.x);
+    __syncthreads();
+    const unsigned long long b = p.tile_start[t], e = p.tile_start[t + 1];
+    const int64_t c0 = (int64_t)t * p.tile_cells;
+    const TV *val = reinterpret_cast<const TV *>(p.sorted_val);
+    for (unsigned long long j0 = b; j0 < e; j0 += (unsigned long long)kU * blockDim.x) {
+        uint32_t gc[kU];
+        TV vv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const unsigned long long j = j0 + (unsigned long long)(u * blockDim.x + threadIdx.x);
+            gc[u] = 0xFFFFFFFFu;
+            vv[u] = TV(0);
+            if (j < e) {
+                gc[u] = p.sorted_cell[j];
+                if constexpr (HAS_VAL) vv[u] = val[j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (gc[u] == 0xFFFFFFFFu) continue;
+            const uint32_t lc = (uint32_t)(gc[u] - c0);
+            double v = 0.0;
+            bool vnan = true;
+            if constexpr (HAS_VAL) {
+                v = (double)vv[u];
+                vnan = v != v;
+            }
+            double add = v;
+            if constexpr (SQDEV) {
+                const double d = v - mean[gc[u]];
+                add = d * d;
+            }
+            region_add<KEYB>(smem, L, lc, vnan, v, add);
+        }
+    }
+    __syncthreads();
+    const Window w = load_window(window, W, H, wcap_px);
+    int64_t c1 = c0 + p.tile_cells;
+    if (c1 > ncells) c1 = ncells;
+    for (int64_t gcell = c0 + threadIdx.x; gcell < c1; gcell += blockDim.x) {
+        const int64_t pix = gcell / ncat;
+        const int32_t py = (int32_t)(pix / W), px = (int32_t)(pix - (int64_t)py * W);
+        if (in_window(w, px, py)) continue;  // written by the window merge
+        region_store<KEYB>(smem, L, (uint32_t)(gcell - c0), out, gcell);
+    }
+}
+
+// ---- host side --------------------------------------------------------------------------------
+struct Plan {
+    uint32_t flags;
+    int keyb, vbytes;
+    Lay wl;              // window layout
+    Lay tl;              // tile layout
+    int64_t ncells;
+    int32_t ntiles;
+    int scan_groups;
+    size_t scan_lds, tile_lds;
+    // workspace offsets
+    size_t o_window, o_hist_sel, o_count, o_partials, o_spill_cell, o_spill_val, o_hist,
+        o_totals, o_tile_start, o_sorted_cell, o_sorted_val, total;
+    bool whole;          // the window is the whole canvas
+    int bshift, cw, ch;
+};
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_planes *pl, Plan &P) {
+    if (plane_flags(pl) == 0) return DSX_ERR_ACCUM;
+    P.flags = internal_flags(plane_flags(pl));
+    const bool has_val = has_value(d);
+    P.vbytes = has_val ? (d->val_dtype == DSX_F32 ? 4 : 8) : 0;
+    P.keyb = (has_val && d->val_dtype == DSX_F32) ? 4 : 8;
+    const int ncat = d->cat.kind == DSX_CAT_NONE ? 1 : d->cat.ncat;
+    P.ncells = (int64_t)v->width * v->height * ncat;
+    if (P.ncells >= (1ll << 32)) return DSX_ERR_SHAPE;
+    const uint32_t cb = cell_bytes(P.flags, P.keyb);
+    // window capacity: LDS minus the spill staging area and control words
+    const size_t stage = (size_t)kChunk * (4 + P.vbytes);
+    const int64_t wcap_cells = (int64_t)((kLdsBytes - stage - kCtrlBytes - 16) / cb);
+    int64_t wcap_px = wcap_cells / ncat;
+    if (wcap_px < 1) return DSX_ERR_SHAPE;
+    P.whole = (int64_t)v->width * v->height <= wcap_px;
+    if (P.whole) wcap_px = (int64_t)v->width * v->height;
+    P.wl = make_lay(P.flags, P.keyb, (uint32_t)(wcap_px * ncat));
+    P.scan_lds = P.wl.bytes + stage + kCtrlBytes;
+    // tiles: equal runs of cells whose planes fit LDS
+    int64_t tcap = (kLdsBytes - 16) / cb;
+    P.ntiles = (int32_t)((P.ncells + tcap - 1) / tcap);
+    if (P.ntiles > kMaxTiles) return DSX_ERR_SHAPE;
+    tcap = (P.ncells + P.ntiles - 1) / P.ntiles;
+    P.tl = make_lay(P.flags, P.keyb, (uint32_t)tcap);
+    P.tile_lds = P.tl.bytes;
+    // coarse selection grid: bins of 2^bshift pixels on a side, at most kSelBins bins
+    P.bshift = 0;
+    while ((int64_t)((v->width + (1 << P.bshift) - 1) >> P.bshift) *
+               ((v->height + (1 << P.bshift) - 1) >> P.bshift) > kSelBins)
+        ++P.bshift;
+    P.cw = (v->width + (1 << P.bshift) - 1) >> P.bshift;
+    P.ch = (v->height + (1 << P.bshift) - 1) >> P.bshift;
+    // scan workgroups: one per CU at most, fewer for small inputs
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0)
+            cus = c;
+    }
+    const int64_t chunks = (d->n + kChunk - 1) / kChunk;
+    P.scan_groups = (int)(chunks < cus ? (chunks < 1 ? 1 : chunks) : cus);
+    // workspace
+    const int64_t n = d->n < 1 ? 1 : d->n;
+    size_t o = 0;
+    P.o_window = o; o = align_up(o + 16, 256);
+    P.o_hist_sel = o; o = align_up(o + 4 * (size_t)kSelBins, 256);
+    P.o_count = o; o = align_up(o + 8, 256);
+    P.o_partials = o; o = align_up(o + (size_t)P.scan_groups * P.wl.bytes, 256);
+    P.o_spill_cell = o; o = align_up(o + 4 * (size_t)n, 256);
+    P.o_spill_val = o; o = align_up(o + (size_t)P.vbytes * n, 256);
+    P.o_hist = o; o = align_up(o + 4 * (size_t)kPartGroups * P.ntiles, 256);
+    P.o_totals = o; o = align_up(o + 8 * (size_t)P.ntiles, 256);
+    P.o_tile_start = o; o = align_up(o + 8 * (size_t)(P.ntiles + 1), 256);
+    P.o_sorted_cell = o; o = align_up(o + 4 * (size_t)n, 256);
+    P.o_sorted_val = o; o = align_up(o + (size_t)P.vbytes * n, 256);
+    P.total = o;
+    return DSX_OK;
 }
 
 static int check_points(const dsx_points_desc *d, const dsx_view *v) {
     if (!d || !v) return DSX_ERR_NULL;
     if (d->n < 0 || v->width <= 0 || v->height <= 0) return DSX_ERR_SHAPE;
+    if (d->n >= (1ll << 32)) return DSX_ERR_SHAPE;  // spill positions are 32-bit per tile
     if (d->n > 0 && (!d->x || !d->y)) return DSX_ERR_NULL;
     if (d->xy_dtype != DSX_F32 && d->xy_dtype != DSX_F64) return DSX_ERR_DTYPE;
     if (d->val && d->val_dtype != DSX_F32 && d->val_dtype != DSX_F64) return DSX_ERR_DTYPE;
@@ -84,62 +802,118 @@ static PointArgs args_of(const dsx_points_desc *d, const dsx_view *v) {
     return a;
 }
 
+template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
+static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &P,
+                      const Planes &out, const double *mean, uint8_t *ws, hipStream_t st) {
+    int32_t *window = d->window ? d->window : reinterpret_cast<int32_t *>(ws + P.o_window);
+    uint32_t *hist_sel = reinterpret_cast<uint32_t *>(ws + P.o_hist_sel);
+    unsigned long long *count = reinterpret_cast<unsigned long long *>(ws + P.o_count);
+    const int ncat = a.cat.ncat;
+    const int64_t wcap_px = (int64_t)(P.wl.cap / (uint32_t)ncat);
+    int s;
+    // counters start at zero (fill kernels, no hipMemset)
+    if ((s = fill_async(count, DSX_FILL_ZERO64, 1, st))) return s;
+    if (P.whole) {
+        hipLaunchKernelGGL(set_window_kernel, dim3(1), dim3(64), 0, st, window, 0, 0, a.W, a.H);
+    } else if (d->window && d->reuse_window) {
+        // the caller's stored window (sanitised by every kernel that reads it)
+    } else {
+        if ((s = fill_async(hist_sel, DSX_FILL_ZERO32, kSelBins, st))) return s;
+        hipLaunchKernelGGL((select_hist_kernel<TC>), dim3(64), dim3(1024), 0, st, a, P.bshift,
+                           P.cw, P.ch, hist_sel);
+        hipLaunchKernelGGL(select_pick_kernel, dim3(1), dim3(1024), 0, st, (const uint32_t *)hist_sel,
+                           P.bshift, P.cw, P.ch, a.W, a.H, (int64_t)(P.wl.cap / (uint32_t)ncat),
+                           window);
+    }
+    if ((s = launch_status())) return s;
+    ScanArgs sa;
+    sa.wl = P.wl;
+    sa.window = window;
+    sa.partials = ws + P.o_partials;
+    sa.spill_cell = reinterpret_cast<uint32_t *>(ws + P.o_spill_cell);
+    sa.spill_val = ws + P.o_spill_val;
+    sa.spill_count = count;
+    sa.spill_cap = a.n;
+    sa.mean = mean;
+    auto scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV>;
+    hipError_t e = hipFuncSetAttribute((const void *)scan,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.scan_lds);
+    if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
+    hipLaunchKernelGGL(scan, dim3(P.scan_groups), dim3(kScanThreads), P.scan_lds, st, a, sa);
+    if ((s = launch_status())) return s;
+    hipLaunchKernelGGL((merge_window_kernel<KEYB>), dim3((P.wl.cap + kMergeCells - 1) / kMergeCells),
+                       dim3(kMergeCells * kMergeWays), 0, st, P.wl, (const uint8_t *)sa.partials,
+                       P.scan_groups, (const int32_t *)window, a.W, a.H, ncat, out);
+    if ((s = launch_status())) return s;
+    PartArgs pa;
+    pa.cell = sa.spill_cell;
+    pa.val = sa.spill_val;
+    pa.count = count;
+    pa.tile_cells = P.tl.cap;
+    pa.ntiles = P.ntiles;
+    pa.hist = reinterpret_cast<uint32_t *>(ws + P.o_hist);
+    pa.tile_start = reinterpret_cast<unsigned long long *>(ws + P.o_tile_start);
+    pa.sorted_cell = reinterpret_cast<uint32_t *>(ws + P.o_sorted_cell);
+    pa.sorted_val = ws + P.o_sorted_val;
+    unsigned long long *totals = reinterpret_cast<unsigned long long *>(ws + P.o_totals);
+    hipLaunchKernelGGL(spill_hist_kernel, dim3(kPartGroups), dim3(kPartThreads), 0, st, pa);
+    hipLaunchKernelGGL(spill_colscan_kernel, dim3(P.ntiles), dim3(kPartThreads), 0, st, pa, totals);
+    hipLaunchKernelGGL(spill_tilescan_kernel, dim3(1), dim3(kPartThreads), 0, st, pa,
+                       (const unsigned long long *)totals);
+    hipLaunchKernelGGL((spill_scatter_kernel<TV, HAS_VAL>), dim3(kPartGroups), dim3(kPartThreads),
+                       0, st, pa);
+    if ((s = launch_status())) return s;
+    auto tile = tile_agg_kernel<TV, HAS_VAL, KEYB, SQDEV>;
+    e = hipFuncSetAttribute((const void *)tile, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)P.tile_lds);
+    if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
+    hipLaunchKernelGGL(tile, dim3(P.ntiles), dim3(kPartThreads), P.tile_lds, st, pa, P.tl,
+                       (const int32_t *)window, a.W, a.H, wcap_px, ncat, P.ncells, mean, out);
+    return launch_status();
+}
+
+template <typename TC>
+static int dispatch_val(const dsx_points_desc *d, const PointArgs &a, const Plan &P,
+                        const Planes &out, const double *mean, uint8_t *ws, hipStream_t st) {
+    const bool sq = mean != nullptr;
+    if (!has_value(d)) return run_points<TC, float, false, 8, false>(d, a, P, out, mean, ws, st);
+    if (d->val_dtype == DSX_F32)
+        return sq ? run_points<TC, float, true, 4, true>(d, a, P, out, mean, ws, st)
+                  : run_points<TC, float, true, 4, false>(d, a, P, out, mean, ws, st);
+    return sq ? run_points<TC, double, true, 8, true>(d, a, P, out, mean, ws, st)
+              : run_points<TC, double, true, 8, false>(d, a, P, out, mean, ws, st);
+}
+
 }  // namespace dsx
 
 using namespace dsx;
 
-extern "C" int dsx_points_agg(const dsx_points_desc *d, const dsx_view *v,
-                              const dsx_accums *acc, void *stream) {
-    int st = check_points(d, v);
-    if (st) return st;
-    if (!acc) return DSX_ERR_NULL;
-    const bool f32v = d->val && d->val_dtype == DSX_F32;
-    const bool needs_val = acc->countv || acc->sum_hi || acc->sum64 || acc->min32 ||
-                           acc->max32 || acc->min64 || acc->max64;
-    if (needs_val && !d->val) return DSX_ERR_ACCUM;
-    if ((acc->sum_hi != nullptr) != (acc->sum_lo != nullptr)) return DSX_ERR_ACCUM;
-    if (d->val && f32v && (acc->sum64 || acc->min64 || acc->max64)) return DSX_ERR_ACCUM;
-    if (d->val && !f32v && (acc->sum_hi || acc->min32 || acc->max32)) return DSX_ERR_ACCUM;
-    if (d->n == 0) return DSX_OK;
-    const PointArgs a = args_of(d, v);
-    const Planes p = planes_of(acc);
-    const int block = 256;
-    const int grid = grid_for(d->n, block);
-    hipStream_t s = as_stream(stream);
-    const bool c32 = d->xy_dtype == DSX_F32;
-    if (!d->val) {
-        if (c32) points_agg_kernel<float, float, false><<<grid, block, 0, s>>>(a, p);
-        else points_agg_kernel<double, float, false><<<grid, block, 0, s>>>(a, p);
-    } else if (f32v) {
-        if (c32) points_agg_kernel<float, float, true><<<grid, block, 0, s>>>(a, p);
-        else points_agg_kernel<double, float, true><<<grid, block, 0, s>>>(a, p);
-    } else {
-        if (c32) points_agg_kernel<float, double, true><<<grid, block, 0, s>>>(a, p);
-        else points_agg_kernel<double, double, true><<<grid, block, 0, s>>>(a, p);
-    }
-    return launch_status();
+extern "C" size_t dsx_points_workspace_size(const dsx_points_desc *d, const dsx_view *v,
+                                            const dsx_planes *p) {
+    if (check_points(d, v) || !p) return 0;
+    Plan P;
+    if (make_plan(d, v, p, P)) return 0;
+    return P.total;
 }
 
-extern "C" int dsx_points_m2(const dsx_points_desc *d, const dsx_view *v, const double *mean,
-                             float *m2_hi, float *m2_lo, double *m2_64, void *stream) {
+extern "C" int dsx_points_agg(const dsx_points_desc *d, const dsx_view *v, const dsx_planes *p,
+                              const double *mean, void *workspace, size_t workspace_bytes,
+                              void *stream) {
     int st = check_points(d, v);
     if (st) return st;
-    if (!d->val || !mean) return DSX_ERR_NULL;
-    const bool f32v = d->val_dtype == DSX_F32;
-    if (f32v && (!m2_hi || !m2_lo)) return DSX_ERR_ACCUM;
-    if (!f32v && !m2_64) return DSX_ERR_ACCUM;
-    if (d->n == 0) return DSX_OK;
+    if (!p) return DSX_ERR_NULL;
+    const bool needs_val = p->countv || p->sum || p->min || p->max;
+    if (needs_val && !has_value(d)) return DSX_ERR_ACCUM;
+    if (mean && (!has_value(d) || !p->sum || p->count || p->countv || p->min || p->max))
+        return DSX_ERR_ACCUM;
+    Plan P;
+    if ((st = make_plan(d, v, p, P))) return st;
+    if (!workspace) return DSX_ERR_WORKSPACE;
+    if (workspace_bytes < P.total) return DSX_ERR_WORKSPACE;
     const PointArgs a = args_of(d, v);
-    const int block = 256;
-    const int grid = grid_for(d->n, block);
+    const Planes out = planes_of(p);
     hipStream_t s = as_stream(stream);
-    const bool c32 = d->xy_dtype == DSX_F32;
-    if (f32v) {
-        if (c32) points_m2_kernel<float, float><<<grid, block, 0, s>>>(a, mean, m2_hi, m2_lo, m2_64);
-        else points_m2_kernel<double, float><<<grid, block, 0, s>>>(a, mean, m2_hi, m2_lo, m2_64);
-    } else {
-        if (c32) points_m2_kernel<float, double><<<grid, block, 0, s>>>(a, mean, m2_hi, m2_lo, m2_64);
-        else points_m2_kernel<double, double><<<grid, block, 0, s>>>(a, mean, m2_hi, m2_lo, m2_64);
-    }
-    return launch_status();
+    uint8_t *ws = reinterpret_cast<uint8_t *>(workspace);
+    if (d->xy_dtype == DSX_F32) return dispatch_val<float>(d, a, P, out, mean, ws, s);
+    return dispatch_val<double>(d, a, P, out, mean, ws, s);
 }

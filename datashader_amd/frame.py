@@ -81,6 +81,8 @@ class DeviceFrame:
         if len(lens) > 1:
             raise ValueError("all columns of a DeviceFrame must have the same length")
         self._len = lens.pop() if lens else 0
+        # resident frames keep per-canvas tuning state (the hot-window cache)
+        self.persistent = True
         for name, t in self._cols.items():
             if t.dim() != 1:
                 raise ValueError(f"column {name!r} is not 1-D")

@@ -1,21 +1,21 @@
 /*
  * dsx.h -- C ABI of libdsx.so, the MI355X (gfx950) points / line aggregation library.
  *
- * Every pointer argument named "device" is a caller-owned device allocation (the Python
- * host passes torch-ROCm tensor data pointers); the library never allocates or frees
- * caller memory.  Every call is asynchronous on the given HIP stream (a hipStream_t
- * passed as void*; NULL = the default stream) unless stated otherwise, returns DSX_OK
- * (0) or a negative dsx_status, and never throws.  The library keeps no mutable global
- * state: calls on disjoint buffers are reentrant, and calls that share accumulator
- * planes on one stream are ordered by that stream.
+ * Every pointer named "device" is a caller-owned device allocation (the Python host
+ * passes torch-ROCm tensor data pointers); the library never allocates or frees caller
+ * memory.  Every call is asynchronous on the given HIP stream (a hipStream_t passed as
+ * void*; NULL = the default stream), returns DSX_OK (0) or a negative dsx_status, and
+ * never throws.  Descriptors are validated on the host before anything is launched.
+ * The library keeps no mutable global state: calls on disjoint buffers are reentrant;
+ * calls sharing planes or a workspace must be ordered (one stream).
  *
  * Correspondence with the reference (datashader 0.19.1) -- this ABI replaces the numba
- * kernels that the reference's backends call:
+ * kernels its backends call:
  *   dsx_points_agg     glyphs/points.py:188-221 (_perform_extend_points / extend_cpu /
- *                      extend_cuda) with the appends that compiler.py:321-475 generates
- *                      from reductions.py (count 554-584, _sum_zero 958-962, sum
- *                      1056-1063, min 1178-1182, max 1223-1227, any 842-861, by 789-798)
- *   dsx_points_m2      the m2 append of reductions.py:1128-1137 (second pass, see below)
+ *                      extend_cuda) with the appends compiler.py:321-475 generates from
+ *                      reductions.py (count 554-584, _sum_zero 958-962, sum 1056-1063,
+ *                      min 1178-1182, max 1223-1227, any 842-861, by 789-798); with a
+ *                      mean plane it is the m2 append of reductions.py:1128-1137
  *   dsx_lines_agg      glyphs/line.py:1033-1097 draw_segment (+ _liang_barsky 734-800,
  *                      map_onto_pixel_snap 688-720, _bresenham 986-1031) driven by the
  *                      extend loops of line.py:1099-1136 (axis 0), 1152-1193 (axis 0,
@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define DSX_ABI_VERSION 1
+#define DSX_ABI_VERSION 2
 
 /* ---- status codes --------------------------------------------------------------- */
 typedef enum {
@@ -46,8 +46,8 @@ typedef enum {
     DSX_ERR_SHAPE = -3,       /* non-positive canvas, negative length, bad stride */
     DSX_ERR_WORKSPACE = -4,   /* workspace missing or smaller than dsx_*_workspace_size */
     DSX_ERR_HIP = -5,         /* a HIP runtime call failed (launch or memory) */
-    DSX_ERR_ACCUM = -6,       /* accumulator set inconsistent with the value column */
-    DSX_ERR_MODE = -7         /* unknown finalize / fill mode */
+    DSX_ERR_ACCUM = -6,       /* planes inconsistent with the value column */
+    DSX_ERR_MODE = -7         /* unknown finalize / fill / categorizer mode */
 } dsx_status;
 
 /* ---- dtype codes ---------------------------------------------------------------- */
@@ -56,30 +56,28 @@ typedef enum {
     DSX_I8 = 3, DSX_U8 = 4, DSX_I16 = 5, DSX_I32 = 6, DSX_I64 = 7
 } dsx_dtype;
 
-/* ---- accumulator planes --------------------------------------------------------- *
+/* ---- aggregate planes ----------------------------------------------------------- *
  * Each non-NULL plane is a dense device array of H*W*ncat cells, cell index
- * (y*W + x)*ncat + category (the reference's agg[y, x, cat] layout, C order).
- * Initial values (dsx_fill): counts 0, sums 0, min keys DSX_MIN_EMPTY*, max keys 0.
- *
- * Float accumulation: a float32 value column accumulates its sum (and the squared
- * deviations of dsx_points_m2) in float32 with an exact two-sum compensation term:
- * sum = (double)hi + (double)lo.  A float64 value column accumulates in float64.
- * min / max are kept as order-preserving unsigned keys of the value's own width
- * (key(v) = bits(v) ^ (sign ? all-ones : sign-bit)), so min/max are exact.          */
+ * (y*W + x)*ncat + category (the reference's agg[y, x, cat] layout, C order):
+ *   count   u32   rows in range                                  (count())
+ *   countv  u32   rows in range whose value is not NaN           (count(v))
+ *   sum     f64   sum of the non-NaN values (float64 accumulation; float32 values are
+ *                 widened exactly)                                (_sum_zero(v))
+ *   min/max u32 keys for a float32 value column, u64 keys for float64: order-preserving
+ *           keys key(v) = bits(v) ^ (sign ? all-ones : sign-bit), so min/max are exact;
+ *           an empty min cell holds DSX_MIN_EMPTY*, an empty max cell holds 0.
+ * dsx_points_agg WRITES every cell of every requested plane (no initialisation needed).
+ * dsx_lines_agg ADDS into planes the caller initialised with dsx_fill.               */
 #define DSX_MIN_EMPTY32 0xFFFFFFFFu
 #define DSX_MIN_EMPTY64 0xFFFFFFFFFFFFFFFFull
 
 typedef struct {
-    uint32_t *count;      /* count()                      (rows in range)             */
-    uint32_t *countv;     /* count(v): rows whose value is not NaN                      */
-    float *sum_hi;        /* float32 value column: compensated sum, high part          */
-    float *sum_lo;        /*                                        low (error) part   */
-    double *sum64;        /* float64 value column: sum                                 */
-    uint32_t *min32;      /* float32 value column: min key                             */
-    uint32_t *max32;      /*                        max key                            */
-    uint64_t *min64;      /* float64 value column: min key                             */
-    uint64_t *max64;      /*                        max key                            */
-} dsx_accums;
+    uint32_t *count;
+    uint32_t *countv;
+    double *sum;
+    void *min;
+    void *max;
+} dsx_planes;
 
 /* ---- view: the extend() arguments vt + bounds (points.py:223, line.py:362) --------- */
 typedef struct {
@@ -90,7 +88,6 @@ typedef struct {
     int32_t xlog, ylog;           /* 1: the axis maps v -> log10(v) before scaling        */
 } dsx_view;                       /*    (LogAxis, core.py:127-145); bounds stay raw       */
 
-/* ---- points --------------------------------------------------------------------- */
 /* ---- categorizer: by(...)'s category of a row (reductions.py:143-259) ------------- *
  * kind DSX_CAT_NONE    no categories (ncat must be 1)
  *      DSX_CAT_CODES   category = col[i] (pandas category codes; -1 wraps to ncat-1)
@@ -112,6 +109,7 @@ typedef struct {
     double bin0, binsize;
 } dsx_categorizer;
 
+/* ---- points --------------------------------------------------------------------- */
 typedef struct {
     int64_t n;                    /* rows                                                 */
     const void *x, *y;            /* device, n elements each                              */
@@ -119,23 +117,30 @@ typedef struct {
     int32_t val_dtype;            /* DSX_F32 | DSX_F64, 0 when val == NULL                */
     const void *val;              /* device value column or NULL                          */
     dsx_categorizer cat;
+    /* Hot window (a performance hint; results do not depend on it): a device int32[4]
+     * {x0, y0, w, h} in pixels, or NULL.  reuse_window = 0: the call picks the window
+     * from a sample of the rows and, when window != NULL, stores it there; 1: the call
+     * uses the stored window (an invalid one is treated as empty).                     */
+    int32_t *window;
+    int32_t reuse_window;
 } dsx_points_desc;
 
-/* Project rows to bins and update every non-NULL plane of acc (one pass over the
- * columns).  A row is dropped unless xmin <= x <= xmax and ymin <= y <= ymax (NaN
- * fails); its bin is trunc(x*sx + tx) (float64, two separate roundings), clamped to
- * W-1 at the upper edge.  Planes that need the value column (countv, sums, min/max)
- * require val. */
-int dsx_points_agg(const dsx_points_desc *d, const dsx_view *v, const dsx_accums *acc,
-                   void *stream);
+/* Device workspace (bytes) dsx_points_agg needs for these inputs and planes. */
+size_t dsx_points_workspace_size(const dsx_points_desc *d, const dsx_view *v,
+                                 const dsx_planes *p);
 
-/* Second pass for var / std: for every row that dsx_points_agg counted in countv,
- * accumulate (v - mean[bin])^2 into m2 (float32 value column: m2_hi/m2_lo compensated;
- * float64: m2_64).  mean is a device float64 plane of H*W*ncat cells.  The reference's
- * streaming update (reductions.py:1128-1137) computes the same sum of squared
- * deviations; the two-pass form is its exact algebraic equivalent. */
-int dsx_points_m2(const dsx_points_desc *d, const dsx_view *v, const double *mean,
-                  float *m2_hi, float *m2_lo, double *m2_64, void *stream);
+/* Project every row to its bin and write the requested planes.  A row is dropped unless
+ * xmin <= x <= xmax and ymin <= y <= ymax (NaN fails); its bin is trunc(x*sx + tx)
+ * (float64, two separate roundings), clamped to W-1 at the upper edge.  Planes that read
+ * the value column (countv, sum, min, max) require val.
+ *
+ * With mean != NULL (a device f64 plane of H*W*ncat cells) the call computes the m2 of
+ * var / std instead: sum receives sum((v - mean[bin])^2) over the rows counted in
+ * countv; only p->sum may be requested.  The reference's streaming update
+ * (reductions.py:1128-1137) computes the same sum of squared deviations; this two-pass
+ * form is its exact algebraic equivalent. */
+int dsx_points_agg(const dsx_points_desc *d, const dsx_view *v, const dsx_planes *p,
+                   const double *mean, void *workspace, size_t workspace_bytes, void *stream);
 
 /* ---- lines ---------------------------------------------------------------------- *
  * nlines lines of nverts vertices each.  Vertex k of line l is
@@ -157,7 +162,7 @@ typedef struct {
     dsx_categorizer cat;          /* indexed like val                                     */
 } dsx_lines_desc;
 
-int dsx_lines_agg(const dsx_lines_desc *d, const dsx_view *v, const dsx_accums *acc,
+int dsx_lines_agg(const dsx_lines_desc *d, const dsx_view *v, const dsx_planes *p,
                   void *stream);
 
 /* ---- bounds (autorange) ----------------------------------------------------------- *
@@ -183,7 +188,7 @@ typedef enum {
     DSX_FIN_ANY = 1,          /* out u8  = count > 0                                      */
     DSX_FIN_SUM = 2,          /* out f64 = countv > 0 ? sum : NaN                         */
     DSX_FIN_MEAN = 3,         /* out f64 = countv > 0 ? sum / countv : NaN                */
-    DSX_FIN_VAR = 4,          /* out f64 = countv > 0 ? m2 / countv : NaN                 */
+    DSX_FIN_VAR = 4,          /* out f64 = countv > 0 ? m2 / countv : NaN   (m2 in sum)   */
     DSX_FIN_STD = 5,          /* out f64 = countv > 0 ? sqrt(m2 / countv) : NaN           */
     DSX_FIN_MIN = 6,          /* out f64 = decoded min key, NaN when empty                */
     DSX_FIN_MAX = 7,          /* out f64 = decoded max key, NaN when empty                */
@@ -192,12 +197,11 @@ typedef enum {
 
 typedef struct {
     int32_t mode;             /* dsx_finalize_mode                                        */
-    int32_t val_dtype;        /* DSX_F32 | DSX_F64: which sum / key planes are read       */
+    int32_t key_bytes;        /* DSX_FIN_MIN / MAX: 4 (float32 values) or 8 (float64)     */
     int64_t ncells;
-    const uint32_t *count;    /* DSX_FIN_ANY: count or countv                             */
+    const uint32_t *count;    /* DSX_FIN_ANY: the count (or countv) plane                 */
     const uint32_t *countv;
-    const float *hi, *lo;     /* f32: sum or m2 (hi + lo)                                 */
-    const double *v64;        /* f64: sum or m2                                           */
+    const double *sum;        /* sum, or m2 for DSX_FIN_VAR / DSX_FIN_STD                 */
     const void *key;          /* min / max key plane                                      */
     void *out;                /* device output (u8 or f64)                                */
 } dsx_finalize_desc;

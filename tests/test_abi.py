@@ -37,7 +37,7 @@ def test_library_exports_every_declared_function():
 
 
 STRUCTS = {
-    "dsx_accums": _abi.Accums, "dsx_view": _abi.View, "dsx_categorizer": _abi.Categorizer,
+    "dsx_planes": _abi.Planes, "dsx_view": _abi.View, "dsx_categorizer": _abi.Categorizer,
     "dsx_points_desc": _abi.PointsDesc, "dsx_lines_desc": _abi.LinesDesc,
     "dsx_finalize_desc": _abi.FinalizeDesc,
 }
@@ -85,56 +85,61 @@ def _view(W=4, H=4):
 def test_points_descriptor_validation():
     """Invalid descriptors are rejected on the host, before any HIP call."""
     lib = _abi.load()
-    acc = _abi.Accums()
+    pl = _abi.Planes()
     d = _abi.PointsDesc()
     d.n = 10
-    assert lib.dsx_points_agg(None, _view(), acc, None) == -1
-    assert lib.dsx_points_agg(d, _view(), acc, None) == -1          # x / y NULL
+    ws, wsz = ctypes.c_void_p(0x1000), 1 << 40
+
+    def call(desc, view=None, planes=pl, mean=None, work=ws, size=wsz):
+        return lib.dsx_points_agg(desc, view or _view(), planes, mean, work, size, None)
+
+    assert lib.dsx_points_agg(None, _view(), pl, None, ws, wsz, None) == -1
+    assert call(d) == -1                                            # x / y NULL
     d.x = d.y = ctypes.c_void_p(0x1000)
     d.xy_dtype = 7
-    assert lib.dsx_points_agg(d, _view(), acc, None) == -2          # coordinates must be float
+    assert call(d) == -2                                            # coordinates must be float
     d.xy_dtype = _abi.F32
-    assert lib.dsx_points_agg(d, _view(0, 4), acc, None) == -3      # empty canvas
+    assert call(d, _view(0, 4)) == -3                               # empty canvas
     d.n = -1
-    assert lib.dsx_points_agg(d, _view(), acc, None) == -3
+    assert call(d) == -3
     d.n = 10
-    acc.countv = ctypes.c_void_p(0x2000)
-    assert lib.dsx_points_agg(d, _view(), acc, None) == -6          # countv without a value
-    acc = _abi.Accums()
-    acc.sum_hi = ctypes.c_void_p(0x2000)
+    assert call(d) == -6                                            # no plane requested
+    pl.countv = ctypes.c_void_p(0x2000)
+    assert call(d) == -6                                            # countv without a value
+    pl = _abi.Planes()
+    pl.count = ctypes.c_void_p(0x2000)
+    assert call(d, planes=pl, work=None) == -4                      # workspace missing
+    assert call(d, planes=pl, size=16) == -4                        # workspace too small
+    assert lib.dsx_points_workspace_size(d, _view(), pl) > 0
     d.val, d.val_dtype = ctypes.c_void_p(0x3000), _abi.F32
-    assert lib.dsx_points_agg(d, _view(), acc, None) == -6          # hi without lo
-    acc = _abi.Accums()
-    acc.sum64 = ctypes.c_void_p(0x2000)
-    assert lib.dsx_points_agg(d, _view(), acc, None) == -6          # f64 plane, f32 value
-    acc = _abi.Accums()
+    assert call(d, planes=pl, mean=ctypes.c_void_p(0x5000)) == -6   # m2 mode takes sum only
     d.cat.kind = _abi.CAT_CODES
-    assert lib.dsx_points_agg(d, _view(), acc, None) == -1          # categorizer column NULL
+    assert call(d, planes=pl) == -1                                 # categorizer column NULL
     d.cat.col, d.cat.dtype, d.cat.ncat = ctypes.c_void_p(0x4000), _abi.F32, 3
-    assert lib.dsx_points_agg(d, _view(), acc, None) == -2          # codes must be integers
+    assert call(d, planes=pl) == -2                                 # codes must be integers
     d.cat.kind, d.cat.dtype, d.cat.modulo = _abi.CAT_MODULO, _abi.I64, 0
-    assert lib.dsx_points_agg(d, _view(), acc, None) == -3          # modulo 0
+    assert call(d, planes=pl) == -3                                 # modulo 0
     d.cat.kind = 9
-    assert lib.dsx_points_agg(d, _view(), acc, None) == -7
-    d.cat = _abi.Categorizer()
-    d.n = 0
-    assert lib.dsx_points_agg(d, _view(), acc, None) == 0           # nothing to do
+    assert call(d, planes=pl) == -7
+    assert lib.dsx_points_workspace_size(d, _view(), pl) == 0
 
 
 def test_lines_and_misc_validation():
     lib = _abi.load()
-    acc = _abi.Accums()
+    pl = _abi.Planes()
     d = _abi.LinesDesc()
     d.nlines, d.nverts, d.x_dtype, d.y_dtype = 1, 3, _abi.F64, _abi.F64
     d.axis = 2
-    assert lib.dsx_lines_agg(d, _view(), acc, None) == -3
+    assert lib.dsx_lines_agg(d, _view(), pl, None) == -3
     d.axis = 0
     d.x_dtype = 5
-    assert lib.dsx_lines_agg(d, _view(), acc, None) == -2
+    assert lib.dsx_lines_agg(d, _view(), pl, None) == -2
     d.x_dtype = _abi.F64
-    assert lib.dsx_lines_agg(d, _view(), acc, None) == -1           # x / y NULL
+    assert lib.dsx_lines_agg(d, _view(), pl, None) == -1            # x / y NULL
     d.nverts = 1
-    assert lib.dsx_lines_agg(d, _view(), acc, None) == 0            # no segment
+    assert lib.dsx_lines_agg(d, _view(), pl, None) == 0             # no segment
+    pl.sum = ctypes.c_void_p(0x2000)
+    assert lib.dsx_lines_agg(d, _view(), pl, None) == -6            # sum without a value
     assert lib.dsx_fill(None, _abi.FILL_ZERO32, 5, None) == -1
     assert lib.dsx_fill(ctypes.c_void_p(0x1000), 99, 5, None) == -7
     assert lib.dsx_fill(None, _abi.FILL_ZERO32, 0, None) == 0
@@ -142,8 +147,9 @@ def test_lines_and_misc_validation():
     f.mode, f.ncells, f.out = 42, 4, ctypes.c_void_p(0x1000)
     assert lib.dsx_finalize(f, None) == -7
     f.mode = _abi.FIN_MEAN
-    f.val_dtype = _abi.F32
-    assert lib.dsx_finalize(f, None) == -1                          # countv NULL
+    assert lib.dsx_finalize(f, None) == -1                          # countv / sum NULL
+    f.mode, f.key, f.key_bytes = _abi.FIN_MIN, ctypes.c_void_p(0x1000), 3
+    assert lib.dsx_finalize(f, None) == -2
     assert lib.dsx_bounds(None, _abi.F32, 4, ctypes.c_void_p(0x1000), None, 0, None) == -1
     assert lib.dsx_bounds(ctypes.c_void_p(0x1000), 9, 4, ctypes.c_void_p(0x1000), None, 0,
                           None) == -2

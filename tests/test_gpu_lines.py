@@ -40,7 +40,7 @@ def abi_lines(xs, ys, view, W, H, axis=0, plot_start=True, x_ls=0, x_vs=1, y_ls=
     v.sx, v.tx, v.sy, v.ty = view.sx, view.tx, view.sy, view.ty
     v.xmin, v.xmax, v.ymin, v.ymax = view.xmin, view.xmax, view.ymin, view.ymax
     v.xxmax, v.yymax, v.width, v.height = view.xxmax, view.yymax, W, H
-    acc = _abi.Accums()
+    acc = _abi.Planes()
     acc.count = _device.ptr(cnt)
     _abi.check(lib.dsx_lines_agg(d, v, acc, _device.stream_handle()), "dsx_lines_agg")
     return cnt.view(torch.uint32).cpu().numpy().reshape(H, W)
