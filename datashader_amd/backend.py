@@ -8,10 +8,11 @@ columns running as a libdsx kernel:
 * reductions are grouped into *passes*: one kernel launch per (value column,
   categorizer) pair, producing the union of the accumulator planes its reductions need
   (``compiler.py:36-150`` dedups bases the same way);
-* planes are created with the reference's initial values by ``dsx_fill``
-  (``reductions.py:446-472``);
+* ``dsx_points_agg`` writes every cell of its planes; line planes are created with the
+  reference's initial values by ``dsx_fill`` (``reductions.py:446-472``);
 * ``dsx_points_agg`` / ``dsx_lines_agg`` is the ``extend`` call;
-* var / std add a second pass (``dsx_points_m2``) over the per-bin means;
+* var / std add a second ``dsx_points_agg`` pass that is given the per-bin means and
+  accumulates squared deviations from them;
 * ``dsx_finalize`` turns planes into the reference's output dtypes, and the result is
   wrapped with the same dims / coords / attrs as ``pandas.py:61-66``.
 """

@@ -7,15 +7,16 @@
 //   select   a sampled coarse density histogram picks the hot rectangle ("window")
 //            of the canvas whose aggregate planes fit the 160 KB LDS of a workgroup
 //   scan     one streaming pass over the columns: rows that land in the window update
-//            the workgroup's LDS copy of the window; every other kept row is compacted
-//            (wave ballots into an LDS staging area, one global reservation per 4096-row
-//            chunk) into a spill list of (cell, value)
-//   merge    the per-workgroup window partials are summed cell by cell in a fixed order
-//            and written to the output planes (plain stores)
-//   hist / offsets / scatter
-//            the spill list is counting-sorted by canvas tile (a tile is a run of cells
-//            whose planes fit LDS); each partition workgroup owns a fixed slice of the
-//            list, so every entry's destination is known without atomics
+//            the workgroup's LDS copy of the window; every other kept row is appended
+//            to its wave's own sub-segment of a spill list of (cell, value) at the
+//            wave's running count (a ballot prefix, no atomic, no barrier), and counted
+//            in an LDS histogram of canvas tiles (a tile is a run of cells whose planes
+//            fit LDS)
+//   merge    the per-workgroup window partials are summed cell by cell in a fixed tree
+//            order and written to the output planes (plain stores)
+//   offsets / scatter
+//            the spill segments are counting-sorted by tile; the histograms give every
+//            (segment, tile) run its destination, so no atomics are needed
 //   tiles    one workgroup per tile accumulates its spill entries in LDS and writes
 //            every cell of the tile outside the window (plain stores)
 //
@@ -34,8 +35,6 @@ constexpr int kChunk = kScanThreads * kItems;  // rows per workgroup iteration
 constexpr int kLdsBytes = 160 * 1024;           // LDS of one gfx950 workgroup
 constexpr int kCtrlBytes = 64;
 constexpr int kPartThreads = 1024;
-constexpr int kPartGroups = 512;                // workgroups of the partition passes
-constexpr int kSubChunk = 4096;                 // spill entries sorted per LDS round
 constexpr int kMaxTiles = 4096;
 constexpr int kSelBins = 8192;                  // coarse density histogram bins
 constexpr int kSampleRuns = 256;
@@ -285,14 +284,23 @@ __global__ __launch_bounds__(1024) void select_pick_kernel(const uint32_t *hist,
 }
 
 // ---- scan ------------------------------------------------------------------------------------
+// Workgroup g of the scan handles chunks g, g + G, ...; wave w of it appends the rows it
+// spills to its own sub-segment [(g*kWaves + w)*seg_cap, ... + seg_count[g*kWaves + w])
+// of the spill list, so no reservation is shared by two waves (the running count stays
+// in a scalar register).
+constexpr int kWaves = kScanThreads / 64;
+
 struct ScanArgs {
     Lay wl;                          // window planes (cap = window capacity in cells)
     const int32_t *window;           // device [x0, y0, w, h]
     uint8_t *partials;               // gridDim.x regions of wl.bytes
-    uint32_t *spill_cell;            // spill list
+    uint32_t *spill_cell;            // spill sub-segments
     void *spill_val;
-    unsigned long long *spill_count; // entries appended (device, starts at 0)
-    int64_t spill_cap;
+    uint32_t *seg_count;             // [G * kWaves] entries in each sub-segment
+    int64_t seg_cap;                 // entries a sub-segment can hold (its wave's rows)
+    uint32_t *hist;                  // [G][ntiles] spilled entries per tile
+    uint32_t tile_cells;
+    int32_t ntiles;
     const double *mean;              // squared-deviation mode (m2 pass)
 };
 
@@ -321,17 +329,17 @@ template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArgs s) {
     extern __shared__ __align__(16) uint8_t smem[];
     uint8_t *win = smem;
-    uint32_t *stg_cell = reinterpret_cast<uint32_t *>(smem + s.wl.bytes);
-    TV *stg_val = reinterpret_cast<TV *>(stg_cell + kChunk);
-    uint32_t *ctrl = reinterpret_cast<uint32_t *>(
-        reinterpret_cast<uint8_t *>(stg_val) + (HAS_VAL ? kChunk * sizeof(TV) : 0));
-    unsigned long long *gbase = reinterpret_cast<unsigned long long *>(ctrl + 2);
+    uint32_t *thist = reinterpret_cast<uint32_t *>(smem + s.wl.bytes);  // [ntiles]
     const int tid = threadIdx.x;
     const int ncat = a.cat.ncat;
     const Window w = load_window(s.window, a.W, a.H, (int64_t)(s.wl.cap / (uint32_t)ncat));
     region_init<KEYB>(win, s.wl, tid, kScanThreads);
-    if (tid == 0) ctrl[0] = 0u;
+    for (int t = tid; t < s.ntiles; t += kScanThreads) thist[t] = 0u;
     __syncthreads();
+    const int wave = tid >> 6;
+    const size_t seg0 = ((size_t)blockIdx.x * kWaves + wave) * (size_t)s.seg_cap;
+    uint32_t filled = 0;  // wave-uniform
+    TV *spill_val = reinterpret_cast<TV *>(s.spill_val);
     const int lane = threadIdx.x & 63;
     const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
     // software pipeline: the next chunk's loads are in flight while this one is binned
@@ -361,36 +369,19 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
                 }
                 region_add<KEYB>(win, s.wl, lc, vnan, v, add);
             }
+            // spill: appended to this wave's sub-segment at the wave's running count
             const bool spill = keep && !inw;
             const unsigned long long m = __ballot(spill);
-            if (m) {
-                const uint32_t nw = (uint32_t)__popcll(m);
-                const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                uint32_t b0 = 0;
-                if (lane == 0) b0 = atomicAdd(&ctrl[0], nw);
-                b0 = __shfl(b0, 0, 64);
-                if (spill) {
-                    stg_cell[b0 + rank] = (uint32_t)(((int64_t)py * a.W + px) * ncat + pc);
-                    if constexpr (HAS_VAL) stg_val[b0 + rank] = va[it];
+            if (spill) {
+                const uint32_t cell = (uint32_t)(((int64_t)py * a.W + px) * ncat + pc);
+                const size_t slot = filled + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (slot < (size_t)s.seg_cap) {
+                    s.spill_cell[seg0 + slot] = cell;
+                    if constexpr (HAS_VAL) spill_val[seg0 + slot] = va[it];
                 }
+                atomicAdd(&thist[cell / s.tile_cells], 1u);
             }
-        }
-        __syncthreads();
-        const uint32_t staged = ctrl[0];
-        if (staged) {
-            if (tid == 0) gbase[0] = atomicAdd(s.spill_count, (unsigned long long)staged);
-            __syncthreads();
-            const unsigned long long g0 = gbase[0];
-            for (uint32_t j = tid; j < staged; j += kScanThreads) {
-                const unsigned long long g = g0 + j;
-                if (g < (unsigned long long)s.spill_cap) {
-                    s.spill_cell[g] = stg_cell[j];
-                    if constexpr (HAS_VAL) reinterpret_cast<TV *>(s.spill_val)[g] = stg_val[j];
-                }
-            }
-            __syncthreads();
-            if (tid == 0) ctrl[0] = 0u;
-            __syncthreads();
+            filled += (uint32_t)__popcll(m);
         }
 #pragma unroll
         for (int it = 0; it < kItems; ++it) {
@@ -400,10 +391,15 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
         }
     }
     __syncthreads();
-    // this workgroup's window partial
+    // this workgroup's window partial, tile histogram and segment length
     uint4 *dst = reinterpret_cast<uint4 *>(s.partials + (size_t)blockIdx.x * s.wl.bytes);
     const uint4 *src = reinterpret_cast<const uint4 *>(win);
     for (uint32_t j = tid; j < s.wl.bytes / 16u; j += kScanThreads) dst[j] = src[j];
+    for (int t = tid; t < s.ntiles; t += kScanThreads)
+        s.hist[(size_t)blockIdx.x * s.ntiles + t] = thist[t];
+    if (lane == 0)
+        s.seg_count[(size_t)blockIdx.x * kWaves + wave] =
+            filled < (uint32_t)s.seg_cap ? filled : (uint32_t)s.seg_cap;
 }
 
 // ---- merge the window partials --------------------------------------------------------------
@@ -477,39 +473,21 @@ __global__ __launch_bounds__(kMergeCells * kMergeWays) void merge_window_kernel(
 }
 
 // ---- partition the spill list by tile --------------------------------------------------------
+// The spill list arrives as G per-workgroup segments with their tile histograms
+// (written by the scan); group g of the partition owns segment g.
 struct PartArgs {
     const uint32_t *cell;
     const void *val;
-    const unsigned long long *count;  // spill entries
+    const uint32_t *seg_count;        // [G]
+    int64_t seg_cap;
+    int32_t ngroups;                  // G (<= kPartThreads)
     uint32_t tile_cells;
     int32_t ntiles;
-    uint32_t *hist;                   // [kPartGroups][ntiles]
+    uint32_t *hist;                   // [G][ntiles]
     unsigned long long *tile_start;   // [ntiles + 1]
     uint32_t *sorted_cell;
     void *sorted_val;
 };
-
-__device__ __forceinline__ void slice_of(unsigned long long m, int g, unsigned long long &b,
-                                         unsigned long long &e) {
-    const unsigned long long per = (m + kPartGroups - 1) / kPartGroups;
-    b = per * (unsigned long long)g;
-    e = b + per;
-    if (b > m) b = m;
-    if (e > m) e = m;
-}
-
-__global__ __launch_bounds__(kPartThreads) void spill_hist_kernel(PartArgs p) {
-    __shared__ uint32_t h[kMaxTiles];
-    for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x) h[t] = 0u;
-    __syncthreads();
-    unsigned long long b, e;
-    slice_of(*p.count, blockIdx.x, b, e);
-    for (unsigned long long j = b + threadIdx.x; j < e; j += blockDim.x)
-        atomicAdd(&h[p.cell[j] / p.tile_cells], 1u);
-    __syncthreads();
-    for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x)
-        p.hist[(size_t)blockIdx.x * p.ntiles + t] = h[t];
-}
 
 // Block-wide exclusive scan of one value per thread (1024 threads).
 __device__ __forceinline__ unsigned long long block_exscan(unsigned long long v,
@@ -545,11 +523,11 @@ __global__ __launch_bounds__(kPartThreads) void spill_colscan_kernel(PartArgs p,
                                                                     unsigned long long *totals) {
     __shared__ unsigned long long sh[kPartThreads / 64 + 1];
     const int t = blockIdx.x;
-    const int g = threadIdx.x;  // kPartGroups <= kPartThreads
-    const unsigned long long v = g < kPartGroups ? p.hist[(size_t)g * p.ntiles + t] : 0ull;
+    const int g = threadIdx.x;  // ngroups <= kPartThreads
+    const unsigned long long v = g < p.ngroups ? p.hist[(size_t)g * p.ntiles + t] : 0ull;
     unsigned long long tot;
     const unsigned long long ex = block_exscan(v, sh, tot);
-    if (g < kPartGroups) p.hist[(size_t)g * p.ntiles + t] = (uint32_t)ex;
+    if (g < p.ngroups) p.hist[(size_t)g * p.ntiles + t] = (uint32_t)ex;
     if (g == 0) totals[t] = tot;
 }
 
@@ -568,37 +546,58 @@ __global__ __launch_bounds__(kPartThreads) void spill_tilescan_kernel(PartArgs p
     if (threadIdx.x == 0) p.tile_start[p.ntiles] = carry;
 }
 
-// Scatter: each partition group walks its slice in sub-chunks, counting-sorts each
-// sub-chunk by tile in LDS and writes it out run by run (coalesced within runs).
+// Scatter: partition group g walks the kWaves sub-segments of scan workgroup g as one
+// concatenated sequence in sub-chunks, counting-sorts each sub-chunk by tile in LDS and
+// writes it out run by run (coalesced within runs).
+template <typename TV>
+struct ScatterSize {  // entries per LDS round: staging fits LDS next to the tile arrays
+    static constexpr int value = sizeof(TV) == 8 ? 8192 : 12288;
+};
+
 template <typename TV, bool HAS_VAL>
 __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p) {
+    constexpr int kSub = HAS_VAL ? ScatterSize<TV>::value : 12288;
     __shared__ uint32_t cur[kMaxTiles];    // next slot of this group in each tile
     __shared__ uint32_t cnt[kMaxTiles];    // sub-chunk counts, then sub-chunk starts
-    __shared__ uint32_t s_cell[kSubChunk];
-    __shared__ uint16_t s_tile[kSubChunk];
-    __shared__ TV s_val[HAS_VAL ? kSubChunk : 1];
+    __shared__ uint32_t s_cell[kSub];
+    __shared__ uint16_t s_tile[kSub];
+    __shared__ TV s_val[HAS_VAL ? kSub : 1];
     __shared__ unsigned long long sh[kPartThreads / 64 + 1];
+    __shared__ unsigned long long pre[kWaves + 1];  // sub-segment prefix lengths
     const int g = blockIdx.x;
     for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x) {
         cur[t] = p.hist[(size_t)g * p.ntiles + t];
         cnt[t] = 0u;
     }
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (int wv = 0; wv < kWaves; ++wv) {
+            pre[wv] = run;
+            unsigned long long len = p.seg_count[(size_t)g * kWaves + wv];
+            if (len > (unsigned long long)p.seg_cap) len = (unsigned long long)p.seg_cap;
+            run += len;
+        }
+        pre[kWaves] = run;
+    }
     __syncthreads();
-    unsigned long long b, e;
-    slice_of(*p.count, g, b, e);
+    const unsigned long long total = pre[kWaves];
+    const unsigned long long base = (unsigned long long)g * kWaves * (unsigned long long)p.seg_cap;
     const TV *val = reinterpret_cast<const TV *>(p.val);
-    constexpr int kPer = kSubChunk / kPartThreads;
-    for (unsigned long long c0 = b; c0 < e; c0 += kSubChunk) {
+    constexpr int kPer = kSub / kPartThreads;
+    for (unsigned long long c0 = 0; c0 < total; c0 += kSub) {
         uint32_t rc[kPer], rt[kPer], rr[kPer];
         TV rv[kPer];
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            const unsigned long long j = c0 + (unsigned long long)(k * kPartThreads + threadIdx.x);
+            const unsigned long long v = c0 + (unsigned long long)(k * kPartThreads + threadIdx.x);
             rt[k] = 0xFFFFFFFFu;
             rc[k] = 0u;
             rr[k] = 0u;
             rv[k] = TV(0);
-            if (j < e) {
+            if (v < total) {
+                int wv = 0;
+                while (wv + 1 < kWaves && pre[wv + 1] <= v) ++wv;
+                const unsigned long long j = base + (unsigned long long)wv * p.seg_cap + (v - pre[wv]);
                 rc[k] = p.cell[j];
                 if constexpr (HAS_VAL) rv[k] = val[j];
                 rt[k] = rc[k] / p.tile_cells;
@@ -627,7 +626,7 @@ __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p)
             }
         }
         __syncthreads();
-        const uint32_t nsub = (uint32_t)((e - c0) < (unsigned long long)kSubChunk ? (e - c0) : kSubChunk);
+        const uint32_t nsub = (uint32_t)((total - c0) < (unsigned long long)kSub ? (total - c0) : kSub);
         for (uint32_t j = threadIdx.x; j < nsub; j += blockDim.x) {
             const uint32_t t = s_tile[j];
             const unsigned long long dst = p.tile_start[t] + cur[t] + (j - cnt[t]);
@@ -713,6 +712,7 @@ struct Plan {
     int64_t ncells;
     int32_t ntiles;
     int scan_groups;
+    int64_t seg_cap;
     size_t scan_lds, tile_lds;
     // workspace offsets
     size_t o_window, o_hist_sel, o_count, o_partials, o_spill_cell, o_spill_val, o_hist,
@@ -733,15 +733,6 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     P.ncells = (int64_t)v->width * v->height * ncat;
     if (P.ncells >= (1ll << 32)) return DSX_ERR_SHAPE;
     const uint32_t cb = cell_bytes(P.flags, P.keyb);
-    // window capacity: LDS minus the spill staging area and control words
-    const size_t stage = (size_t)kChunk * (4 + P.vbytes);
-    const int64_t wcap_cells = (int64_t)((kLdsBytes - stage - kCtrlBytes - 16) / cb);
-    int64_t wcap_px = wcap_cells / ncat;
-    if (wcap_px < 1) return DSX_ERR_SHAPE;
-    P.whole = (int64_t)v->width * v->height <= wcap_px;
-    if (P.whole) wcap_px = (int64_t)v->width * v->height;
-    P.wl = make_lay(P.flags, P.keyb, (uint32_t)(wcap_px * ncat));
-    P.scan_lds = P.wl.bytes + stage + kCtrlBytes;
     // tiles: equal runs of cells whose planes fit LDS
     int64_t tcap = (kLdsBytes - 16) / cb;
     P.ntiles = (int32_t)((P.ncells + tcap - 1) / tcap);
@@ -749,6 +740,15 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     tcap = (P.ncells + P.ntiles - 1) / P.ntiles;
     P.tl = make_lay(P.flags, P.keyb, (uint32_t)tcap);
     P.tile_lds = P.tl.bytes;
+    // window capacity: LDS minus the scan's tile histogram and control words
+    const size_t side = (size_t)4 * P.ntiles + kCtrlBytes;
+    const int64_t wcap_cells = (int64_t)((kLdsBytes - side - 16) / cb);
+    int64_t wcap_px = wcap_cells / ncat;
+    if (wcap_px < 1) return DSX_ERR_SHAPE;
+    P.whole = (int64_t)v->width * v->height <= wcap_px;
+    if (P.whole) wcap_px = (int64_t)v->width * v->height;
+    P.wl = make_lay(P.flags, P.keyb, (uint32_t)(wcap_px * ncat));
+    P.scan_lds = P.wl.bytes + side;
     // coarse selection grid: bins of 2^bshift pixels on a side, at most kSelBins bins
     P.bshift = 0;
     while ((int64_t)((v->width + (1 << P.bshift) - 1) >> P.bshift) *
@@ -765,16 +765,20 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     }
     const int64_t chunks = (d->n + kChunk - 1) / kChunk;
     P.scan_groups = (int)(chunks < cus ? (chunks < 1 ? 1 : chunks) : cus);
+    // each wave's spill sub-segment holds every row that wave could spill
+    P.seg_cap = ((chunks + P.scan_groups - 1) / P.scan_groups) * (int64_t)(kChunk / kWaves);
+    if (P.seg_cap < 1) P.seg_cap = 1;
+    const size_t nseg = (size_t)P.scan_groups * kWaves * (size_t)P.seg_cap;
     // workspace
     const int64_t n = d->n < 1 ? 1 : d->n;
     size_t o = 0;
     P.o_window = o; o = align_up(o + 16, 256);
     P.o_hist_sel = o; o = align_up(o + 4 * (size_t)kSelBins, 256);
-    P.o_count = o; o = align_up(o + 8, 256);
+    P.o_count = o; o = align_up(o + 4 * (size_t)P.scan_groups * kWaves, 256);  // sub-segment lengths
     P.o_partials = o; o = align_up(o + (size_t)P.scan_groups * P.wl.bytes, 256);
-    P.o_spill_cell = o; o = align_up(o + 4 * (size_t)n, 256);
-    P.o_spill_val = o; o = align_up(o + (size_t)P.vbytes * n, 256);
-    P.o_hist = o; o = align_up(o + 4 * (size_t)kPartGroups * P.ntiles, 256);
+    P.o_spill_cell = o; o = align_up(o + 4 * nseg, 256);
+    P.o_spill_val = o; o = align_up(o + (size_t)P.vbytes * nseg, 256);
+    P.o_hist = o; o = align_up(o + 4 * (size_t)P.scan_groups * P.ntiles, 256);
     P.o_totals = o; o = align_up(o + 8 * (size_t)P.ntiles, 256);
     P.o_tile_start = o; o = align_up(o + 8 * (size_t)(P.ntiles + 1), 256);
     P.o_sorted_cell = o; o = align_up(o + 4 * (size_t)n, 256);
@@ -807,12 +811,10 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
                       const Planes &out, const double *mean, uint8_t *ws, hipStream_t st) {
     int32_t *window = d->window ? d->window : reinterpret_cast<int32_t *>(ws + P.o_window);
     uint32_t *hist_sel = reinterpret_cast<uint32_t *>(ws + P.o_hist_sel);
-    unsigned long long *count = reinterpret_cast<unsigned long long *>(ws + P.o_count);
+    uint32_t *seg_count = reinterpret_cast<uint32_t *>(ws + P.o_count);
     const int ncat = a.cat.ncat;
     const int64_t wcap_px = (int64_t)(P.wl.cap / (uint32_t)ncat);
     int s;
-    // counters start at zero (fill kernels, no hipMemset)
-    if ((s = fill_async(count, DSX_FILL_ZERO64, 1, st))) return s;
     if (P.whole) {
         hipLaunchKernelGGL(set_window_kernel, dim3(1), dim3(64), 0, st, window, 0, 0, a.W, a.H);
     } else if (d->window && d->reuse_window) {
@@ -832,8 +834,11 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     sa.partials = ws + P.o_partials;
     sa.spill_cell = reinterpret_cast<uint32_t *>(ws + P.o_spill_cell);
     sa.spill_val = ws + P.o_spill_val;
-    sa.spill_count = count;
-    sa.spill_cap = a.n;
+    sa.seg_count = seg_count;
+    sa.seg_cap = P.seg_cap;
+    sa.hist = reinterpret_cast<uint32_t *>(ws + P.o_hist);
+    sa.tile_cells = P.tl.cap;
+    sa.ntiles = P.ntiles;
     sa.mean = mean;
     auto scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV>;
     hipError_t e = hipFuncSetAttribute((const void *)scan,
@@ -848,7 +853,9 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     PartArgs pa;
     pa.cell = sa.spill_cell;
     pa.val = sa.spill_val;
-    pa.count = count;
+    pa.seg_count = seg_count;
+    pa.seg_cap = P.seg_cap;
+    pa.ngroups = P.scan_groups;
     pa.tile_cells = P.tl.cap;
     pa.ntiles = P.ntiles;
     pa.hist = reinterpret_cast<uint32_t *>(ws + P.o_hist);
@@ -856,12 +863,11 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     pa.sorted_cell = reinterpret_cast<uint32_t *>(ws + P.o_sorted_cell);
     pa.sorted_val = ws + P.o_sorted_val;
     unsigned long long *totals = reinterpret_cast<unsigned long long *>(ws + P.o_totals);
-    hipLaunchKernelGGL(spill_hist_kernel, dim3(kPartGroups), dim3(kPartThreads), 0, st, pa);
     hipLaunchKernelGGL(spill_colscan_kernel, dim3(P.ntiles), dim3(kPartThreads), 0, st, pa, totals);
     hipLaunchKernelGGL(spill_tilescan_kernel, dim3(1), dim3(kPartThreads), 0, st, pa,
                        (const unsigned long long *)totals);
-    hipLaunchKernelGGL((spill_scatter_kernel<TV, HAS_VAL>), dim3(kPartGroups), dim3(kPartThreads),
-                       0, st, pa);
+    hipLaunchKernelGGL((spill_scatter_kernel<TV, HAS_VAL>), dim3(P.scan_groups),
+                       dim3(kPartThreads), 0, st, pa);
     if ((s = launch_status())) return s;
     auto tile = tile_agg_kernel<TV, HAS_VAL, KEYB, SQDEV>;
     e = hipFuncSetAttribute((const void *)tile, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -269,3 +269,26 @@ def test_device_frame_result_stays_on_device():
     agg = dsa.Canvas(2, 2, x_range=(0, 1), y_range=(0, 1)).points(f, "x", "y")
     assert agg.data.is_cuda and agg.data.dtype == torch.uint32
     np.testing.assert_array_equal(agg.values, [[1, 0], [0, 2]])
+
+
+def test_window_cache_reuse_and_stale_hint():
+    """The hot window only steers performance: a resident frame reuses the window picked
+    on its first call, and a stale or nonsensical stored window (sanitised to empty by
+    the kernels) still gives the same aggregate."""
+    from datashader_amd import synthetic
+    x, y, v = synthetic.points_skewed_np(500_000, seed=9)
+    f = dsa.DeviceFrame({k: torch.from_numpy(a).cuda() for k, a in (("x", x), ("y", y), ("v", v))})
+    c = dsa.Canvas(640, 480, x_range=(0, 1), y_range=(0, 1))
+    agg = dsa.summary(n=dsa.count(), m=dsa.mean("v"), hi=dsa.max("v"))
+    first = c.points(f, "x", "y", agg)
+    (slot,) = f._window_cache.values()
+    assert int(slot[2]) > 0 and int(slot[3]) > 0   # a window was picked and stored
+    second = c.points(f, "x", "y", agg)
+    for bogus in ([-5, 10**6, 7, 9], [600, 400, 100, 100], [0, 0, 0, 0]):
+        slot.copy_(torch.tensor(bogus, dtype=torch.int32))
+        third = c.points(f, "x", "y", agg)
+        for k in ("n", "hi"):
+            np.testing.assert_array_equal(third[k].values, first[k].values)
+        np.testing.assert_allclose(third["m"].values, first["m"].values, rtol=1e-12)
+    for k in ("n", "m", "hi"):
+        np.testing.assert_array_equal(second[k].values, first[k].values)
