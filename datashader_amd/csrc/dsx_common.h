@@ -31,8 +31,7 @@ __device__ __forceinline__ int64_t load_code(const void *p, int32_t dt, int64_t 
     }
 }
 
-// by()'s categorizer (reductions.py:143-259).  Returns the category of row i in
-// [0, ncat), or -1 when the row has no category inside the aggregate.
+// by()'s categorizer (reductions.py:143-259): codes, modulo or binning of one column.
 struct Cat {
     int32_t kind, dtype, ncat, nbins, bin_under, bin_over;
     const void *col;
@@ -49,7 +48,19 @@ inline Cat cat_of(const dsx_categorizer &c) {
     return k;
 }
 
-__device__ __forceinline__ int64_t category(const Cat &c, int64_t i) {
+// The categorizer's input for row i as 64 bits: the integer code, or the bits of the
+// value as a double for a float column (exact for float32).
+__device__ __forceinline__ int64_t load_cat_raw(const Cat &c, int64_t i) {
+    switch (c.dtype) {
+    case DSX_F32: return __double_as_longlong((double)reinterpret_cast<const float *>(c.col)[i]);
+    case DSX_F64: return __double_as_longlong(reinterpret_cast<const double *>(c.col)[i]);
+    default: return load_code(c.col, c.dtype, i);
+    }
+}
+
+// Category of a row from its raw input (load_cat_raw), in [0, ncat), or -1 when the
+// row has no category inside the aggregate.  Registers only: no memory access.
+__device__ __forceinline__ int64_t category_raw(const Cat &c, int64_t raw) {
     if (c.kind == DSX_CAT_NONE) return 0;
     int64_t k;
     if (c.kind == DSX_CAT_BINNING) {
@@ -58,12 +69,11 @@ __device__ __forceinline__ int64_t category(const Cat &c, int64_t i) {
         bool nan_v;
         int64_t idx = 0;
         if (c.dtype == DSX_F32) {
-            const float v = reinterpret_cast<const float *>(c.col)[i];
+            const float v = (float)__longlong_as_double(raw);
             nan_v = v != v;
             if (!nan_v) idx = (int64_t)((v - (float)c.bin0) / (float)c.binsize);
         } else {
-            const double v = c.dtype == DSX_F64 ? reinterpret_cast<const double *>(c.col)[i]
-                                                : (double)load_code(c.col, c.dtype, i);
+            const double v = c.dtype == DSX_F64 ? __longlong_as_double(raw) : (double)raw;
             nan_v = v != v;
             if (!nan_v) idx = (int64_t)((v - c.bin0) / c.binsize);
         }
@@ -71,19 +81,22 @@ __device__ __forceinline__ int64_t category(const Cat &c, int64_t i) {
         else if (idx < 0) k = c.bin_under;
         else if (idx >= c.nbins) k = c.bin_over;
         else k = idx;
+    } else if (c.kind == DSX_CAT_MODULO) {
+        int64_t r = (raw - c.offset) % c.modulo;  // Python's floor modulo
+        if (r != 0 && ((r < 0) != (c.modulo < 0))) r += c.modulo;
+        k = r;
     } else {
-        const int64_t raw = load_code(c.col, c.dtype, i);
-        if (c.kind == DSX_CAT_MODULO) {
-            int64_t r = (raw - c.offset) % c.modulo;  // Python's floor modulo
-            if (r != 0 && ((r < 0) != (c.modulo < 0))) r += c.modulo;
-            k = r;
-        } else {
-            k = raw;
-        }
+        k = raw;
     }
     if (k < 0) k += c.ncat;  // numba's negative-index wraparound (code -1 = NaN category)
     if (k < 0 || k >= c.ncat) return -1;
     return k;
+}
+
+// by()'s categorizer (reductions.py:143-259) applied to row i.
+__device__ __forceinline__ int64_t category(const Cat &c, int64_t i) {
+    if (c.kind == DSX_CAT_NONE) return 0;
+    return category_raw(c, load_cat_raw(c, i));
 }
 
 static inline int check_cat(const dsx_categorizer &c) {

@@ -93,10 +93,10 @@ static Lay make_lay(uint32_t flags, int keyb, uint32_t cap) {
 }
 
 // ---- projection -----------------------------------------------------------------------
-// Pixel and category of row i; false when the row is dropped.
+// Pixel of a row from its coordinates (glyphs/points.py:197-206); false when dropped.
 template <typename TC>
-__device__ __forceinline__ bool locate(const PointArgs &a, int64_t i, TC xc, TC yc,
-                                       int32_t &px, int32_t &py, int32_t &pc) {
+__device__ __forceinline__ bool locate_xy(const PointArgs &a, TC xc, TC yc, int32_t &px,
+                                          int32_t &py) {
     const double x = (double)xc, y = (double)yc;
     if (!((a.xmin <= x && x <= a.xmax) && (a.ymin <= y && y <= a.ymax))) return false;
     int32_t xx = proj(axis_map(x, a.xlog), a.sx, a.tx);
@@ -106,9 +106,18 @@ __device__ __forceinline__ bool locate(const PointArgs &a, int64_t i, TC xc, TC 
     if (xx < 0) xx += a.W;  // numba's negative-index wraparound
     if (yy < 0) yy += a.H;
     if ((uint32_t)xx >= (uint32_t)a.W || (uint32_t)yy >= (uint32_t)a.H) return false;
+    px = xx; py = yy;
+    return true;
+}
+
+// Pixel and category of row i; false when the row is dropped.
+template <typename TC>
+__device__ __forceinline__ bool locate(const PointArgs &a, int64_t i, TC xc, TC yc,
+                                       int32_t &px, int32_t &py, int32_t &pc) {
+    if (!locate_xy<TC>(a, xc, yc, px, py)) return false;
     const int64_t c = category(a.cat, i);
     if (c < 0) return false;
-    px = xx; py = yy; pc = (int32_t)c;
+    pc = (int32_t)c;
     return true;
 }
 
@@ -304,28 +313,60 @@ struct ScanArgs {
     const double *mean;              // squared-deviation mode (m2 pass)
 };
 
-template <typename TC, typename TV, bool HAS_VAL>
-__device__ __forceinline__ void load_chunk(const PointArgs &a, int64_t ck, int tid, TC *xs,
-                                           TC *ys, TV *vs) {
-    const TC *X = reinterpret_cast<const TC *>(a.x);
-    const TC *Y = reinterpret_cast<const TC *>(a.y);
-    const TV *V = reinterpret_cast<const TV *>(a.val);
-    const int64_t base = ck * kChunk;
+// How the scan reads a row's category: no categorizer, a one-byte code column (int8 /
+// uint8), or any other categorizer column.
+enum : int { CK_NONE = 0, CK_BYTE = 1, CK_ANY = 2 };
+
+// Rows of chunk ck handled by this thread.  Every load is issued unconditionally (a row
+// past the end reads the last row and is dropped when binned), so the loads of a chunk
+// are a fixed-length run and the compiler waits for exactly the chunk it bins while the
+// chunks prefetched behind it stay in flight.  Loaded values are not touched here.
+template <typename TC, typename TV, bool HAS_VAL, int CK>
+struct Rows {
+    TC x[kItems], y[kItems];
+    TV v[kItems];
+    int64_t c[kItems];
+
+    __device__ __forceinline__ void load(const PointArgs &a, int64_t ck, int tid) {
+        const TC *X = reinterpret_cast<const TC *>(a.x);
+        const TC *Y = reinterpret_cast<const TC *>(a.y);
+        const TV *V = reinterpret_cast<const TV *>(a.val);
+        const int64_t last = a.n > 0 ? a.n - 1 : 0;  // an empty input reads a dummy row
+        const int64_t base = ck * kChunk;
 #pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-        const int64_t i = base + it * kScanThreads + tid;
-        xs[it] = TC(0);
-        ys[it] = TC(0);
-        vs[it] = TV(0);
-        if (i < a.n) {
-            xs[it] = X[i];
-            ys[it] = Y[i];
-            if constexpr (HAS_VAL) vs[it] = V[i];
+        for (int it = 0; it < kItems; ++it) {
+            int64_t i = base + it * kScanThreads + tid;
+            i = i < last ? i : last;
+            x[it] = X[i];
+            y[it] = Y[i];
+            if constexpr (HAS_VAL) v[it] = V[i];
+            else v[it] = TV(0);
+            if constexpr (CK == CK_BYTE) c[it] = reinterpret_cast<const uint8_t *>(a.cat.col)[i];
+            else if constexpr (CK == CK_ANY) c[it] = load_cat_raw(a.cat, i);
+            else c[it] = 0;
         }
     }
+
+    __device__ __forceinline__ void take(const Rows &o) {
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            x[it] = o.x[it];
+            y[it] = o.y[it];
+            v[it] = o.v[it];
+            c[it] = o.c[it];
+        }
+    }
+};
+
+// Category of a scanned row from its loaded raw input; -1 when dropped.
+template <int CK>
+__device__ __forceinline__ int32_t scan_category(const Cat &c, int64_t raw) {
+    if constexpr (CK == CK_NONE) return 0;
+    if constexpr (CK == CK_BYTE) raw = c.dtype == DSX_I8 ? (int64_t)(int8_t)(uint8_t)raw : raw;
+    return (int32_t)category_raw(c, raw);
 }
 
-template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
+template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV, int CK>
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArgs s) {
     extern __shared__ __align__(16) uint8_t smem[];
     uint8_t *win = smem;
@@ -342,22 +383,28 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
     TV *spill_val = reinterpret_cast<TV *>(s.spill_val);
     const int lane = threadIdx.x & 63;
     const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
-    // software pipeline: the next chunk's loads are in flight while this one is binned
-    TC xa[kItems], ya[kItems];
-    TV va[kItems];
-    load_chunk<TC, TV, HAS_VAL>(a, blockIdx.x, tid, xa, ya, va);
+    // software pipeline, two chunks deep: while chunk ck is binned, the loads of chunks
+    // ck + G and ck + 2G are in flight (the spill stores of chunk ck - G complete behind
+    // them, so waiting for chunk ck never waits for the latest stores)
+    using R = Rows<TC, TV, HAS_VAL, CK>;
+    R cur, nxt;
+    cur.load(a, blockIdx.x, tid);
+    nxt.load(a, (int64_t)blockIdx.x + gridDim.x, tid);
     for (int64_t ck = blockIdx.x; ck < nchunks; ck += gridDim.x) {
-        TC xb[kItems], yb[kItems];
-        TV vb[kItems];
-        load_chunk<TC, TV, HAS_VAL>(a, ck + gridDim.x, tid, xb, yb, vb);
+        R far;
+        far.load(a, ck + 2 * (int64_t)gridDim.x, tid);
         const int64_t base = ck * kChunk;
 #pragma unroll
         for (int it = 0; it < kItems; ++it) {
             const int64_t i = base + it * kScanThreads + tid;
             int32_t px = 0, py = 0, pc = 0;
-            const bool keep = i < a.n && locate<TC>(a, i, xa[it], ya[it], px, py, pc);
+            bool keep = i < a.n && locate_xy<TC>(a, cur.x[it], cur.y[it], px, py);
+            if constexpr (CK != CK_NONE) {
+                pc = scan_category<CK>(a.cat, cur.c[it]);
+                keep = keep && pc >= 0;
+            }
             const bool inw = keep && in_window(w, px, py);
-            const double v = HAS_VAL ? (double)va[it] : 0.0;
+            const double v = HAS_VAL ? (double)cur.v[it] : 0.0;
             const bool vnan = HAS_VAL ? (v != v) : true;
             if (inw) {
                 const uint32_t lc = (uint32_t)(((py - w.y0) * w.w + (px - w.x0)) * ncat + pc);
@@ -377,18 +424,14 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
                 const size_t slot = filled + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                 if (slot < (size_t)s.seg_cap) {
                     s.spill_cell[seg0 + slot] = cell;
-                    if constexpr (HAS_VAL) spill_val[seg0 + slot] = va[it];
+                    if constexpr (HAS_VAL) spill_val[seg0 + slot] = cur.v[it];
                 }
                 atomicAdd(&thist[cell / s.tile_cells], 1u);
             }
             filled += (uint32_t)__popcll(m);
         }
-#pragma unroll
-        for (int it = 0; it < kItems; ++it) {
-            xa[it] = xb[it];
-            ya[it] = yb[it];
-            va[it] = vb[it];
-        }
+        cur.take(nxt);
+        nxt.take(far);
     }
     __syncthreads();
     // this workgroup's window partial, tile histogram and segment length
@@ -840,11 +883,21 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     sa.tile_cells = P.tl.cap;
     sa.ntiles = P.ntiles;
     sa.mean = mean;
-    auto scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV>;
+    // the scan reads row 0 of each column even for an empty input (and drops it): give
+    // an empty input a readable dummy row in the workspace
+    PointArgs sa_a = a;
+    if (a.n == 0) {
+        sa_a.x = sa_a.y = sa_a.val = hist_sel;
+        sa_a.cat.col = hist_sel;
+    }
+    auto scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_ANY>;
+    if (a.cat.kind == DSX_CAT_NONE) scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE>;
+    else if (a.cat.dtype == DSX_I8 || a.cat.dtype == DSX_U8)
+        scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_BYTE>;
     hipError_t e = hipFuncSetAttribute((const void *)scan,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.scan_lds);
     if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
-    hipLaunchKernelGGL(scan, dim3(P.scan_groups), dim3(kScanThreads), P.scan_lds, st, a, sa);
+    hipLaunchKernelGGL(scan, dim3(P.scan_groups), dim3(kScanThreads), P.scan_lds, st, sa_a, sa);
     if ((s = launch_status())) return s;
     hipLaunchKernelGGL((merge_window_kernel<KEYB>), dim3((P.wl.cap + kMergeCells - 1) / kMergeCells),
                        dim3(kMergeCells * kMergeWays), 0, st, P.wl, (const uint8_t *)sa.partials,
