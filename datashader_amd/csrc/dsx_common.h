@@ -140,8 +140,10 @@ __device__ __forceinline__ int32_t proj(double x, double s, double t) {
     return (int32_t)__dadd_rn(__dmul_rn(x, s), t);
 }
 
-// Axis mapper: identity (LinearAxis) or log10 (LogAxis, core.py:127-145).
-__device__ __forceinline__ double axis_map(double v, int32_t log) { return log ? log10(v) : v; }
+// Axis mapper: identity (LinearAxis) or log10 (LogAxis, core.py:127-145).  log10 is an
+// out-of-line call so that the linear path of a kernel's unrolled loops stays small.
+static __device__ __noinline__ double axis_log10(double v) { return log10(v); }
+__device__ __forceinline__ double axis_map(double v, int32_t log) { return log ? axis_log10(v) : v; }
 
 // ---- aggregate planes ----------------------------------------------------------------
 template <int KEYB> struct KeyOf;

@@ -34,7 +34,12 @@ constexpr int kItems = 4;
 constexpr int kChunk = kScanThreads * kItems;  // rows per workgroup iteration
 constexpr int kLdsBytes = 160 * 1024;           // LDS of one gfx950 workgroup
 constexpr int kCtrlBytes = 64;
+constexpr int kFlush = 32;                      // spill entries a wave writes per store
+constexpr int kStage = kFlush + 64;             // staged spill entries per wave in LDS
+static_assert(kStage >= (kFlush - 1) + 64, "a wave's staging holds a partial line + one item");
 constexpr int kPartThreads = 1024;
+constexpr int kTileThreads = 256;
+constexpr int kTileLds = 40 * 1024;             // target LDS of one tile workgroup
 constexpr int kMaxTiles = 4096;
 constexpr int kSelBins = 8192;                  // coarse density histogram bins
 constexpr int kSampleRuns = 256;
@@ -310,6 +315,7 @@ struct ScanArgs {
     uint32_t *hist;                  // [G][ntiles] spilled entries per tile
     uint32_t tile_cells;
     int32_t ntiles;
+    uint32_t o_thist, o_stage_cell, o_stage_val;  // LDS byte offsets
     const double *mean;              // squared-deviation mode (m2 pass)
 };
 
@@ -328,32 +334,31 @@ struct Rows {
     int64_t c[kItems];
 
     __device__ __forceinline__ void load(const PointArgs &a, int64_t ck, int tid) {
-        const TC *X = reinterpret_cast<const TC *>(a.x);
-        const TC *Y = reinterpret_cast<const TC *>(a.y);
-        const TV *V = reinterpret_cast<const TV *>(a.val);
-        const int64_t last = a.n > 0 ? a.n - 1 : 0;  // an empty input reads a dummy row
-        const int64_t base = ck * kChunk;
-#pragma unroll
-        for (int it = 0; it < kItems; ++it) {
-            int64_t i = base + it * kScanThreads + tid;
-            i = i < last ? i : last;
-            x[it] = X[i];
-            y[it] = Y[i];
-            if constexpr (HAS_VAL) v[it] = V[i];
-            else v[it] = TV(0);
-            if constexpr (CK == CK_BYTE) c[it] = reinterpret_cast<const uint8_t *>(a.cat.col)[i];
-            else if constexpr (CK == CK_ANY) c[it] = load_cat_raw(a.cat, i);
-            else c[it] = 0;
+        // chunk base and last readable in-chunk row are uniform; each thread adds a
+        // 32-bit offset (scalar base + vector offset addressing)
+        int64_t b = ck * kChunk;
+        uint32_t lastj;
+        if (b >= a.n) {  // a prefetch past the end: re-read the last row
+            b = a.n > 0 ? a.n - 1 : 0;
+            lastj = 0;
+        } else {
+            const int64_t rem = a.n - b;
+            lastj = rem >= kChunk ? (uint32_t)(kChunk - 1) : (uint32_t)(rem - 1);
         }
-    }
-
-    __device__ __forceinline__ void take(const Rows &o) {
+        const TC *X = reinterpret_cast<const TC *>(a.x) + b;
+        const TC *Y = reinterpret_cast<const TC *>(a.y) + b;
+        const TV *V = reinterpret_cast<const TV *>(a.val) + (HAS_VAL ? b : 0);
 #pragma unroll
         for (int it = 0; it < kItems; ++it) {
-            x[it] = o.x[it];
-            y[it] = o.y[it];
-            v[it] = o.v[it];
-            c[it] = o.c[it];
+            uint32_t j = (uint32_t)(it * kScanThreads + tid);
+            j = j < lastj ? j : lastj;
+            x[it] = X[j];
+            y[it] = Y[j];
+            if constexpr (HAS_VAL) v[it] = V[j];
+            else v[it] = TV(0);
+            if constexpr (CK == CK_BYTE) c[it] = reinterpret_cast<const uint8_t *>(a.cat.col)[b + j];
+            else if constexpr (CK == CK_ANY) c[it] = load_cat_raw(a.cat, b + j);
+            else c[it] = 0;
         }
     }
 };
@@ -370,7 +375,7 @@ template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV, int CK>
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArgs s) {
     extern __shared__ __align__(16) uint8_t smem[];
     uint8_t *win = smem;
-    uint32_t *thist = reinterpret_cast<uint32_t *>(smem + s.wl.bytes);  // [ntiles]
+    uint32_t *thist = reinterpret_cast<uint32_t *>(smem + s.o_thist);  // [ntiles]
     const int tid = threadIdx.x;
     const int ncat = a.cat.ncat;
     const Window w = load_window(s.window, a.W, a.H, (int64_t)(s.wl.cap / (uint32_t)ncat));
@@ -379,26 +384,30 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
     __syncthreads();
     const int wave = tid >> 6;
     const size_t seg0 = ((size_t)blockIdx.x * kWaves + wave) * (size_t)s.seg_cap;
-    uint32_t filled = 0;  // wave-uniform
+    // spills are staged per wave in LDS and written kFlush at a time as whole cache lines
+    uint32_t *st_cell = reinterpret_cast<uint32_t *>(smem + s.o_stage_cell) + wave * kStage;
+    TV *st_val = reinterpret_cast<TV *>(smem + s.o_stage_val) + wave * kStage;
+    uint32_t staged = 0, flushed = 0;  // wave-uniform
     TV *spill_val = reinterpret_cast<TV *>(s.spill_val);
     const int lane = threadIdx.x & 63;
     const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
     // software pipeline, two chunks deep: while chunk ck is binned, the loads of chunks
-    // ck + G and ck + 2G are in flight (the spill stores of chunk ck - G complete behind
-    // them, so waiting for chunk ck never waits for the latest stores)
+    // ck + G and ck + 2G are in flight.  Three register sets rotate roles (binned / in
+    // flight / being loaded), so no set is ever copied: a copy would wait for its loads.
     using R = Rows<TC, TV, HAS_VAL, CK>;
-    R cur, nxt;
-    cur.load(a, blockIdx.x, tid);
-    nxt.load(a, (int64_t)blockIdx.x + gridDim.x, tid);
-    for (int64_t ck = blockIdx.x; ck < nchunks; ck += gridDim.x) {
-        R far;
+    R ra, rb, rc;
+    ra.load(a, blockIdx.x, tid);
+    rb.load(a, (int64_t)blockIdx.x + gridDim.x, tid);
+    int64_t ck = blockIdx.x;
+    auto step = [&](R &cur, R &far) {
         far.load(a, ck + 2 * (int64_t)gridDim.x, tid);
-        const int64_t base = ck * kChunk;
+        const int64_t rem64 = a.n - ck * kChunk;  // rows of this chunk, uniform
+        const uint32_t rem = rem64 >= kChunk ? (uint32_t)kChunk : (uint32_t)rem64;
 #pragma unroll
         for (int it = 0; it < kItems; ++it) {
-            const int64_t i = base + it * kScanThreads + tid;
+            const uint32_t j = (uint32_t)(it * kScanThreads + tid);
             int32_t px = 0, py = 0, pc = 0;
-            bool keep = i < a.n && locate_xy<TC>(a, cur.x[it], cur.y[it], px, py);
+            bool keep = j < rem && locate_xy<TC>(a, cur.x[it], cur.y[it], px, py);
             if constexpr (CK != CK_NONE) {
                 pc = scan_category<CK>(a.cat, cur.c[it]);
                 keep = keep && pc >= 0;
@@ -416,22 +425,58 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
                 }
                 region_add<KEYB>(win, s.wl, lc, vnan, v, add);
             }
-            // spill: appended to this wave's sub-segment at the wave's running count
+            // spill: staged at the wave's running count, then appended to its sub-segment
             const bool spill = keep && !inw;
             const unsigned long long m = __ballot(spill);
             if (spill) {
                 const uint32_t cell = (uint32_t)(((int64_t)py * a.W + px) * ncat + pc);
-                const size_t slot = filled + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                if (slot < (size_t)s.seg_cap) {
-                    s.spill_cell[seg0 + slot] = cell;
-                    if constexpr (HAS_VAL) spill_val[seg0 + slot] = cur.v[it];
-                }
+                const uint32_t slot = staged + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                st_cell[slot] = cell;
+                if constexpr (HAS_VAL) st_val[slot] = cur.v[it];
                 atomicAdd(&thist[cell / s.tile_cells], 1u);
             }
-            filled += (uint32_t)__popcll(m);
+            staged += (uint32_t)__popcll(m);
+            // staged < kFlush before the item, so staged < kFlush + 64 <= kStage here;
+            // write whole lines until fewer than kFlush entries remain
+            while (staged >= (uint32_t)kFlush) {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                if (lane < kFlush) {
+                    const size_t slot = (size_t)flushed + lane;
+                    const uint32_t c = st_cell[lane];
+                    TV vv = TV(0);
+                    if constexpr (HAS_VAL) vv = st_val[lane];
+                    if (slot < (size_t)s.seg_cap) {
+                        s.spill_cell[seg0 + slot] = c;
+                        if constexpr (HAS_VAL) spill_val[seg0 + slot] = vv;
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                const uint32_t rest = staged - kFlush;  // < 64: move it to the front
+                if ((uint32_t)lane < rest) {
+                    st_cell[lane] = st_cell[kFlush + lane];
+                    if constexpr (HAS_VAL) st_val[lane] = st_val[kFlush + lane];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                flushed += kFlush;
+                staged = rest;
+            }
         }
-        cur.take(nxt);
-        nxt.take(far);
+        ck += gridDim.x;
+    };
+    while (ck < nchunks) {
+        step(ra, rc);
+        if (ck >= nchunks) break;
+        step(rb, ra);
+        if (ck >= nchunks) break;
+        step(rc, rb);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if ((uint32_t)lane < staged) {  // the last partial line
+        const size_t slot = (size_t)flushed + lane;
+        if (slot < (size_t)s.seg_cap) {
+            s.spill_cell[seg0 + slot] = st_cell[lane];
+            if constexpr (HAS_VAL) spill_val[seg0 + slot] = st_val[lane];
+        }
     }
     __syncthreads();
     // this workgroup's window partial, tile histogram and segment length
@@ -440,9 +485,11 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
     for (uint32_t j = tid; j < s.wl.bytes / 16u; j += kScanThreads) dst[j] = src[j];
     for (int t = tid; t < s.ntiles; t += kScanThreads)
         s.hist[(size_t)blockIdx.x * s.ntiles + t] = thist[t];
-    if (lane == 0)
+    if (lane == 0) {
+        const uint32_t filled = flushed + staged;
         s.seg_count[(size_t)blockIdx.x * kWaves + wave] =
             filled < (uint32_t)s.seg_cap ? filled : (uint32_t)s.seg_cap;
+    }
 }
 
 // ---- merge the window partials --------------------------------------------------------------
@@ -521,7 +568,7 @@ __global__ __launch_bounds__(kMergeCells * kMergeWays) void merge_window_kernel(
 struct PartArgs {
     const uint32_t *cell;
     const void *val;
-    const uint32_t *seg_count;        // [G]
+    const uint32_t *seg_count;        // [G * kWaves]
     int64_t seg_cap;
     int32_t ngroups;                  // G (<= kPartThreads)
     uint32_t tile_cells;
@@ -624,27 +671,50 @@ __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p)
     }
     __syncthreads();
     const unsigned long long total = pre[kWaves];
+    if (total == 0) return;
     const unsigned long long base = (unsigned long long)g * kWaves * (unsigned long long)p.seg_cap;
     const TV *val = reinterpret_cast<const TV *>(p.val);
     constexpr int kPer = kSub / kPartThreads;
-    for (unsigned long long c0 = 0; c0 < total; c0 += kSub) {
-        uint32_t rc[kPer], rt[kPer], rr[kPer];
+    // entries of one round in registers; every load is issued (an index past the end
+    // re-reads the last entry), so the next round's loads stay in flight behind this one
+    struct Round {
+        uint32_t rc[kPer];
         TV rv[kPer];
+    };
+    auto load_round = [&](Round &r, unsigned long long c0) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            unsigned long long v = c0 + (unsigned long long)(k * kPartThreads + threadIdx.x);
+            v = v < total ? v : total - 1;
+            int wv = 0;  // the last sub-segment starting at or before v (pre is sorted)
+#pragma unroll
+            for (int step = kWaves / 2; step >= 1; step >>= 1)
+                if (pre[wv + step] <= v) wv += step;
+            const unsigned long long j = base + (unsigned long long)wv * p.seg_cap + (v - pre[wv]);
+            r.rc[k] = p.cell[j];
+            if constexpr (HAS_VAL) r.rv[k] = val[j];
+            else r.rv[k] = TV(0);
+        }
+    };
+    Round nxt;
+    load_round(nxt, 0);
+    for (unsigned long long c0 = 0; c0 < total; c0 += kSub) {
+        Round r = nxt;
+        if (c0 + kSub < total) load_round(nxt, c0 + kSub);
+        uint32_t rt[kPer], rr[kPer];
+        const uint32_t *rc = r.rc;
+        const TV *rv = r.rv;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const unsigned long long v = c0 + (unsigned long long)(k * kPartThreads + threadIdx.x);
             rt[k] = 0xFFFFFFFFu;
-            rc[k] = 0u;
             rr[k] = 0u;
-            rv[k] = TV(0);
             if (v < total) {
-                int wv = 0;
-                while (wv + 1 < kWaves && pre[wv + 1] <= v) ++wv;
-                const unsigned long long j = base + (unsigned long long)wv * p.seg_cap + (v - pre[wv]);
-                rc[k] = p.cell[j];
-                if constexpr (HAS_VAL) rv[k] = val[j];
-                rt[k] = rc[k] / p.tile_cells;
-                rr[k] = atomicAdd(&cnt[rt[k]], 1u);
+                const uint32_t t = rc[k] / p.tile_cells;
+                if (t < (uint32_t)p.ntiles) {  // always, for entries the scan wrote
+                    rt[k] = t;
+                    rr[k] = atomicAdd(&cnt[t], 1u);
+                }
             }
         }
         __syncthreads();
@@ -689,56 +759,83 @@ __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p)
 }
 
 // ---- per-tile aggregation of the sorted spill ------------------------------------------------
+// Entries [b, e) of one tile's sorted spill, kU per thread per step, loaded two steps
+// ahead with unconditional loads (an index past the end re-reads entry b and is dropped).
+template <typename TV, bool HAS_VAL>
+struct TileRows {
+    static constexpr int kU = 4;
+    uint32_t gc[kU];
+    TV vv[kU];
+
+    __device__ __forceinline__ void load(const PartArgs &p, unsigned long long j0,
+                                         unsigned long long b, unsigned long long e) {
+        const TV *val = reinterpret_cast<const TV *>(p.sorted_val);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            unsigned long long j = j0 + (unsigned long long)(u * kTileThreads + threadIdx.x);
+            j = j < e ? j : b;
+            gc[u] = p.sorted_cell[j];
+            if constexpr (HAS_VAL) vv[u] = val[j];
+            else vv[u] = TV(0);
+        }
+    }
+};
+
 template <typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
-__global__ __launch_bounds__(kPartThreads) void tile_agg_kernel(PartArgs p, Lay L,
+__global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay L,
                                                                const int32_t *window, int32_t W,
                                                                int32_t H, int64_t wcap_px,
                                                                int32_t ncat, int64_t ncells,
                                                                const double *mean, Planes out) {
     extern __shared__ __align__(16) uint8_t smem[];
-    constexpr int kU = 4;  // entries in flight per thread
+    using R = TileRows<TV, HAS_VAL>;
+    constexpr unsigned long long kStep = (unsigned long long)R::kU * kTileThreads;
     const int t = blockIdx.x;
-    region_init<KEYB>(smem, L, threadIdx.x, blockDim.x);
+    region_init<KEYB>(smem, L, threadIdx.x, kTileThreads);
     __syncthreads();
     const unsigned long long b = p.tile_start[t], e = p.tile_start[t + 1];
     const int64_t c0 = (int64_t)t * p.tile_cells;
-    const TV *val = reinterpret_cast<const TV *>(p.sorted_val);
-    for (unsigned long long j0 = b; j0 < e; j0 += (unsigned long long)kU * blockDim.x) {
-        uint32_t gc[kU];
-        TV vv[kU];
+    if (b < e) {
+        // three register sets rotate roles, as in the scan
+        R ra, rb, rc;
+        ra.load(p, b, b, e);
+        rb.load(p, b + kStep, b, e);
+        unsigned long long j0 = b;
+        auto step = [&](R &cur, R &far) {
+            far.load(p, j0 + 2 * kStep, b, e);
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const unsigned long long j = j0 + (unsigned long long)(u * blockDim.x + threadIdx.x);
-            gc[u] = 0xFFFFFFFFu;
-            vv[u] = TV(0);
-            if (j < e) {
-                gc[u] = p.sorted_cell[j];
-                if constexpr (HAS_VAL) vv[u] = val[j];
+            for (int u = 0; u < R::kU; ++u) {
+                if (j0 + (unsigned long long)(u * kTileThreads + threadIdx.x) >= e) continue;
+                const uint32_t lc = (uint32_t)(cur.gc[u] - c0);
+                if (lc >= L.cap) continue;  // never, for entries the scatter wrote
+                double v = 0.0;
+                bool vnan = true;
+                if constexpr (HAS_VAL) {
+                    v = (double)cur.vv[u];
+                    vnan = v != v;
+                }
+                double add = v;
+                if constexpr (SQDEV) {
+                    const double d = v - mean[cur.gc[u]];
+                    add = d * d;
+                }
+                region_add<KEYB>(smem, L, lc, vnan, v, add);
             }
-        }
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            if (gc[u] == 0xFFFFFFFFu) continue;
-            const uint32_t lc = (uint32_t)(gc[u] - c0);
-            double v = 0.0;
-            bool vnan = true;
-            if constexpr (HAS_VAL) {
-                v = (double)vv[u];
-                vnan = v != v;
-            }
-            double add = v;
-            if constexpr (SQDEV) {
-                const double d = v - mean[gc[u]];
-                add = d * d;
-            }
-            region_add<KEYB>(smem, L, lc, vnan, v, add);
+            j0 += kStep;
+        };
+        while (j0 < e) {
+            step(ra, rc);
+            if (j0 >= e) break;
+            step(rb, ra);
+            if (j0 >= e) break;
+            step(rc, rb);
         }
     }
     __syncthreads();
     const Window w = load_window(window, W, H, wcap_px);
     int64_t c1 = c0 + p.tile_cells;
     if (c1 > ncells) c1 = ncells;
-    for (int64_t gcell = c0 + threadIdx.x; gcell < c1; gcell += blockDim.x) {
+    for (int64_t gcell = c0 + threadIdx.x; gcell < c1; gcell += kTileThreads) {
         const int64_t pix = gcell / ncat;
         const int32_t py = (int32_t)(pix / W), px = (int32_t)(pix - (int64_t)py * W);
         if (in_window(w, px, py)) continue;  // written by the window merge
@@ -757,6 +854,7 @@ struct Plan {
     int scan_groups;
     int64_t seg_cap;
     size_t scan_lds, tile_lds;
+    uint32_t o_thist, o_stage_cell, o_stage_val;  // scan LDS offsets
     // workspace offsets
     size_t o_window, o_hist_sel, o_count, o_partials, o_spill_cell, o_spill_val, o_hist,
         o_totals, o_tile_start, o_sorted_cell, o_sorted_val, total;
@@ -776,21 +874,31 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     P.ncells = (int64_t)v->width * v->height * ncat;
     if (P.ncells >= (1ll << 32)) return DSX_ERR_SHAPE;
     const uint32_t cb = cell_bytes(P.flags, P.keyb);
-    // tiles: equal runs of cells whose planes fit LDS
-    int64_t tcap = (kLdsBytes - 16) / cb;
+    // tiles: equal runs of cells whose planes fit kTileLds (several tile workgroups share
+    // a CU), or up to the whole LDS when that would make more than kMaxTiles tiles
+    int64_t tcap = (kTileLds - 16) / cb;
+    if ((P.ncells + tcap - 1) / tcap > kMaxTiles) {
+        tcap = (P.ncells + kMaxTiles - 1) / kMaxTiles;
+        if (tcap * (int64_t)cb > kLdsBytes - 16) return DSX_ERR_SHAPE;
+    }
     P.ntiles = (int32_t)((P.ncells + tcap - 1) / tcap);
-    if (P.ntiles > kMaxTiles) return DSX_ERR_SHAPE;
+    if (P.ntiles < 1) P.ntiles = 1;
     tcap = (P.ncells + P.ntiles - 1) / P.ntiles;
     P.tl = make_lay(P.flags, P.keyb, (uint32_t)tcap);
     P.tile_lds = P.tl.bytes;
     // window capacity: LDS minus the scan's tile histogram and control words
-    const size_t side = (size_t)4 * P.ntiles + kCtrlBytes;
+    // scan LDS: window planes, tile histogram, per-wave spill staging, control words
+    const size_t stage = (size_t)kWaves * kStage * (4 + (size_t)P.vbytes);
+    const size_t side = align_up((size_t)4 * P.ntiles, 16) + stage + kCtrlBytes;
     const int64_t wcap_cells = (int64_t)((kLdsBytes - side - 16) / cb);
     int64_t wcap_px = wcap_cells / ncat;
     if (wcap_px < 1) return DSX_ERR_SHAPE;
     P.whole = (int64_t)v->width * v->height <= wcap_px;
     if (P.whole) wcap_px = (int64_t)v->width * v->height;
     P.wl = make_lay(P.flags, P.keyb, (uint32_t)(wcap_px * ncat));
+    P.o_thist = P.wl.bytes;
+    P.o_stage_cell = P.o_thist + (uint32_t)align_up((size_t)4 * P.ntiles, 16);
+    P.o_stage_val = P.o_stage_cell + (uint32_t)(kWaves * kStage * 4);
     P.scan_lds = P.wl.bytes + side;
     // coarse selection grid: bins of 2^bshift pixels on a side, at most kSelBins bins
     P.bshift = 0;
@@ -882,6 +990,9 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     sa.hist = reinterpret_cast<uint32_t *>(ws + P.o_hist);
     sa.tile_cells = P.tl.cap;
     sa.ntiles = P.ntiles;
+    sa.o_thist = P.o_thist;
+    sa.o_stage_cell = P.o_stage_cell;
+    sa.o_stage_val = P.o_stage_val;
     sa.mean = mean;
     // the scan reads row 0 of each column even for an empty input (and drops it): give
     // an empty input a readable dummy row in the workspace
@@ -926,7 +1037,7 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     e = hipFuncSetAttribute((const void *)tile, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)P.tile_lds);
     if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
-    hipLaunchKernelGGL(tile, dim3(P.ntiles), dim3(kPartThreads), P.tile_lds, st, pa, P.tl,
+    hipLaunchKernelGGL(tile, dim3(P.ntiles), dim3(kTileThreads), P.tile_lds, st, pa, P.tl,
                        (const int32_t *)window, a.W, a.H, wcap_px, ncat, P.ncells, mean, out);
     return launch_status();
 }
