@@ -217,6 +217,18 @@ def build_workload(name, torch, scale=1.0):
     raise SystemExit(f"unknown workload {name!r}")
 
 
+def _traffic(workload):
+    """HBM bytes per aggregation call from the committed counter summary (two rocprofv3
+    --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            entry = json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+    return None if entry is None else entry.get("bytes_per_call")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +240,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=20_000_000,
                     help="rows (or line samples) of the CPU oracle baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one plain 1 GiB device copy (the reference dispatch that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -247,6 +262,13 @@ def main():
         res = step()
     torch.cuda.synchronize()
     _log(f"[bench] warmup done ({args.warmup} steps)")
+    if args.copy_calib:
+        src = torch.ones(1 << 30, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        _log("[bench] 1 GiB copy done")
 
     timer = KernelTimer(torch)
     backend.set_kernel_timer(timer)
@@ -280,10 +302,11 @@ def main():
         else "Gsamples/s aggregated (line vertices) + % HBM roofline",
         "value": value, "unit": unit_rows, "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 projection)",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded, generated on the device)", "config": cfg,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": _traffic(args.workload),
                      "kernel_ms": kern_ms, "algorithmic_bytes": algo},
         "cpu_baseline": cpu_baseline,
         "checksum": checksum,

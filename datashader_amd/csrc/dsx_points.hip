@@ -39,8 +39,11 @@ constexpr int kStage = kFlush + 64;             // staged spill entries per wave
 static_assert(kStage >= (kFlush - 1) + 64, "a wave's staging holds a partial line + one item");
 constexpr int kPartThreads = 1024;
 constexpr int kTileThreads = 256;
-constexpr int kTileLds = 40 * 1024;             // target LDS of one tile workgroup
+// target LDS of one tile workgroup: a quarter of the CU's LDS less one 512-byte
+// allocation granule, so that four tile workgroups fit on a CU
+constexpr int kTileLds = 39 * 1024;
 constexpr int kMaxTiles = 4096;
+constexpr int kMinPart = 4096;                  // fewest spill entries a tile part is given
 constexpr int kSelBins = 8192;                  // coarse density histogram bins
 constexpr int kSampleRuns = 256;
 constexpr int kSampleRun = 4096;
@@ -173,6 +176,33 @@ __device__ __forceinline__ void region_store(const uint8_t *r, const Lay &L, uin
         if (L.flags & PL_NAN) out.countv[g] = n - reinterpret_cast<const uint32_t *>(r + L.o_nan)[c];
     }
     if (L.flags & PL_CNTV) out.countv[g] = reinterpret_cast<const uint32_t *>(r + L.o_cntv)[c];
+}
+
+// Fold cell c of another region with layout L (global memory) into region r (LDS); each
+// cell is owned by one thread, so no atomics.
+template <int KEYB>
+__device__ __forceinline__ void region_merge(uint8_t *r, const Lay &L, uint32_t c,
+                                             const uint8_t *o) {
+    using K = KeyOf<KEYB>;
+    using T = typename K::T;
+    if (L.flags & PL_SUM)
+        reinterpret_cast<double *>(r + L.o_sum)[c] += reinterpret_cast<const double *>(o + L.o_sum)[c];
+    if (L.flags & PL_MIN) {
+        T &m = reinterpret_cast<T *>(r + L.o_min)[c];
+        const T x = reinterpret_cast<const T *>(o + L.o_min)[c];
+        m = x < m ? x : m;
+    }
+    if (L.flags & PL_MAX) {
+        T &m = reinterpret_cast<T *>(r + L.o_max)[c];
+        const T x = reinterpret_cast<const T *>(o + L.o_max)[c];
+        m = x > m ? x : m;
+    }
+    if (L.flags & PL_CNT)
+        reinterpret_cast<uint32_t *>(r + L.o_cnt)[c] += reinterpret_cast<const uint32_t *>(o + L.o_cnt)[c];
+    if (L.flags & PL_CNTV)
+        reinterpret_cast<uint32_t *>(r + L.o_cntv)[c] += reinterpret_cast<const uint32_t *>(o + L.o_cntv)[c];
+    if (L.flags & PL_NAN)
+        reinterpret_cast<uint32_t *>(r + L.o_nan)[c] += reinterpret_cast<const uint32_t *>(o + L.o_nan)[c];
 }
 
 // ---- window -------------------------------------------------------------------------------
@@ -429,11 +459,12 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
             const bool spill = keep && !inw;
             const unsigned long long m = __ballot(spill);
             if (spill) {
-                const uint32_t cell = (uint32_t)(((int64_t)py * a.W + px) * ncat + pc);
+                // cells < 2^32 (checked on the host), so 32-bit arithmetic is exact
+                const uint32_t cell =
+                    ((uint32_t)py * (uint32_t)a.W + (uint32_t)px) * (uint32_t)ncat + (uint32_t)pc;
                 const uint32_t slot = staged + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                 st_cell[slot] = cell;
                 if constexpr (HAS_VAL) st_val[slot] = cur.v[it];
-                atomicAdd(&thist[cell / s.tile_cells], 1u);
             }
             staged += (uint32_t)__popcll(m);
             // staged < kFlush before the item, so staged < kFlush + 64 <= kStage here;
@@ -448,6 +479,8 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
                     if (slot < (size_t)s.seg_cap) {
                         s.spill_cell[seg0 + slot] = c;
                         if constexpr (HAS_VAL) spill_val[seg0 + slot] = vv;
+                        // counted here, with every lane busy, rather than per binned row
+                        atomicAdd(&thist[c / s.tile_cells], 1u);
                     }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -474,8 +507,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
     if ((uint32_t)lane < staged) {  // the last partial line
         const size_t slot = (size_t)flushed + lane;
         if (slot < (size_t)s.seg_cap) {
-            s.spill_cell[seg0 + slot] = st_cell[lane];
+            const uint32_t c = st_cell[lane];
+            s.spill_cell[seg0 + slot] = c;
             if constexpr (HAS_VAL) spill_val[seg0 + slot] = st_val[lane];
+            atomicAdd(&thist[c / s.tile_cells], 1u);
         }
     }
     __syncthreads();
@@ -577,6 +612,12 @@ struct PartArgs {
     unsigned long long *tile_start;   // [ntiles + 1]
     uint32_t *sorted_cell;
     void *sorted_val;
+    // tile work split: tile t is processed by parts [part_start[t], part_start[t + 1]),
+    // each of at most *part_len entries; done[t] counts finished parts of a split tile
+    uint32_t *part_start;             // [ntiles + 1]
+    uint32_t *part_len;               // [1]
+    uint32_t *done;                   // [ntiles]
+    uint8_t *scratch;                 // one tile-layout region per part of a split tile
 };
 
 // Block-wide exclusive scan of one value per thread (1024 threads).
@@ -634,6 +675,27 @@ __global__ __launch_bounds__(kPartThreads) void spill_tilescan_kernel(PartArgs p
         carry += tot;
     }
     if (threadIdx.x == 0) p.tile_start[p.ntiles] = carry;
+    // parts of at most E = max(kMinPart, 2 * mean) entries: at most ntiles + ntiles / 2
+    // parts in all, and a tile far heavier than the mean (the edge of a dense cluster
+    // just outside the window) is spread over several workgroups
+    const unsigned long long mean2 = 2 * ((carry + p.ntiles - 1) / p.ntiles);
+    const unsigned long long E = mean2 > kMinPart ? mean2 : (unsigned long long)kMinPart;
+    if (threadIdx.x == 0) *p.part_len = (uint32_t)(E < 0xFFFFFFFFull ? E : 0xFFFFFFFFull);
+    unsigned long long pcarry = 0;
+    for (int t0 = 0; t0 < p.ntiles; t0 += blockDim.x) {
+        const int t = t0 + threadIdx.x;
+        unsigned long long np = 0;
+        if (t < p.ntiles) {
+            np = (totals[t] + E - 1) / E;
+            np = np < 1 ? 1 : np;
+            p.done[t] = 0u;
+        }
+        unsigned long long tot;
+        const unsigned long long ex = block_exscan(np, sh, tot);
+        if (t < p.ntiles) p.part_start[t] = (uint32_t)(pcarry + ex);
+        pcarry += tot;
+    }
+    if (threadIdx.x == 0) p.part_start[p.ntiles] = (uint32_t)pcarry;
 }
 
 // Scatter: partition group g walks the kWaves sub-segments of scan workgroup g as one
@@ -788,12 +850,31 @@ __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay 
                                                                int32_t ncat, int64_t ncells,
                                                                const double *mean, Planes out) {
     extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ int32_t s_tile, s_last;
     using R = TileRows<TV, HAS_VAL>;
     constexpr unsigned long long kStep = (unsigned long long)R::kU * kTileThreads;
-    const int t = blockIdx.x;
+    // this workgroup's part: the last tile whose first part is at or before it
+    const uint32_t part = blockIdx.x;
+    if (part >= p.part_start[p.ntiles]) return;  // the grid is an upper bound
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = p.ntiles - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (p.part_start[mid] <= part) lo = mid;
+            else hi = mid - 1;
+        }
+        s_tile = lo;
+    }
     region_init<KEYB>(smem, L, threadIdx.x, kTileThreads);
     __syncthreads();
-    const unsigned long long b = p.tile_start[t], e = p.tile_start[t + 1];
+    const int t = s_tile;
+    const uint32_t first = p.part_start[t], nparts = p.part_start[t + 1] - first;
+    const unsigned long long len = *p.part_len;
+    const unsigned long long tb = p.tile_start[t], te = p.tile_start[t + 1];
+    unsigned long long b = tb + (unsigned long long)(part - first) * len;
+    unsigned long long e = b + len;
+    b = b < te ? b : te;
+    e = e < te ? e : te;
     const int64_t c0 = (int64_t)t * p.tile_cells;
     if (b < e) {
         // three register sets rotate roles, as in the scan
@@ -831,6 +912,26 @@ __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay 
             step(rc, rb);
         }
     }
+    if (nparts > 1) {
+        // a split tile: every part leaves its planes in its scratch region; the part that
+        // finishes last adds the others' regions to its own and writes the tile
+        __syncthreads();
+        uint4 *mine = reinterpret_cast<uint4 *>(p.scratch + (size_t)part * L.bytes);
+        const uint4 *src = reinterpret_cast<const uint4 *>(smem);
+        for (uint32_t j = threadIdx.x; j < L.bytes / 16u; j += kTileThreads) mine[j] = src[j];
+        __threadfence();  // release: the region is visible before the count below
+        __syncthreads();
+        if (threadIdx.x == 0) s_last = atomicAdd(p.done + t, 1u) == nparts - 1;
+        __syncthreads();
+        if (!s_last) return;
+        __threadfence();  // acquire: the other parts' regions
+        for (uint32_t q = 0; q < nparts; ++q) {
+            if (first + q == part) continue;
+            const uint8_t *other = p.scratch + (size_t)(first + q) * L.bytes;
+            for (uint32_t c = threadIdx.x; c < L.cap; c += kTileThreads)
+                region_merge<KEYB>(smem, L, c, other);
+        }
+    }
     __syncthreads();
     const Window w = load_window(window, W, H, wcap_px);
     int64_t c1 = c0 + p.tile_cells;
@@ -857,7 +958,9 @@ struct Plan {
     uint32_t o_thist, o_stage_cell, o_stage_val;  // scan LDS offsets
     // workspace offsets
     size_t o_window, o_hist_sel, o_count, o_partials, o_spill_cell, o_spill_val, o_hist,
-        o_totals, o_tile_start, o_sorted_cell, o_sorted_val, total;
+        o_totals, o_tile_start, o_sorted_cell, o_sorted_val, o_part_start, o_part_len, o_done,
+        o_scratch, total;
+    int32_t max_parts;   // upper bound of the tile parts (grid of the tile kernel)
     bool whole;          // the window is the whole canvas
     int bshift, cw, ch;
 };
@@ -934,6 +1037,11 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     P.o_tile_start = o; o = align_up(o + 8 * (size_t)(P.ntiles + 1), 256);
     P.o_sorted_cell = o; o = align_up(o + 4 * (size_t)n, 256);
     P.o_sorted_val = o; o = align_up(o + (size_t)P.vbytes * n, 256);
+    P.max_parts = P.ntiles + P.ntiles / 2 + 1;
+    P.o_part_start = o; o = align_up(o + 4 * (size_t)(P.ntiles + 1), 256);
+    P.o_part_len = o; o = align_up(o + 4, 256);
+    P.o_done = o; o = align_up(o + 4 * (size_t)P.ntiles, 256);
+    P.o_scratch = o; o = align_up(o + (size_t)P.max_parts * P.tl.bytes, 256);
     P.total = o;
     return DSX_OK;
 }
@@ -1026,6 +1134,10 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     pa.tile_start = reinterpret_cast<unsigned long long *>(ws + P.o_tile_start);
     pa.sorted_cell = reinterpret_cast<uint32_t *>(ws + P.o_sorted_cell);
     pa.sorted_val = ws + P.o_sorted_val;
+    pa.part_start = reinterpret_cast<uint32_t *>(ws + P.o_part_start);
+    pa.part_len = reinterpret_cast<uint32_t *>(ws + P.o_part_len);
+    pa.done = reinterpret_cast<uint32_t *>(ws + P.o_done);
+    pa.scratch = ws + P.o_scratch;
     unsigned long long *totals = reinterpret_cast<unsigned long long *>(ws + P.o_totals);
     hipLaunchKernelGGL(spill_colscan_kernel, dim3(P.ntiles), dim3(kPartThreads), 0, st, pa, totals);
     hipLaunchKernelGGL(spill_tilescan_kernel, dim3(1), dim3(kPartThreads), 0, st, pa,
@@ -1037,7 +1149,7 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     e = hipFuncSetAttribute((const void *)tile, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)P.tile_lds);
     if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
-    hipLaunchKernelGGL(tile, dim3(P.ntiles), dim3(kTileThreads), P.tile_lds, st, pa, P.tl,
+    hipLaunchKernelGGL(tile, dim3(P.max_parts), dim3(kTileThreads), P.tile_lds, st, pa, P.tl,
                        (const int32_t *)window, a.W, a.H, wcap_px, ncat, P.ncells, mean, out);
     return launch_status();
 }

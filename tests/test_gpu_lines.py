@@ -183,3 +183,38 @@ def test_random_walks_2048x1024_vs_oracle():
     view = G.view_for(2048, 1024, xr_, yr_, yf32=True)
     exp = oracle.lines_xconst(xs, ys.astype(np.float64), view, 2048, 1024)["count"]
     np.testing.assert_array_equal(agg.values, exp)
+
+
+@pytest.mark.parametrize("case", ["linspace", "shuffled_x", "zoomed", "tall", "nan_x", "few_lines"])
+def test_xconst_strips_vs_oracle_and_general_path(case):
+    """x-constant lines take the LDS-strip kernel; pixels outside a strip's band (x not
+    monotone, zoomed-in views, wraparound) take its global fallback.  The same lines
+    given with one x row per line take the per-segment kernel.  Both match the oracle."""
+    rng = np.random.default_rng(11)
+    nlines, nverts, W, H = 2000, 40, 256, 1000
+    xs = np.linspace(0, 1, nverts)
+    xr_, yr_ = (0.0, 1.0), (-1.0, 1.0)
+    if case == "shuffled_x":
+        rng.shuffle(xs)
+    elif case == "zoomed":
+        xr_ = (0.3, 0.4)
+    elif case == "tall":
+        H = 3000
+    elif case == "nan_x":
+        xs[[0, 17, 18]] = np.nan
+    elif case == "few_lines":
+        nlines = 3
+    ys = rng.uniform(-1.2, 1.2, (nlines, 1)) + np.cumsum(rng.normal(0, 0.1, (nlines, nverts)),
+                                                         axis=1)
+    ys[rng.random(ys.shape) < 0.01] = np.nan
+    view = G.view_for(W, H, xr_, yr_)
+    exp = oracle.lines_xconst(xs, ys, view, W, H)["count"]
+    assert exp.sum() > 0
+    # one shared x row, y as a [nverts, nlines] block (column j = vertex j of every line)
+    strip = abi_lines(xs, ys.T, view, W, H, axis=1, x_ls=0, x_vs=1, y_ls=1, y_vs=nlines,
+                      nlines=nlines, nverts=nverts)
+    np.testing.assert_array_equal(strip, exp)
+    # the same lines with an x row per line
+    general = abi_lines(np.tile(xs, (nlines, 1)), ys, view, W, H, axis=1, x_ls=nverts, x_vs=1,
+                        y_ls=nverts, y_vs=1, nlines=nlines, nverts=nverts)
+    np.testing.assert_array_equal(general, exp)
