@@ -229,13 +229,82 @@ def _traffic(workload):
     return None if entry is None else entry.get("bytes_per_call")
 
 
+def run_serving(args, torch):
+    """Batched / served viewports (count + mean) over a resident point set against the
+    plain per-request form; prints one JSON line."""
+    import numpy as np
+
+    import datashader_amd as dsa
+    from datashader_amd import serving, synthetic
+
+    n = int(1_000_000_000 * args.scale)
+    _log(f"[bench] serving: generating {n} resident rows")
+    x, y, v = synthetic.points_mixture_dev(n, seed=5, nan_frac=0.0, dtype=torch.float32,
+                                           with_value=True)
+    frame = dsa.DeviceFrame({"x": x, "y": y, "v": v})
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    index = serving.PointIndex(x, y, v, grid=1024)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    _log(f"[bench] index built in {build_s:.2f} s")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(6)
+    pick = torch.randint(0, n, (100_000,), generator=g, device="cuda")
+    centres = (x[pick].cpu().numpy(), y[pick].cpu().numpy())
+    extent = ((index.x0, index.x1), (index.y0, index.y1))
+    reqs = serving.make_requests(args.requests, centres, extent, seed=5, rate=200.0)
+    # warm both forms (library load, first-call selection kernels)
+    for r in reqs[:3]:
+        serving.run_plain(frame, r)
+        serving.run_indexed(index, r)
+    torch.cuda.synchronize()
+    # plain: one Canvas.points call over the whole set per request, back to back
+    k = min(args.plain_requests, len(reqs))
+    plat = []
+    for r in reqs[:k]:
+        t1 = time.perf_counter()
+        serving.run_plain(frame, r)
+        torch.cuda.synchronize()
+        plat.append(time.perf_counter() - t1)
+    plat = np.array(plat)
+    plain_rps = k / plat.sum()
+    _log(f"[bench] plain: {plain_rps:.1f} req/s over {k} requests")
+    server = serving.ViewportServer(index, slots=4)
+    open_ = server.serve(reqs, realtime=True)
+    _log(f"[bench] served (Poisson 200 req/s): {len(reqs) / open_['wall']:.1f} req/s")
+    sat = server.serve(reqs, realtime=False)
+    sat_rps = len(reqs) / sat["wall"]
+    _log(f"[bench] served (saturated): {sat_rps:.1f} req/s")
+
+    def pct(a):
+        return {"p50_ms": float(np.percentile(a, 50) * 1e3), "p99_ms": float(np.percentile(a, 99) * 1e3)}
+
+    out = {
+        "metric": "viewport req/s (count + mean), served vs plain per-request form",
+        "value": sat_rps, "unit": "req/s", "n_gpus": 1, "steps": len(reqs), "warmup": 3,
+        "ms_per_step": 1e3 / sat_rps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded, generated on the device)",
+        "config": {"workload": "serving: 1B resident float32 rows (5-Gaussian mixture), value "
+                               "U(0,1); Poisson 200 req/s, zoom 2^U(0,12), centres at random "
+                               "rows, canvases 256^2/512^2/1024x768/1920x1080",
+                   "rows": n, "requests": len(reqs), "grid": index.grid, "slots": 4,
+                   "index_build_s": build_s},
+        "plain": {"req_per_s": plain_rps, "requests": k, **pct(plat)},
+        "served_poisson_200": {"req_per_s": len(reqs) / open_["wall"], **pct(open_["latency"])},
+        "served_saturated": {"req_per_s": sat_rps, **pct(sat["latency"])},
+        "speedup_vs_plain": sat_rps / plain_rps,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="points_skewed",
-                    choices=["points_skewed", "points800", "bycat", "lines"])
+                    choices=["points_skewed", "points800", "bycat", "lines", "serving"])
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload size")
     ap.add_argument("--cpu-sample", type=int, default=20_000_000,
                     help="rows (or line samples) of the CPU oracle baseline sample")
@@ -243,6 +312,9 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one plain 1 GiB device copy (the reference dispatch that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
+    ap.add_argument("--requests", type=int, default=2000, help="serving: viewport requests")
+    ap.add_argument("--plain-requests", type=int, default=200,
+                    help="serving: requests timed in the plain per-request form")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -255,6 +327,9 @@ def main():
     from datashader_amd import backend
 
     torch.cuda.set_device(0)
+    if args.workload == "serving":
+        run_serving(args, torch)
+        return
     _log(f"[bench] building workload {args.workload} (scale {args.scale})")
     step, rows, algo, cfg, cpu, check = build_workload(args.workload, torch, args.scale)
     torch.cuda.synchronize()

@@ -12,7 +12,7 @@ import os
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libdsx.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # dtype codes
 F32, F64, I8, U8, I16, I32, I64 = 1, 2, 3, 4, 5, 6, 7
@@ -23,6 +23,8 @@ FIN_ANY, FIN_SUM, FIN_MEAN, FIN_VAR, FIN_STD, FIN_MIN, FIN_MAX, FIN_SUMZERO = ra
 
 EXPORTS = ("dsx_points_workspace_size", "dsx_points_agg", "dsx_lines_agg",
            "dsx_bounds_workspace_size", "dsx_bounds", "dsx_fill", "dsx_finalize",
+           "dsx_grid_index_workspace_size", "dsx_grid_index_build", "dsx_gather_rows",
+           "dsx_points_indexed_workspace_size", "dsx_points_agg_indexed",
            "dsx_abi_version", "dsx_strerror", "dsx_last_hip_error")
 
 vp = ctypes.c_void_p
@@ -74,6 +76,25 @@ class FinalizeDesc(ctypes.Structure):
                 ("key", vp), ("out", vp)]
 
 
+class GridIndexDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("grid", ctypes.c_int32), ("dtype", ctypes.c_int32),
+                ("x", vp), ("y", vp), ("val", vp),
+                ("x0", ctypes.c_double), ("x1", ctypes.c_double),
+                ("y0", ctypes.c_double), ("y1", ctypes.c_double),
+                ("out_x", vp), ("out_y", vp), ("out_val", vp), ("cell_start", vp)]
+
+
+class GridRef(ctypes.Structure):
+    _fields_ = [("grid", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("x0", ctypes.c_double), ("x1", ctypes.c_double),
+                ("y0", ctypes.c_double), ("y1", ctypes.c_double), ("cell_start", vp)]
+
+
+class GatherDesc(ctypes.Structure):
+    _fields_ = [("ncols", ctypes.c_int32), ("elem_bytes", ctypes.c_int32 * 4),
+                ("src", vp * 4), ("dst", vp * 4), ("items", vp), ("nitems", ctypes.c_int64)]
+
+
 class DsxError(RuntimeError):
     """A libdsx call returned a non-zero status."""
 
@@ -101,13 +122,23 @@ def load():
         lib.dsx_bounds.argtypes = [vp, i32, i64, vp, vp, sz, vp]
         lib.dsx_fill.argtypes = [vp, i32, i64, vp]
         lib.dsx_finalize.argtypes = [P(FinalizeDesc), vp]
+        lib.dsx_grid_index_workspace_size.argtypes = [P(GridIndexDesc)]
+        lib.dsx_grid_index_workspace_size.restype = sz
+        lib.dsx_grid_index_build.argtypes = [P(GridIndexDesc), vp, sz, vp]
+        lib.dsx_gather_rows.argtypes = [P(GatherDesc), vp]
+        lib.dsx_points_indexed_workspace_size.argtypes = [P(PointsDesc), P(View), P(Planes),
+                                                          P(GridRef)]
+        lib.dsx_points_indexed_workspace_size.restype = sz
+        lib.dsx_points_agg_indexed.argtypes = [P(PointsDesc), P(View), P(Planes), vp,
+                                               P(GridRef), vp, sz, vp]
         lib.dsx_abi_version.argtypes = []
         lib.dsx_strerror.argtypes = [ctypes.c_int]
         lib.dsx_strerror.restype = ctypes.c_char_p
         lib.dsx_last_hip_error.argtypes = []
         lib.dsx_last_hip_error.restype = ctypes.c_char_p
-        for f in ("dsx_points_agg", "dsx_lines_agg", "dsx_bounds",
-                  "dsx_fill", "dsx_finalize", "dsx_abi_version"):
+        for f in ("dsx_points_agg", "dsx_lines_agg", "dsx_bounds", "dsx_fill", "dsx_finalize",
+                  "dsx_grid_index_build", "dsx_gather_rows", "dsx_points_agg_indexed",
+                  "dsx_abi_version"):
             getattr(lib, f).restype = ctypes.c_int
         if lib.dsx_abi_version() != ABI_VERSION:
             raise RuntimeError("libdsx.so ABI version mismatch; rebuild the library")

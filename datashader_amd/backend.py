@@ -18,6 +18,8 @@ columns running as a libdsx kernel:
 """
 from __future__ import annotations
 
+from collections import OrderedDict
+
 import numpy as np
 import pandas as pd
 
@@ -134,22 +136,54 @@ class PassResult:
         self.planes = planes
 
 
+_WINDOW_CACHE_MAX = 64
+
+
 def _window_slot(frame, key, device):
     """(device int32[4] window, reuse flag) of the hot-window cache kept on a
     DeviceFrame.  The window only steers performance (which canvas cells are kept in
     LDS during the scan); any stored window gives the same result."""
     t = dev.torch()
-    cache = frame.__dict__.setdefault("_window_cache", {})
+    cache = frame.__dict__.setdefault("_window_cache", OrderedDict())
     slot = cache.get(key)
     if slot is None:
+        while len(cache) >= _WINDOW_CACHE_MAX:  # oldest first: a server sees many views
+            cache.popitem(last=False)
         slot = cache[key] = t.zeros(4, dtype=t.int32, device=device)
         return slot, 0
+    cache.move_to_end(key)
     return slot, 1
 
 
-def _points_call(lib, d, vw, planes_struct, mean, device, stream):
-    """dsx_points_agg with a workspace from torch's caching allocator."""
+def _grid_ref(frame, glyph, p, view):
+    """The lattice of a frame sorted by a grid index (``serving.PointIndex``) when this
+    pass can read it: the glyph's x / y are the indexed columns, no categorizer, linear
+    axes.  None otherwise."""
+    g = getattr(frame, "grid_index", None)
+    if g is None or p.cat is not None or view["xlog"] or view["ylog"]:
+        return None
+    if (glyph.x, glyph.y) != (g["x"], g["y"]):
+        return None
+    r = _abi.GridRef()
+    r.grid = g["grid"]
+    r.x0, r.x1, r.y0, r.y1 = g["extent"]
+    r.cell_start = dev.ptr(g["cell_start"])
+    return r
+
+
+def _points_call(lib, d, vw, planes_struct, mean, device, stream, grid=None):
+    """dsx_points_agg (or, over an indexed frame, dsx_points_agg_indexed) with a
+    workspace from torch's caching allocator."""
     t = dev.torch()
+    if grid is not None:
+        ws = int(lib.dsx_points_indexed_workspace_size(d, vw, planes_struct, grid))
+        if ws == 0:
+            raise _abi.DsxError("dsx_points_indexed_workspace_size rejected the call")
+        work = t.empty(ws, dtype=t.uint8, device=device)
+        _abi.check(_timed(lambda: lib.dsx_points_agg_indexed(
+            d, vw, planes_struct, dev.ptr(mean), grid, dev.ptr(work), ws, stream)),
+            "dsx_points_agg_indexed")
+        return
     ws = int(lib.dsx_points_workspace_size(d, vw, planes_struct))
     if ws == 0:
         raise _abi.DsxError("dsx_points_workspace_size rejected the call (canvas too large "
@@ -200,20 +234,21 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W):
         d.xy_dtype = dev.dtype_code(x)
         d.val, d.val_dtype = dev.ptr(val), vdt
         d.cat = catc
-        if getattr(frame, "persistent", False):
+        grid = _grid_ref(frame, glyph, p, view)
+        if grid is None and getattr(frame, "persistent", False):
             key = (glyph.x, glyph.y, p.val, None if p.cat is None else p.cat._key(),
                    tuple(sorted(p.planes)),
                    W, H, tuple(view[k] for k in ("xmin", "xmax", "ymin", "ymax")))
             win, reuse = _window_slot(frame, key, device)
             d.window, d.reuse_window = dev.ptr(win), reuse
-        _points_call(lib, d, vw, ps, None, device, stream)
+        _points_call(lib, d, vw, ps, None, device, stream, grid)
         if p.m2:
             mean = t.empty(ncells, dtype=t.float64, device=device)
             _finalize(lib, _abi.FIN_MEAN, keyb, ncells, planes, mean)
             planes["m2"] = _alloc(ncells, "f64", device)
             m2s = _abi.Planes()
             m2s.sum = dev.ptr(planes["m2"])
-            _points_call(lib, d, vw, m2s, mean, device, stream)
+            _points_call(lib, d, vw, m2s, mean, device, stream, grid)
     else:
         if p.m2:
             raise NotImplementedError("var / std of lines are not supported")

@@ -615,6 +615,8 @@ struct PartArgs {
     // tile work split: tile t is processed by parts [part_start[t], part_start[t + 1]),
     // each of at most *part_len entries; done[t] counts finished parts of a split tile
     uint32_t *part_start;             // [ntiles + 1]
+    uint32_t *part_tile;              // [max_parts] tile of each part
+    int32_t max_parts;
     uint32_t *part_len;               // [1]
     uint32_t *done;                   // [ntiles]
     uint8_t *scratch;                 // one tile-layout region per part of a split tile
@@ -692,7 +694,12 @@ __global__ __launch_bounds__(kPartThreads) void spill_tilescan_kernel(PartArgs p
         }
         unsigned long long tot;
         const unsigned long long ex = block_exscan(np, sh, tot);
-        if (t < p.ntiles) p.part_start[t] = (uint32_t)(pcarry + ex);
+        if (t < p.ntiles) {
+            p.part_start[t] = (uint32_t)(pcarry + ex);
+            for (unsigned long long q = 0; q < np; ++q)
+                if (pcarry + ex + q < (unsigned long long)p.max_parts)  // always (bound above)
+                    p.part_tile[pcarry + ex + q] = (uint32_t)t;
+        }
         pcarry += tot;
     }
     if (threadIdx.x == 0) p.part_start[p.ntiles] = (uint32_t)pcarry;
@@ -717,8 +724,10 @@ __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p)
     __shared__ unsigned long long sh[kPartThreads / 64 + 1];
     __shared__ unsigned long long pre[kWaves + 1];  // sub-segment prefix lengths
     const int g = blockIdx.x;
+    // cur[t]: where this group's next entry of tile t goes in the sorted spill (spill
+    // positions fit 32 bits: the host rejects inputs of 2^32 rows or more)
     for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x) {
-        cur[t] = p.hist[(size_t)g * p.ntiles + t];
+        cur[t] = (uint32_t)(p.tile_start[t] + p.hist[(size_t)g * p.ntiles + t]);
         cnt[t] = 0u;
     }
     if (threadIdx.x == 0) {
@@ -804,7 +813,7 @@ __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p)
         const uint32_t nsub = (uint32_t)((total - c0) < (unsigned long long)kSub ? (total - c0) : kSub);
         for (uint32_t j = threadIdx.x; j < nsub; j += blockDim.x) {
             const uint32_t t = s_tile[j];
-            const unsigned long long dst = p.tile_start[t] + cur[t] + (j - cnt[t]);
+            const uint32_t dst = cur[t] + (j - cnt[t]);
             p.sorted_cell[dst] = s_cell[j];
             if constexpr (HAS_VAL) reinterpret_cast<TV *>(p.sorted_val)[dst] = s_val[j];
         }
@@ -850,24 +859,15 @@ __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay 
                                                                int32_t ncat, int64_t ncells,
                                                                const double *mean, Planes out) {
     extern __shared__ __align__(16) uint8_t smem[];
-    __shared__ int32_t s_tile, s_last;
+    __shared__ int32_t s_last;
     using R = TileRows<TV, HAS_VAL>;
     constexpr unsigned long long kStep = (unsigned long long)R::kU * kTileThreads;
     // this workgroup's part: the last tile whose first part is at or before it
     const uint32_t part = blockIdx.x;
-    if (part >= p.part_start[p.ntiles]) return;  // the grid is an upper bound
-    if (threadIdx.x == 0) {
-        int lo = 0, hi = p.ntiles - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (p.part_start[mid] <= part) lo = mid;
-            else hi = mid - 1;
-        }
-        s_tile = lo;
-    }
+    if (part >= p.part_start[p.ntiles] || part >= (uint32_t)p.max_parts) return;  // the grid is an upper bound
+    const int t = (int)p.part_tile[part];
     region_init<KEYB>(smem, L, threadIdx.x, kTileThreads);
     __syncthreads();
-    const int t = s_tile;
     const uint32_t first = p.part_start[t], nparts = p.part_start[t + 1] - first;
     const unsigned long long len = *p.part_len;
     const unsigned long long tb = p.tile_start[t], te = p.tile_start[t + 1];
@@ -958,7 +958,7 @@ struct Plan {
     uint32_t o_thist, o_stage_cell, o_stage_val;  // scan LDS offsets
     // workspace offsets
     size_t o_window, o_hist_sel, o_count, o_partials, o_spill_cell, o_spill_val, o_hist,
-        o_totals, o_tile_start, o_sorted_cell, o_sorted_val, o_part_start, o_part_len, o_done,
+        o_totals, o_tile_start, o_sorted_cell, o_sorted_val, o_part_start, o_part_len, o_part_tile, o_done,
         o_scratch, total;
     int32_t max_parts;   // upper bound of the tile parts (grid of the tile kernel)
     bool whole;          // the window is the whole canvas
@@ -1040,6 +1040,7 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     P.max_parts = P.ntiles + P.ntiles / 2 + 1;
     P.o_part_start = o; o = align_up(o + 4 * (size_t)(P.ntiles + 1), 256);
     P.o_part_len = o; o = align_up(o + 4, 256);
+    P.o_part_tile = o; o = align_up(o + 4 * (size_t)P.max_parts, 256);
     P.o_done = o; o = align_up(o + 4 * (size_t)P.ntiles, 256);
     P.o_scratch = o; o = align_up(o + (size_t)P.max_parts * P.tl.bytes, 256);
     P.total = o;
@@ -1136,6 +1137,8 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     pa.sorted_val = ws + P.o_sorted_val;
     pa.part_start = reinterpret_cast<uint32_t *>(ws + P.o_part_start);
     pa.part_len = reinterpret_cast<uint32_t *>(ws + P.o_part_len);
+    pa.part_tile = reinterpret_cast<uint32_t *>(ws + P.o_part_tile);
+    pa.max_parts = P.max_parts;
     pa.done = reinterpret_cast<uint32_t *>(ws + P.o_done);
     pa.scratch = ws + P.o_scratch;
     unsigned long long *totals = reinterpret_cast<unsigned long long *>(ws + P.o_totals);
@@ -1164,6 +1167,314 @@ static int dispatch_val(const dsx_points_desc *d, const PointArgs &a, const Plan
                   : run_points<TC, float, true, 4, false>(d, a, P, out, mean, ws, st);
     return sq ? run_points<TC, double, true, 8, true>(d, a, P, out, mean, ws, st)
               : run_points<TC, double, true, 8, false>(d, a, P, out, mean, ws, st);
+}
+
+// ---- indexed aggregation (rows sorted by the cells of a lattice) ------------------------
+// The canvas is cut into tiles of tw x th pixels whose planes fit LDS next to a table of
+// the lattice rows under the tile.  Tile t reads the rows of the lattice cells its pixels
+// can receive (one extra cell on each side), keeps the rows whose pixel is in the tile
+// and writes every cell of the tile: each row lands in exactly one tile, and no row is
+// spilled.  A tile with many rows is split into parts of at most part_len rows; the part
+// that finishes last merges the others' regions, as in the spill tiles.
+constexpr int kIdxThreads = 1024;
+constexpr int kIdxMaxRows = 1024;       // lattice rows a tile may span (grid <= 1024)
+constexpr int kIdxTable = kIdxMaxRows * 16 + 16;
+constexpr unsigned long long kIdxMinPart = 1ull << 16;
+
+struct IdxArgs {
+    PointArgs a;                     // the sorted columns and the view
+    Lay L;                           // tile planes: cap = tw * th cells
+    int32_t tw, th, ntx, nty, ntiles;
+    int32_t grid;
+    double ex0, esx, ey0, esy;       // lattice: cell = (v - e0) * es
+    const int64_t *cs;               // cell_start
+    unsigned long long *tile_rows;   // [ntiles]
+    uint32_t *part_start;            // [ntiles + 1]
+    uint32_t *part_tile;             // [max_parts]
+    uint32_t *done;                  // [ntiles]
+    unsigned long long *part_len;    // [1]
+    int32_t max_parts, cus;
+    uint8_t *scratch;
+    const double *mean;
+    Planes out;
+};
+
+__device__ __forceinline__ int idx_cell(double v, double e0, double es, int grid) {
+    const double top = (double)(grid - 1);
+    double f = (v - e0) * es;
+    f = f < 0.0 ? 0.0 : (f > top ? top : f);
+    return (int)f;
+}
+
+// Pixels and lattice cells of tile t.
+__device__ __forceinline__ void idx_rect(const IdxArgs &A, int t, int &px0, int &px1, int &py0,
+                                         int &py1, int &gx0, int &gx1, int &gy0, int &gy1) {
+    const PointArgs &a = A.a;
+    const int tx = t % A.ntx, ty = t / A.ntx;
+    px0 = tx * A.tw;
+    px1 = px0 + A.tw < a.W ? px0 + A.tw : a.W;
+    py0 = ty * A.th;
+    py1 = py0 + A.th < a.H ? py0 + A.th : a.H;
+    // the coordinates that project into the pixels (sx, sy > 0); the last column / row
+    // also receives the rows clamped from W / H, i.e. up to the view's upper bound
+    double xl = ((double)px0 - a.tx) / a.sx, xh = ((double)px1 - a.tx) / a.sx;
+    double yl = ((double)py0 - a.ty) / a.sy, yh = ((double)py1 - a.ty) / a.sy;
+    xl = xl > a.xmin ? xl : a.xmin;
+    yl = yl > a.ymin ? yl : a.ymin;
+    xh = (px1 == a.W || xh > a.xmax) ? a.xmax : xh;
+    yh = (py1 == a.H || yh > a.ymax) ? a.ymax : yh;
+    gx0 = idx_cell(xl, A.ex0, A.esx, A.grid) - 1;
+    gx1 = idx_cell(xh, A.ex0, A.esx, A.grid) + 1;
+    gy0 = idx_cell(yl, A.ey0, A.esy, A.grid) - 1;
+    gy1 = idx_cell(yh, A.ey0, A.esy, A.grid) + 1;
+    gx0 = gx0 < 0 ? 0 : gx0;
+    gy0 = gy0 < 0 ? 0 : gy0;
+    gx1 = gx1 >= A.grid ? A.grid - 1 : gx1;
+    gy1 = gy1 >= A.grid ? A.grid - 1 : gy1;
+}
+
+// Rows under each tile (one workgroup per tile).
+__global__ __launch_bounds__(256) void idx_rows_kernel(IdxArgs A) {
+    __shared__ unsigned long long red[256 / 64];
+    const int t = blockIdx.x;
+    int px0, px1, py0, py1, gx0, gx1, gy0, gy1;
+    idx_rect(A, t, px0, px1, py0, py1, gx0, gx1, gy0, gy1);
+    unsigned long long sum = 0;
+    for (int gy = gy0 + threadIdx.x; gy <= gy1; gy += blockDim.x) {
+        const int64_t r = (int64_t)gy * A.grid;
+        sum += (unsigned long long)(A.cs[r + gx1 + 1] - A.cs[r + gx0]);
+    }
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s2 = 0;
+        for (int k = 0; k < 256 / 64; ++k) s2 += red[k];
+        A.tile_rows[t] = s2;
+    }
+}
+
+// Parts of at most E rows, E = max(kIdxMinPart, total / (2 * CUs)): at most
+// ntiles + 2 * CUs parts.
+__global__ __launch_bounds__(kPartThreads) void idx_plan_kernel(IdxArgs A) {
+    __shared__ unsigned long long sh[kPartThreads / 64 + 1];
+    unsigned long long total = 0;
+    for (int t0 = 0; t0 < A.ntiles; t0 += blockDim.x) {
+        const int t = t0 + threadIdx.x;
+        const unsigned long long v = t < A.ntiles ? A.tile_rows[t] : 0ull;
+        unsigned long long tot;
+        block_exscan(v, sh, tot);
+        total += tot;
+    }
+    const unsigned long long share = (total + 2ull * A.cus - 1) / (2ull * A.cus);
+    const unsigned long long E = share > kIdxMinPart ? share : kIdxMinPart;
+    if (threadIdx.x == 0) *A.part_len = E;
+    unsigned long long pcarry = 0;
+    for (int t0 = 0; t0 < A.ntiles; t0 += blockDim.x) {
+        const int t = t0 + threadIdx.x;
+        unsigned long long np = 0;
+        if (t < A.ntiles) {
+            np = (A.tile_rows[t] + E - 1) / E;
+            np = np < 1 ? 1 : np;
+            A.done[t] = 0u;
+        }
+        unsigned long long tot;
+        const unsigned long long ex = block_exscan(np, sh, tot);
+        if (t < A.ntiles) {
+            A.part_start[t] = (uint32_t)(pcarry + ex);
+            for (unsigned long long q = 0; q < np; ++q)
+                if (pcarry + ex + q < (unsigned long long)A.max_parts)  // always (bound above)
+                    A.part_tile[pcarry + ex + q] = (uint32_t)t;
+        }
+        pcarry += tot;
+    }
+    if (threadIdx.x == 0) A.part_start[A.ntiles] = (uint32_t)pcarry;
+}
+
+template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
+__global__ __launch_bounds__(kIdxThreads) void idx_tile_kernel(IdxArgs A) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ unsigned long long sh[kIdxThreads / 64 + 1];
+    __shared__ int32_t s_last;
+    const PointArgs &a = A.a;
+    const Lay &L = A.L;
+    const uint32_t part = blockIdx.x;
+    if (part >= A.part_start[A.ntiles] || part >= (uint32_t)A.max_parts) return;
+    const int t = (int)A.part_tile[part];
+    const uint32_t first = A.part_start[t], nparts = A.part_start[t + 1] - first;
+    int px0, px1, py0, py1, gx0, gx1, gy0, gy1;
+    idx_rect(A, t, px0, px1, py0, py1, gx0, gx1, gy0, gy1);
+    const int nrow = gy1 - gy0 + 1;  // <= kIdxMaxRows (grid <= 1024)
+    // table of the lattice rows under the tile: first sorted row and row-count prefix
+    int64_t *s_start = reinterpret_cast<int64_t *>(smem + L.bytes);
+    unsigned long long *s_pre = reinterpret_cast<unsigned long long *>(smem + L.bytes) + kIdxMaxRows;
+    unsigned long long len = 0;
+    int64_t st0 = 0;
+    if ((int)threadIdx.x < nrow) {
+        const int64_t r = (int64_t)(gy0 + (int)threadIdx.x) * A.grid;
+        st0 = A.cs[r + gx0];
+        len = (unsigned long long)(A.cs[r + gx1 + 1] - st0);
+    }
+    unsigned long long tot;
+    const unsigned long long ex = block_exscan(len, sh, tot);
+    if ((int)threadIdx.x < nrow) {
+        s_start[threadIdx.x] = st0;
+        s_pre[threadIdx.x] = ex;
+    }
+    region_init<KEYB>(smem, L, threadIdx.x, kIdxThreads);
+    __syncthreads();
+    const unsigned long long E = *A.part_len;
+    unsigned long long lo = (unsigned long long)(part - first) * E, hi = lo + E;
+    lo = lo < tot ? lo : tot;
+    hi = hi < tot ? hi : tot;
+    const TC *X = reinterpret_cast<const TC *>(a.x);
+    const TC *Y = reinterpret_cast<const TC *>(a.y);
+    const TV *V = reinterpret_cast<const TV *>(a.val);
+    for (unsigned long long c0 = lo; c0 < hi; c0 += (unsigned long long)kChunk) {
+        TC xs[kItems], ys[kItems];
+        TV vs[kItems];
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            unsigned long long j = c0 + (unsigned long long)(it * kIdxThreads + threadIdx.x);
+            j = j < hi ? j : lo;
+            int k = 0;  // the last lattice row starting at or before j
+            for (int step = kIdxMaxRows / 2; step >= 1; step >>= 1)
+                if (k + step < nrow && s_pre[k + step] <= j) k += step;
+            const int64_t row = s_start[k] + (int64_t)(j - s_pre[k]);
+            xs[it] = X[row];
+            ys[it] = Y[row];
+            if constexpr (HAS_VAL) vs[it] = V[row];
+            else vs[it] = TV(0);
+        }
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            const unsigned long long j = c0 + (unsigned long long)(it * kIdxThreads + threadIdx.x);
+            int32_t px, py;
+            if (j >= hi || !locate_xy<TC>(a, xs[it], ys[it], px, py)) continue;
+            if (px < px0 || px >= px1 || py < py0 || py >= py1) continue;  // another tile's
+            const uint32_t lc = (uint32_t)((py - py0) * A.tw + (px - px0));
+            const double v = HAS_VAL ? (double)vs[it] : 0.0;
+            const bool vnan = HAS_VAL ? (v != v) : true;
+            double add = v;
+            if constexpr (SQDEV) {
+                const double d = v - A.mean[(int64_t)py * a.W + px];
+                add = d * d;
+            }
+            region_add<KEYB>(smem, L, lc, vnan, v, add);
+        }
+    }
+    if (nparts > 1) {
+        __syncthreads();
+        uint4 *mine = reinterpret_cast<uint4 *>(A.scratch + (size_t)part * L.bytes);
+        const uint4 *src = reinterpret_cast<const uint4 *>(smem);
+        for (uint32_t j = threadIdx.x; j < L.bytes / 16u; j += kIdxThreads) mine[j] = src[j];
+        __threadfence();  // release: the region is visible before the count below
+        __syncthreads();
+        if (threadIdx.x == 0) s_last = atomicAdd(A.done + t, 1u) == nparts - 1;
+        __syncthreads();
+        if (!s_last) return;
+        __threadfence();  // acquire: the other parts' regions
+        for (uint32_t q = 0; q < nparts; ++q) {
+            if (first + q == part) continue;
+            const uint8_t *other = A.scratch + (size_t)(first + q) * L.bytes;
+            for (uint32_t c = threadIdx.x; c < L.cap; c += kIdxThreads)
+                region_merge<KEYB>(smem, L, c, other);
+        }
+    }
+    __syncthreads();
+    const int w = px1 - px0, h = py1 - py0;
+    for (int c = threadIdx.x; c < w * h; c += kIdxThreads) {
+        const int ly = c / w, lx = c - ly * w;
+        region_store<KEYB>(smem, L, (uint32_t)(ly * A.tw + lx), A.out,
+                           (int64_t)(py0 + ly) * a.W + (px0 + lx));
+    }
+}
+
+struct IdxPlan {
+    Lay L;
+    int32_t tw, th, ntx, nty, ntiles, max_parts, cus;
+    size_t lds, o_rows, o_pstart, o_ptile, o_done, o_plen, o_scratch, total;
+};
+
+static int idx_make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_planes *pl,
+                         IdxPlan &P) {
+    if (plane_flags(pl) == 0) return DSX_ERR_ACCUM;
+    const uint32_t flags = internal_flags(plane_flags(pl));
+    const int keyb = (has_value(d) && d->val_dtype == DSX_F32) ? 4 : 8;
+    const uint32_t cb = cell_bytes(flags, keyb);
+    // LDS: planes + lattice-row table + the kernel's static words (under 512 bytes)
+    const int64_t cap = (int64_t)(kLdsBytes - kIdxTable - 512) / cb;
+    int th = (int)sqrt((double)cap);
+    th = th < 1 ? 1 : (th > v->height ? v->height : th);
+    int tw = (int)(cap / th);
+    tw = tw > v->width ? v->width : tw;
+    if (tw < 1) return DSX_ERR_SHAPE;
+    P.tw = tw; P.th = th;
+    P.ntx = (v->width + tw - 1) / tw;
+    P.nty = (v->height + th - 1) / th;
+    if ((int64_t)P.ntx * P.nty > (1 << 24)) return DSX_ERR_SHAPE;
+    P.ntiles = P.ntx * P.nty;
+    P.L = make_lay(flags, keyb, (uint32_t)(tw * th));
+    P.lds = P.L.bytes + kIdxTable;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0)
+            cus = c;
+    }
+    P.cus = cus;
+    P.max_parts = P.ntiles + 2 * cus + 1;
+    size_t o = 0;
+    P.o_rows = o; o = align_up(o + 8 * (size_t)P.ntiles, 256);
+    P.o_pstart = o; o = align_up(o + 4 * (size_t)(P.ntiles + 1), 256);
+    P.o_ptile = o; o = align_up(o + 4 * (size_t)P.max_parts, 256);
+    P.o_done = o; o = align_up(o + 4 * (size_t)P.ntiles, 256);
+    P.o_plen = o; o = align_up(o + 8, 256);
+    P.o_scratch = o; o = align_up(o + (size_t)P.max_parts * P.L.bytes, 256);
+    P.total = o;
+    return DSX_OK;
+}
+
+static int idx_check(const dsx_points_desc *d, const dsx_view *v, const dsx_planes *p,
+                     const double *mean, const dsx_grid_ref *g) {
+    int st = check_points(d, v);
+    if (st) return st;
+    if (!p || !g) return DSX_ERR_NULL;
+    if (d->cat.kind != DSX_CAT_NONE || v->xlog || v->ylog) return DSX_ERR_MODE;
+    if (!(v->sx > 0) || !(v->sy > 0)) return DSX_ERR_SHAPE;
+    if (g->grid < 1 || g->grid > kIdxMaxRows) return DSX_ERR_SHAPE;
+    if (!(g->x1 > g->x0) || !(g->y1 > g->y0)) return DSX_ERR_SHAPE;
+    if (!g->cell_start) return DSX_ERR_NULL;
+    const bool needs_val = p->countv || p->sum || p->min || p->max;
+    if (needs_val && !has_value(d)) return DSX_ERR_ACCUM;
+    if (mean && (!has_value(d) || !p->sum || p->count || p->countv || p->min || p->max))
+        return DSX_ERR_ACCUM;
+    return DSX_OK;
+}
+
+template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
+static int idx_run(IdxArgs &A, const IdxPlan &P, hipStream_t st) {
+    hipLaunchKernelGGL(idx_rows_kernel, dim3(P.ntiles), dim3(256), 0, st, A);
+    hipLaunchKernelGGL(idx_plan_kernel, dim3(1), dim3(kPartThreads), 0, st, A);
+    int s;
+    if ((s = launch_status())) return s;
+    auto kern = idx_tile_kernel<TC, TV, HAS_VAL, KEYB, SQDEV>;
+    const hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds);
+    if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
+    hipLaunchKernelGGL(kern, dim3(P.max_parts), dim3(kIdxThreads), P.lds, st, A);
+    return launch_status();
+}
+
+template <typename TC>
+static int idx_dispatch(const dsx_points_desc *d, IdxArgs &A, const IdxPlan &P, hipStream_t st) {
+    const bool sq = A.mean != nullptr;
+    if (!has_value(d)) return idx_run<TC, float, false, 8, false>(A, P, st);
+    if (d->val_dtype == DSX_F32)
+        return sq ? idx_run<TC, float, true, 4, true>(A, P, st)
+                  : idx_run<TC, float, true, 4, false>(A, P, st);
+    return sq ? idx_run<TC, double, true, 8, true>(A, P, st)
+              : idx_run<TC, double, true, 8, false>(A, P, st);
 }
 
 }  // namespace dsx
@@ -1198,4 +1509,48 @@ extern "C" int dsx_points_agg(const dsx_points_desc *d, const dsx_view *v, const
     uint8_t *ws = reinterpret_cast<uint8_t *>(workspace);
     if (d->xy_dtype == DSX_F32) return dispatch_val<float>(d, a, P, out, mean, ws, s);
     return dispatch_val<double>(d, a, P, out, mean, ws, s);
+}
+
+extern "C" size_t dsx_points_indexed_workspace_size(const dsx_points_desc *d, const dsx_view *v,
+                                                    const dsx_planes *p, const dsx_grid_ref *g) {
+    if (idx_check(d, v, p, nullptr, g)) return 0;
+    IdxPlan P;
+    if (idx_make_plan(d, v, p, P)) return 0;
+    return P.total;
+}
+
+extern "C" int dsx_points_agg_indexed(const dsx_points_desc *d, const dsx_view *v,
+                                      const dsx_planes *p, const double *mean,
+                                      const dsx_grid_ref *g, void *workspace,
+                                      size_t workspace_bytes, void *stream) {
+    int st = idx_check(d, v, p, mean, g);
+    if (st) return st;
+    IdxPlan P;
+    if ((st = idx_make_plan(d, v, p, P))) return st;
+    if (!workspace || workspace_bytes < P.total) return DSX_ERR_WORKSPACE;
+    uint8_t *ws = reinterpret_cast<uint8_t *>(workspace);
+    IdxArgs A;
+    A.a = args_of(d, v);
+    if (d->n == 0) {  // no rows: every tile reads nothing (the columns are never touched)
+        A.a.x = A.a.y = A.a.val = g->cell_start;
+    }
+    A.L = P.L;
+    A.tw = P.tw; A.th = P.th; A.ntx = P.ntx; A.nty = P.nty; A.ntiles = P.ntiles;
+    A.grid = g->grid;
+    A.ex0 = g->x0; A.esx = (double)g->grid / (g->x1 - g->x0);
+    A.ey0 = g->y0; A.esy = (double)g->grid / (g->y1 - g->y0);
+    A.cs = g->cell_start;
+    A.tile_rows = reinterpret_cast<unsigned long long *>(ws + P.o_rows);
+    A.part_start = reinterpret_cast<uint32_t *>(ws + P.o_pstart);
+    A.part_tile = reinterpret_cast<uint32_t *>(ws + P.o_ptile);
+    A.done = reinterpret_cast<uint32_t *>(ws + P.o_done);
+    A.part_len = reinterpret_cast<unsigned long long *>(ws + P.o_plen);
+    A.max_parts = P.max_parts;
+    A.cus = P.cus;
+    A.scratch = ws + P.o_scratch;
+    A.mean = mean;
+    A.out = planes_of(p);
+    hipStream_t s = as_stream(stream);
+    if (d->xy_dtype == DSX_F32) return idx_dispatch<float>(d, A, P, s);
+    return idx_dispatch<double>(d, A, P, s);
 }

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define DSX_ABI_VERSION 2
+#define DSX_ABI_VERSION 3
 
 /* ---- status codes --------------------------------------------------------------- */
 typedef enum {
@@ -207,6 +207,62 @@ typedef struct {
 } dsx_finalize_desc;
 
 int dsx_finalize(const dsx_finalize_desc *f, void *stream);
+
+/* ---- serving: grid index over a resident point set --------------------------------- *
+ * No counterpart in the reference: the reference scans every row on every call.  A
+ * viewport server keeps its point set resident, sorted by the cell of a grid x grid
+ * lattice over the set's extent, so that one viewport reads only the rows of the cells
+ * it overlaps.  dsx_grid_index_build sorts the rows (x, y and an optional value column,
+ * all of one dtype) by cell -- rows with a NaN coordinate go last, outside every cell --
+ * writes the sorted columns and cell_start[grid*grid + 1]: the rows of cell
+ * (gy*grid + gx) are [cell_start[c], cell_start[c+1]).  A row's cell is
+ * min(grid-1, (int)((x - x0) * (grid / (x1 - x0)))) (the same for y), so a viewport
+ * must take one extra cell on each side of the cells its range maps to. */
+typedef struct {
+    int64_t n;
+    int32_t grid;             /* cells per side, 1 .. 4096                                */
+    int32_t dtype;            /* DSX_F32 or DSX_F64 for x, y and val                       */
+    const void *x, *y, *val;  /* device columns (val may be NULL)                          */
+    double x0, x1, y0, y1;    /* the lattice's extent (x1 > x0, y1 > y0)                   */
+    void *out_x, *out_y, *out_val;  /* device, n rows each (out_val NULL without val)      */
+    int64_t *cell_start;      /* device [grid*grid + 1]                                   */
+} dsx_grid_index_desc;
+
+size_t dsx_grid_index_workspace_size(const dsx_grid_index_desc *g);
+int dsx_grid_index_build(const dsx_grid_index_desc *g, void *workspace,
+                         size_t workspace_bytes, void *stream);
+
+/* The lattice of a built index, as dsx_points_agg_indexed reads it. */
+typedef struct {
+    int32_t grid;             /* cells per side, 1 .. 1024                                */
+    int32_t reserved;
+    double x0, x1, y0, y1;    /* the lattice's extent, as given to dsx_grid_index_build    */
+    const int64_t *cell_start;  /* device [grid*grid + 1]                                 */
+} dsx_grid_ref;
+
+/* dsx_points_agg over the sorted columns of an index (d->x, d->y, d->val are the
+ * out_x / out_y / out_val of dsx_grid_index_build, d->n its row count): the canvas is
+ * cut into tiles whose planes fit LDS and each tile reads only the rows of the lattice
+ * cells under it, so no row is spilled or partitioned.  Same planes, results and error
+ * codes as dsx_points_agg; linear axes (xlog = ylog = 0) and no categorizer. */
+size_t dsx_points_indexed_workspace_size(const dsx_points_desc *d, const dsx_view *v,
+                                         const dsx_planes *p, const dsx_grid_ref *g);
+int dsx_points_agg_indexed(const dsx_points_desc *d, const dsx_view *v, const dsx_planes *p,
+                           const double *mean, const dsx_grid_ref *g, void *workspace,
+                           size_t workspace_bytes, void *stream);
+
+/* Copy row ranges of up to four columns into contiguous destinations: item k of the
+ * device array items[3*nitems] is (src_row, dst_row, rows). */
+typedef struct {
+    int32_t ncols;
+    int32_t elem_bytes[4];    /* 4 or 8                                                   */
+    const void *src[4];
+    void *dst[4];
+    const int64_t *items;
+    int64_t nitems;
+} dsx_gather_desc;
+
+int dsx_gather_rows(const dsx_gather_desc *g, void *stream);
 
 /* ---- misc --------------------------------------------------------------------------- */
 int dsx_abi_version(void);

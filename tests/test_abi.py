@@ -39,7 +39,8 @@ def test_library_exports_every_declared_function():
 STRUCTS = {
     "dsx_planes": _abi.Planes, "dsx_view": _abi.View, "dsx_categorizer": _abi.Categorizer,
     "dsx_points_desc": _abi.PointsDesc, "dsx_lines_desc": _abi.LinesDesc,
-    "dsx_finalize_desc": _abi.FinalizeDesc,
+    "dsx_finalize_desc": _abi.FinalizeDesc, "dsx_grid_index_desc": _abi.GridIndexDesc,
+    "dsx_gather_desc": _abi.GatherDesc, "dsx_grid_ref": _abi.GridRef,
 }
 
 
@@ -153,3 +154,71 @@ def test_lines_and_misc_validation():
     assert lib.dsx_bounds(None, _abi.F32, 4, ctypes.c_void_p(0x1000), None, 0, None) == -1
     assert lib.dsx_bounds(ctypes.c_void_p(0x1000), 9, 4, ctypes.c_void_p(0x1000), None, 0,
                           None) == -2
+
+
+def test_serving_validation():
+    """Host-side validation of the grid index and gather descriptors (nothing launched)."""
+    lib = _abi.load()
+    g = _abi.GridIndexDesc()
+    assert lib.dsx_grid_index_build(None, None, 0, None) == -1
+    g.n, g.grid, g.dtype = 0, 16, _abi.F32
+    assert lib.dsx_grid_index_build(g, None, 0, None) == -3          # no rows
+    g.n = 10
+    g.dtype = _abi.I32
+    assert lib.dsx_grid_index_build(g, None, 0, None) == -2
+    g.dtype = _abi.F32
+    assert lib.dsx_grid_index_build(g, None, 0, None) == -1          # columns NULL
+    for f in ("x", "y", "out_x", "out_y", "cell_start"):
+        setattr(g, f, ctypes.c_void_p(0x1000))
+    g.val = ctypes.c_void_p(0x1000)
+    assert lib.dsx_grid_index_build(g, None, 0, None) == -1          # val without out_val
+    g.val = None
+    g.x0, g.x1, g.y0, g.y1 = 0.0, 0.0, 0.0, 1.0
+    assert lib.dsx_grid_index_build(g, None, 0, None) == -3          # empty extent
+    g.x1, g.grid = 1.0, 5000
+    assert lib.dsx_grid_index_build(g, None, 0, None) == -3          # grid too fine
+    g.grid = 16
+    assert lib.dsx_grid_index_build(g, None, 0, None) == -4          # no workspace
+    d = _abi.GatherDesc()
+    assert lib.dsx_gather_rows(None, None) == -1
+    d.ncols = 0
+    assert lib.dsx_gather_rows(d, None) == -3
+    d.ncols, d.nitems = 1, 0
+    assert lib.dsx_gather_rows(d, None) == 0                        # nothing to copy
+    d.nitems = 2
+    assert lib.dsx_gather_rows(d, None) == -1                       # items NULL
+    d.items = ctypes.c_void_p(0x1000)
+    d.src[0] = d.dst[0] = 0x2000
+    d.elem_bytes[0] = 2
+    assert lib.dsx_gather_rows(d, None) == -2
+
+
+def test_indexed_validation():
+    lib = _abi.load()
+    d = _abi.PointsDesc()
+    d.n, d.xy_dtype = 10, _abi.F32
+    d.x = d.y = ctypes.c_void_p(0x1000)
+    pl = _abi.Planes()
+    pl.count = ctypes.c_void_p(0x2000)
+    v = _view()
+    g = _abi.GridRef()
+    g.grid, g.x0, g.x1, g.y0, g.y1 = 64, 0.0, 1.0, 0.0, 1.0
+
+    def call(**kw):
+        return lib.dsx_points_agg_indexed(d, kw.get("view", v), pl, None, kw.get("g", g),
+                                          None, 0, None)
+    assert call() == -1                                             # cell_start NULL
+    g.cell_start = ctypes.c_void_p(0x3000)
+    assert call() == -4                                             # no workspace
+    g.grid = 2048
+    assert call() == -3                                             # lattice too fine
+    g.grid = 64
+    v.xlog = 1
+    assert call() == -7                                             # log axis
+    v.xlog = 0
+    d.cat.kind, d.cat.dtype, d.cat.ncat, d.cat.col = _abi.CAT_CODES, _abi.I8, 4, 0x4000
+    assert call() == -7                                             # categorizer
+    d.cat.kind, d.cat.ncat = _abi.CAT_NONE, 1
+    pl.sum = ctypes.c_void_p(0x5000)
+    assert call() == -6                                             # sum without a value
+    assert lib.dsx_points_indexed_workspace_size(d, v, pl, g) == 0
