@@ -67,7 +67,8 @@ class LinesDesc(ctypes.Structure):
                 ("x_line_stride", ctypes.c_int64), ("x_vert_stride", ctypes.c_int64),
                 ("y_line_stride", ctypes.c_int64), ("y_vert_stride", ctypes.c_int64),
                 ("axis", ctypes.c_int32), ("plot_start", ctypes.c_int32),
-                ("val", vp), ("val_dtype", ctypes.c_int32), ("cat", Categorizer)]
+                ("val", vp), ("val_dtype", ctypes.c_int32), ("cat", Categorizer),
+                ("mean", vp)]
 
 
 class FinalizeDesc(ctypes.Structure):

@@ -250,8 +250,6 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W):
             m2s.sum = dev.ptr(planes["m2"])
             _points_call(lib, d, vw, m2s, mean, device, stream, grid)
     else:
-        if p.m2:
-            raise NotImplementedError("var / std of lines are not supported")
         lay = glyph.vertex_layout(frame)
         d = _abi.LinesDesc()
         d.nlines, d.nverts = lay["nlines"], lay["nverts"]
@@ -264,6 +262,16 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W):
         d.val, d.val_dtype = dev.ptr(val), vdt
         d.cat = catc
         _abi.check(_timed(lambda: lib.dsx_lines_agg(d, vw, ps, stream)), "dsx_lines_agg")
+        if p.m2:
+            # var / std: squared deviations of every pixel's value from the cell mean
+            mean = t.empty(ncells, dtype=t.float64, device=device)
+            _finalize(lib, _abi.FIN_MEAN, keyb, ncells, planes, mean)
+            planes["m2"] = _alloc(ncells, "f64", device)
+            _fill(lib, planes["m2"], _abi.FILL_ZERO64)
+            m2s = _abi.Planes()
+            m2s.sum = dev.ptr(planes["m2"])
+            d.mean = dev.ptr(mean)
+            _abi.check(lib.dsx_lines_agg(d, vw, m2s, stream), "dsx_lines_agg")
     return PassResult(ncat, keyb, planes)
 
 

@@ -12,6 +12,7 @@ namespace dsx {
 struct LineArgs {
     int64_t nlines, nverts;
     const void *x, *y, *val;
+    const double *mean;  // squared-deviation mode (var / std second pass)
     Cat cat;
     int64_t xls, xvs, yls, yvs;
     int32_t axis, plot_start;
@@ -36,16 +37,24 @@ __device__ __forceinline__ bool clipt(double p, double q, double &t0, double &t1
     return true;
 }
 
-template <typename TV, bool HAS_VAL>
+template <typename TV, bool HAS_VAL, bool SQDEV = false>
 struct Sink {
     const Planes *p;
+    const double *mean;
     int64_t W, H, ncat, c;
     TV v;
     __device__ __forceinline__ void operator()(int64_t x, int64_t y) const {
         if (x < 0) x += W;  // numba's negative-index wraparound
         if (y < 0) y += H;
         if ((uint64_t)x >= (uint64_t)W || (uint64_t)y >= (uint64_t)H) return;
-        append<TV, HAS_VAL>(*p, (y * W + x) * ncat + c, v);
+        const int64_t cell = (y * W + x) * ncat + c;
+        if constexpr (SQDEV) {
+            if (v != v) return;
+            const double d = (double)v - mean[cell];
+            unsafeAtomicAdd(p->sum + cell, d * d);
+        } else {
+            append<TV, HAS_VAL>(*p, cell, v);
+        }
     }
     template <typename I>
     __device__ __forceinline__ void walk(bool start, I x0, I x1, I y0, I y1, bool clipped) const;
@@ -92,10 +101,10 @@ __device__ __forceinline__ void bresenham(const S &put, bool segment_start, I x0
     }
 }
 
-template <typename TV, bool HAS_VAL>
+template <typename TV, bool HAS_VAL, bool SQDEV>
 template <typename I>
-__device__ __forceinline__ void Sink<TV, HAS_VAL>::walk(bool start, I x0, I x1, I y0, I y1,
-                                                       bool clipped) const {
+__device__ __forceinline__ void Sink<TV, HAS_VAL, SQDEV>::walk(bool start, I x0, I x1, I y0,
+                                                              I y1, bool clipped) const {
     bresenham<I>(*this, start, x0, x1, y0, y1, clipped);
 }
 
@@ -153,7 +162,7 @@ __device__ __forceinline__ void draw_segment(const LineArgs &a, const S &put, bo
     put.template walk<I>(segment_start, (I)px0, (I)px1, (I)py0, (I)py1, clipped);
 }
 
-template <typename TX, typename TY, typename TV, bool HAS_VAL>
+template <typename TX, typename TY, typename TV, bool HAS_VAL, bool SQDEV>
 __global__ __launch_bounds__(256) void lines_kernel(LineArgs a, Planes p) {
     const int64_t nseg = a.nverts - 1;
     const int64_t total = a.nlines * nseg;
@@ -173,8 +182,8 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a, Planes p) {
             start = (xp != xp) || (yp != yp);
         }
         const int64_t row = a.axis == 0 ? k : l;
-        Sink<TV, HAS_VAL> put;
-        put.p = &p; put.W = a.W; put.H = a.H; put.ncat = a.cat.ncat;
+        Sink<TV, HAS_VAL, SQDEV> put;
+        put.p = &p; put.mean = a.mean; put.W = a.W; put.H = a.H; put.ncat = a.cat.ncat;
         put.v = TV(0);
         put.c = category(a.cat, row);
         if (put.c < 0) continue;
@@ -338,9 +347,11 @@ __global__ __launch_bounds__(kStripThreads) void lines_xconst_strip_kernel(LineA
 template <typename TX, typename TY>
 static void launch_lines(const LineArgs &a, const Planes &p, int32_t vdt, int grid, int block,
                          hipStream_t s) {
-    if (!a.val) lines_kernel<TX, TY, float, false><<<grid, block, 0, s>>>(a, p);
-    else if (vdt == DSX_F32) lines_kernel<TX, TY, float, true><<<grid, block, 0, s>>>(a, p);
-    else lines_kernel<TX, TY, double, true><<<grid, block, 0, s>>>(a, p);
+    if (!a.val) lines_kernel<TX, TY, float, false, false><<<grid, block, 0, s>>>(a, p);
+    else if (a.mean && vdt == DSX_F32) lines_kernel<TX, TY, float, true, true><<<grid, block, 0, s>>>(a, p);
+    else if (a.mean) lines_kernel<TX, TY, double, true, true><<<grid, block, 0, s>>>(a, p);
+    else if (vdt == DSX_F32) lines_kernel<TX, TY, float, true, false><<<grid, block, 0, s>>>(a, p);
+    else lines_kernel<TX, TY, double, true, false><<<grid, block, 0, s>>>(a, p);
 }
 
 }  // namespace dsx
@@ -360,11 +371,13 @@ extern "C" int dsx_lines_agg(const dsx_lines_desc *d, const dsx_view *v, const d
     if (cst) return cst;
     const bool needs_val = acc->countv || acc->sum || acc->min || acc->max;
     if (needs_val && !d->val) return DSX_ERR_ACCUM;
+    if (d->mean && (!d->val || !acc->sum || acc->count || acc->countv || acc->min || acc->max))
+        return DSX_ERR_ACCUM;
     if (d->nlines == 0 || d->nverts < 2) return DSX_OK;
     if (!d->x || !d->y) return DSX_ERR_NULL;
     LineArgs a;
     a.nlines = d->nlines; a.nverts = d->nverts;
-    a.x = d->x; a.y = d->y; a.val = d->val; a.cat = cat_of(d->cat);
+    a.x = d->x; a.y = d->y; a.val = d->val; a.cat = cat_of(d->cat); a.mean = d->mean;
     a.xls = d->x_line_stride; a.xvs = d->x_vert_stride;
     a.yls = d->y_line_stride; a.yvs = d->y_vert_stride;
     a.axis = d->axis; a.plot_start = d->plot_start;

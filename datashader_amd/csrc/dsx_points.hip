@@ -178,6 +178,32 @@ __device__ __forceinline__ void region_store(const uint8_t *r, const Lay &L, uin
     if (L.flags & PL_CNTV) out.countv[g] = reinterpret_cast<const uint32_t *>(r + L.o_cntv)[c];
 }
 
+// Add LDS region cell c (a tile with a NaN plane) to the output planes at global cell g,
+// which the window merge already wrote: count += cnt, countv += cnt - nan, and the value
+// planes combined.  Nothing to do for an empty cell.
+template <int KEYB>
+__device__ __forceinline__ void region_add_to(const uint8_t *r, const Lay &L, uint32_t c,
+                                              const Planes &out, int64_t g) {
+    using K = KeyOf<KEYB>;
+    using T = typename K::T;
+    const uint32_t n = reinterpret_cast<const uint32_t *>(r + L.o_cnt)[c];
+    if (n == 0) return;
+    const uint32_t nn = reinterpret_cast<const uint32_t *>(r + L.o_nan)[c];
+    if (out.count) out.count[g] += n;
+    out.countv[g] += n - nn;
+    if (L.flags & PL_SUM) out.sum[g] += reinterpret_cast<const double *>(r + L.o_sum)[c];
+    if (L.flags & PL_MIN) {
+        T &m = reinterpret_cast<T *>(out.min)[g];
+        const T x = reinterpret_cast<const T *>(r + L.o_min)[c];
+        m = x < m ? x : m;
+    }
+    if (L.flags & PL_MAX) {
+        T &m = reinterpret_cast<T *>(out.max)[g];
+        const T x = reinterpret_cast<const T *>(r + L.o_max)[c];
+        m = x > m ? x : m;
+    }
+}
+
 // Fold cell c of another region with layout L (global memory) into region r (LDS); each
 // cell is owned by one thread, so no atomics.
 template <int KEYB>
@@ -346,6 +372,7 @@ struct ScanArgs {
     uint32_t tile_cells;
     int32_t ntiles;
     uint32_t o_thist, o_stage_cell, o_stage_val;  // LDS byte offsets
+    uint32_t spill_nan;              // rows with a NaN value are spilled (count with countv)
     const double *mean;              // squared-deviation mode (m2 pass)
 };
 
@@ -442,9 +469,9 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
                 pc = scan_category<CK>(a.cat, cur.c[it]);
                 keep = keep && pc >= 0;
             }
-            const bool inw = keep && in_window(w, px, py);
             const double v = HAS_VAL ? (double)cur.v[it] : 0.0;
             const bool vnan = HAS_VAL ? (v != v) : true;
+            const bool inw = keep && in_window(w, px, py) && !(HAS_VAL && s.spill_nan && vnan);
             if (inw) {
                 const uint32_t lc = (uint32_t)(((py - w.y0) * w.w + (px - w.x0)) * ncat + pc);
                 double add = v;
@@ -593,6 +620,10 @@ __global__ __launch_bounds__(kMergeCells * kMergeWays) void merge_window_kernel(
     if (L.flags & PL_CNT) {
         if (out.count) out.count[g] = s_cnt[0][cl];
         if (L.flags & PL_NAN) out.countv[g] = s_cnt[0][cl] - s_nan[0][cl];
+        // count() with count(v): the window holds the rows with a value only (rows with
+        // a NaN value are spilled), so its count is also its countv; the tile adds the
+        // spilled rows of the cell afterwards
+        else if (out.countv) out.countv[g] = s_cnt[0][cl];
     }
     if (L.flags & PL_CNTV) out.countv[g] = s_cntv[0][cl];
 }
@@ -939,7 +970,12 @@ __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay 
     for (int64_t gcell = c0 + threadIdx.x; gcell < c1; gcell += kTileThreads) {
         const int64_t pix = gcell / ncat;
         const int32_t py = (int32_t)(pix / W), px = (int32_t)(pix - (int64_t)py * W);
-        if (in_window(w, px, py)) continue;  // written by the window merge
+        if (in_window(w, px, py)) {
+            // written by the window merge; the only spilled rows of a window cell are
+            // rows with a NaN value (count with countv): add them
+            if (L.flags & PL_NAN) region_add_to<KEYB>(smem, L, (uint32_t)(gcell - c0), out, gcell);
+            continue;
+        }
         region_store<KEYB>(smem, L, (uint32_t)(gcell - c0), out, gcell);
     }
 }
@@ -989,16 +1025,19 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     tcap = (P.ncells + P.ntiles - 1) / P.ntiles;
     P.tl = make_lay(P.flags, P.keyb, (uint32_t)tcap);
     P.tile_lds = P.tl.bytes;
-    // window capacity: LDS minus the scan's tile histogram and control words
-    // scan LDS: window planes, tile histogram, per-wave spill staging, control words
+    // scan LDS: window planes, tile histogram, per-wave spill staging, control words.
+    // The window keeps no NaN plane: with count() and count(v) both asked for, rows with
+    // a NaN value are spilled and the tile adds them to the window's cells.
+    const uint32_t wflags = P.flags & ~PL_NAN;
+    const uint32_t cbw = cell_bytes(wflags, P.keyb);
     const size_t stage = (size_t)kWaves * kStage * (4 + (size_t)P.vbytes);
     const size_t side = align_up((size_t)4 * P.ntiles, 16) + stage + kCtrlBytes;
-    const int64_t wcap_cells = (int64_t)((kLdsBytes - side - 16) / cb);
+    const int64_t wcap_cells = (int64_t)((kLdsBytes - side - 16) / cbw);
     int64_t wcap_px = wcap_cells / ncat;
     if (wcap_px < 1) return DSX_ERR_SHAPE;
     P.whole = (int64_t)v->width * v->height <= wcap_px;
     if (P.whole) wcap_px = (int64_t)v->width * v->height;
-    P.wl = make_lay(P.flags, P.keyb, (uint32_t)(wcap_px * ncat));
+    P.wl = make_lay(wflags, P.keyb, (uint32_t)(wcap_px * ncat));
     P.o_thist = P.wl.bytes;
     P.o_stage_cell = P.o_thist + (uint32_t)align_up((size_t)4 * P.ntiles, 16);
     P.o_stage_val = P.o_stage_cell + (uint32_t)(kWaves * kStage * 4);
@@ -1102,6 +1141,7 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     sa.o_thist = P.o_thist;
     sa.o_stage_cell = P.o_stage_cell;
     sa.o_stage_val = P.o_stage_val;
+    sa.spill_nan = (P.flags & PL_NAN) ? 1u : 0u;
     sa.mean = mean;
     // the scan reads row 0 of each column even for an empty input (and drops it): give
     // an empty input a readable dummy row in the workspace

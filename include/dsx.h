@@ -160,6 +160,9 @@ typedef struct {
     int32_t plot_start;           /* axis 0: draw the first vertex                        */
     const void *val; int32_t val_dtype;
     dsx_categorizer cat;          /* indexed like val                                     */
+    const double *mean;           /* NULL, or per-cell means: then the only plane given,   */
+                                  /* sum, accumulates (v - mean[cell])^2 of every pixel    */
+                                  /* with a value (the var / std second pass)              */
 } dsx_lines_desc;
 
 int dsx_lines_agg(const dsx_lines_desc *d, const dsx_view *v, const dsx_planes *p,

@@ -218,3 +218,36 @@ def test_xconst_strips_vs_oracle_and_general_path(case):
     general = abi_lines(np.tile(xs, (nlines, 1)), ys, view, W, H, axis=1, x_ls=nverts, x_vs=1,
                         y_ls=nverts, y_vs=1, nlines=nlines, nverts=nverts)
     np.testing.assert_array_equal(general, exp)
+
+
+def test_line_value_reductions_vs_oracle():
+    """sum / mean / min / max / var / std along lines (axis 0): each pixel of a segment takes
+    the value of the segment's first row; var / std are two-pass on the GPU and the
+    reference's sequential m2 in the oracle (float64 values: relative 1e-9)."""
+    rng = np.random.default_rng(9)
+    n, W, H = 30000, 96, 64
+    x = np.cumsum(rng.normal(0, 0.05, n))
+    y = np.cumsum(rng.normal(0, 0.05, n))
+    v = rng.normal(3.0, 2.0, n)
+    v[rng.random(n) < 0.01] = np.nan
+    df = pd.DataFrame({"x": x, "y": y, "v": v})
+    xr_ = (float(np.percentile(x, 5)), float(np.percentile(x, 95)))
+    yr_ = (float(np.percentile(y, 5)), float(np.percentile(y, 95)))
+    c = dsa.Canvas(W, H, x_range=xr_, y_range=yr_)
+    view = G.view_for(W, H, xr_, yr_)
+    r = oracle.line_axis0(x, y, view, W, H, ("countv", "sumz", "min", "max", "m2"), val=v)
+    agg = c.line(df, "x", "y", agg=dsa.summary(s=dsa.sum("v"), m=dsa.mean("v"), mn=dsa.min("v"),
+                                              mx=dsa.max("v"), var=dsa.var("v"),
+                                              std=dsa.std("v"), n=dsa.count("v")))
+    cv = r["countv"]
+    np.testing.assert_array_equal(agg["n"].values, cv)
+    has = cv > 0
+    for name, exp, tol in (("s", r["sumz"], 1e-12), ("m", r["sumz"] / np.maximum(cv, 1), 1e-12),
+                           ("var", r["m2"] / np.maximum(cv, 1), 1e-9),
+                           ("std", np.sqrt(r["m2"] / np.maximum(cv, 1)), 1e-9)):
+        got = agg[name].values
+        assert np.all(np.isnan(got[~has]))
+        np.testing.assert_allclose(got[has], exp[has], rtol=tol, atol=1e-9)
+    G.assert_same(agg["mn"].values, r["min"])
+    G.assert_same(agg["mx"].values, r["max"])
+    assert has.sum() > 100
