@@ -145,19 +145,22 @@ __device__ __forceinline__ void region_init(uint8_t *r, const Lay &L, int tid, i
 
 // LDS atomic update of cell c by one row with value v (vnan: the value is NaN / absent
 // for value-reading planes).  `add` is the quantity summed (v, or its squared deviation).
-template <int KEYB>
+// WF, when not 0, is the region's plane set known at compile time (it must equal L.flags):
+// the per-plane tests then fold away.
+template <int KEYB, uint32_t WF = 0>
 __device__ __forceinline__ void region_add(uint8_t *r, const Lay &L, uint32_t c, bool vnan,
                                            double v, double add) {
     using K = KeyOf<KEYB>;
-    if (L.flags & PL_CNT) atomicAdd(reinterpret_cast<uint32_t *>(r + L.o_cnt) + c, 1u);
+    const uint32_t f = WF ? WF : L.flags;
+    if (f & PL_CNT) atomicAdd(reinterpret_cast<uint32_t *>(r + L.o_cnt) + c, 1u);
     if (vnan) {
-        if (L.flags & PL_NAN) atomicAdd(reinterpret_cast<uint32_t *>(r + L.o_nan) + c, 1u);
+        if (f & PL_NAN) atomicAdd(reinterpret_cast<uint32_t *>(r + L.o_nan) + c, 1u);
         return;
     }
-    if (L.flags & PL_CNTV) atomicAdd(reinterpret_cast<uint32_t *>(r + L.o_cntv) + c, 1u);
-    if (L.flags & PL_SUM) atomicAdd(reinterpret_cast<double *>(r + L.o_sum) + c, add);
-    if (L.flags & PL_MIN) atomicMin(reinterpret_cast<typename K::T *>(r + L.o_min) + c, K::key(v));
-    if (L.flags & PL_MAX) atomicMax(reinterpret_cast<typename K::T *>(r + L.o_max) + c, K::key(v));
+    if (f & PL_CNTV) atomicAdd(reinterpret_cast<uint32_t *>(r + L.o_cntv) + c, 1u);
+    if (f & PL_SUM) atomicAdd(reinterpret_cast<double *>(r + L.o_sum) + c, add);
+    if (f & PL_MIN) atomicMin(reinterpret_cast<typename K::T *>(r + L.o_min) + c, K::key(v));
+    if (f & PL_MAX) atomicMax(reinterpret_cast<typename K::T *>(r + L.o_max) + c, K::key(v));
 }
 
 // Copy LDS region cell c into the output planes at global cell g.
@@ -428,7 +431,10 @@ __device__ __forceinline__ int32_t scan_category(const Cat &c, int64_t raw) {
     return (int32_t)category_raw(c, raw);
 }
 
-template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV, int CK>
+// LIN: linear axes (no log10 call in the loop); WF: the window's plane set when known at
+// compile time (0: read from the layout).
+template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV, int CK, bool LIN = false,
+          uint32_t WF = 0>
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArgs s) {
     extern __shared__ __align__(16) uint8_t smem[];
     uint8_t *win = smem;
@@ -464,7 +470,29 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
         for (int it = 0; it < kItems; ++it) {
             const uint32_t j = (uint32_t)(it * kScanThreads + tid);
             int32_t px = 0, py = 0, pc = 0;
-            bool keep = j < rem && locate_xy<TC>(a, cur.x[it], cur.y[it], px, py);
+            bool keep;
+            if constexpr (LIN) {
+                // locate_xy without branches: every test is evaluated and and-ed; the
+                // projection is clamped before the conversion so that a NaN or
+                // out-of-range coordinate converts to a defined value.  The clamp keeps
+                // every value that can survive the tests below (wraparound included:
+                // anything under -W is dropped either way)
+                const double x = (double)cur.x[it], y = (double)cur.y[it];
+                const bool inr = (a.xmin <= x) & (x <= a.xmax) & (a.ymin <= y) & (y <= a.ymax);
+                const double fx = fmin(fmax(__dadd_rn(__dmul_rn(x, a.sx), a.tx), -(double)a.W - 4.0), 2.0e9);
+                const double fy = fmin(fmax(__dadd_rn(__dmul_rn(y, a.sy), a.ty), -(double)a.H - 4.0), 2.0e9);
+                int32_t xx = (int32_t)fx, yy = (int32_t)fy;
+                xx = xx >= a.W ? a.W - 1 : xx;
+                yy = yy >= a.H ? a.H - 1 : yy;
+                xx += xx < 0 ? a.W : 0;  // numba's negative-index wraparound
+                yy += yy < 0 ? a.H : 0;
+                keep = (j < rem) & inr & ((uint32_t)xx < (uint32_t)a.W) &
+                       ((uint32_t)yy < (uint32_t)a.H);
+                px = xx;
+                py = yy;
+            } else {
+                keep = j < rem && locate_xy<TC>(a, cur.x[it], cur.y[it], px, py);
+            }
             if constexpr (CK != CK_NONE) {
                 pc = scan_category<CK>(a.cat, cur.c[it]);
                 keep = keep && pc >= 0;
@@ -480,7 +508,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
                     const double d = v - s.mean[gc];
                     add = d * d;
                 }
-                region_add<KEYB>(win, s.wl, lc, vnan, v, add);
+                region_add<KEYB, WF>(win, s.wl, lc, vnan, v, add);
             }
             // spill: staged at the wave's running count, then appended to its sub-segment
             const bool spill = keep && !inw;
@@ -1154,6 +1182,22 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     if (a.cat.kind == DSX_CAT_NONE) scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE>;
     else if (a.cat.dtype == DSX_I8 || a.cat.dtype == DSX_U8)
         scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_BYTE>;
+    // the common plane sets over linear axes, without categorizer: compile-time planes
+    if (a.cat.kind == DSX_CAT_NONE && !a.xlog && !a.ylog) {
+        const uint32_t wf = P.wl.flags;
+        if constexpr (!HAS_VAL) {
+            if (wf == PL_CNT) scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE, true, PL_CNT>;
+        } else if constexpr (SQDEV) {
+            if (wf == PL_SUM) scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE, true, PL_SUM>;
+        } else {
+            if (wf == (PL_CNT | PL_SUM))
+                scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE, true, PL_CNT | PL_SUM>;
+            else if (wf == (PL_CNTV | PL_SUM))
+                scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE, true, PL_CNTV | PL_SUM>;
+            else if (wf == PL_CNT)
+                scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE, true, PL_CNT>;
+        }
+    }
     hipError_t e = hipFuncSetAttribute((const void *)scan,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.scan_lds);
     if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
