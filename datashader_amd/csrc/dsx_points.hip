@@ -308,13 +308,16 @@ __global__ __launch_bounds__(1024) void select_pick_kernel(const uint32_t *hist,
     __syncthreads();
     const int64_t b = 1ll << bshift;
     unsigned long long mybest = 0ull;  // (count << 32) | (0xFFFFFFFF - encoded choice)
-    const float aspects[5] = {0.25f, 0.5f, 1.0f, 2.0f, 4.0f};
-    for (int s = 0; s < 5; ++s) {
+    // shapes of about cap_px pixels; a shape's last bin column may overhang the capacity
+    // (the window is trimmed to it, pixel by pixel, once chosen)
+    constexpr int kAspects = 9;
+    const float aspects[kAspects] = {0.25f, 0.35f, 0.5f, 0.7f, 1.0f, 1.4f, 2.0f, 2.8f, 4.0f};
+    for (int s = 0; s < kAspects; ++s) {
         int kh = (int)sqrtf((float)cap_px / (aspects[s] * (float)(b * b)));
         if (kh < 1) kh = 1;
         if (kh > ch) kh = ch;
-        int kw = (int)(cap_px / ((int64_t)kh * b * b));
-        if (kw < 1) continue;
+        if ((int64_t)kh * b > cap_px) continue;
+        int kw = (int)((cap_px + (int64_t)kh * b * b - 1) / ((int64_t)kh * b * b));
         if (kw > cw) kw = cw;
         const int nx = cw - kw + 1, ny = ch - kh + 1;
         for (int p = threadIdx.x; p < nx * ny; p += blockDim.x) {
@@ -344,15 +347,40 @@ __global__ __launch_bounds__(1024) void select_pick_kernel(const uint32_t *hist,
         int kh = (int)sqrtf((float)cap_px / (aspects[s] * (float)(b * b)));
         if (kh < 1) kh = 1;
         if (kh > ch) kh = ch;
-        int kw = (int)(cap_px / ((int64_t)kh * b * b));
+        int kw = (int)((cap_px + (int64_t)kh * b * b - 1) / ((int64_t)kh * b * b));
         if (kw > cw) kw = cw;
         if (kw < 1) kw = 1;
-        const int px0 = (int)(x0 * b), py0 = (int)(y0 * b);
+        int px0 = (int)(x0 * b);
+        const int py0 = (int)(y0 * b);
         int ww = (int)(kw * b), wh = (int)(kh * b);
         if (px0 + ww > W) ww = W - px0;
         if (py0 + wh > H) wh = H - py0;
+        if (wh >= 1 && (int64_t)ww * wh > cap_px) {  // trim the overhang, about the centre
+            const int tw = (int)(cap_px / wh);
+            px0 += (ww - tw) / 2;
+            ww = tw;
+        }
         if ((int64_t)ww * wh > cap_px || ww < 1 || wh < 1) { ww = 0; wh = 0; }
-        win[0] = px0; win[1] = py0; win[2] = ww; win[3] = wh;
+        // the coarse shape leaves up to a bin row / column of capacity unused: widen,
+        // then heighten, the window pixel by pixel around its centre to use it
+        int wx0 = px0, wy0 = py0;
+        if (ww > 0) {
+            int64_t mw = cap_px / wh;
+            mw = mw > W ? W : mw;
+            if (mw > ww) {
+                wx0 -= (int)((mw - ww) / 2);
+                ww = (int)mw;
+                wx0 = wx0 < 0 ? 0 : (wx0 + ww > W ? W - ww : wx0);
+            }
+            int64_t mh = cap_px / ww;
+            mh = mh > H ? H : mh;
+            if (mh > wh) {
+                wy0 -= (int)((mh - wh) / 2);
+                wh = (int)mh;
+                wy0 = wy0 < 0 ? 0 : (wy0 + wh > H ? H - wh : wy0);
+            }
+        }
+        win[0] = wx0; win[1] = wy0; win[2] = ww; win[3] = wh;
     }
 }
 
@@ -372,7 +400,7 @@ struct ScanArgs {
     uint32_t *seg_count;             // [G * kWaves] entries in each sub-segment
     int64_t seg_cap;                 // entries a sub-segment can hold (its wave's rows)
     uint32_t *hist;                  // [G][ntiles] spilled entries per tile
-    uint32_t tile_cells;
+    uint32_t tile_cells, tile_shift;  // tile_cells = 1 << tile_shift
     int32_t ntiles;
     uint32_t o_thist, o_stage_cell, o_stage_val;  // LDS byte offsets
     uint32_t spill_nan;              // rows with a NaN value are spilled (count with countv)
@@ -535,7 +563,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
                         s.spill_cell[seg0 + slot] = c;
                         if constexpr (HAS_VAL) spill_val[seg0 + slot] = vv;
                         // counted here, with every lane busy, rather than per binned row
-                        atomicAdd(&thist[c / s.tile_cells], 1u);
+                        atomicAdd(&thist[c >> s.tile_shift], 1u);
                     }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -565,7 +593,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
             const uint32_t c = st_cell[lane];
             s.spill_cell[seg0 + slot] = c;
             if constexpr (HAS_VAL) spill_val[seg0 + slot] = st_val[lane];
-            atomicAdd(&thist[c / s.tile_cells], 1u);
+            atomicAdd(&thist[c >> s.tile_shift], 1u);
         }
     }
     __syncthreads();
@@ -665,14 +693,15 @@ struct PartArgs {
     const uint32_t *seg_count;        // [G * kWaves]
     int64_t seg_cap;
     int32_t ngroups;                  // G (<= kPartThreads)
-    uint32_t tile_cells;
+    uint32_t tile_cells, tile_shift;  // tile_cells = 1 << tile_shift
     int32_t ntiles;
     uint32_t *hist;                   // [G][ntiles]
     unsigned long long *tile_start;   // [ntiles + 1]
     uint32_t *sorted_cell;
     void *sorted_val;
     // tile work split: tile t is processed by parts [part_start[t], part_start[t + 1]),
-    // each of at most *part_len entries; done[t] counts finished parts of a split tile
+    // over which tile t's entries are split evenly (*part_len: the largest part size);
+    // done[t] counts finished parts of a split tile
     uint32_t *part_start;             // [ntiles + 1]
     uint32_t *part_tile;              // [max_parts] tile of each part
     int32_t max_parts;
@@ -736,11 +765,13 @@ __global__ __launch_bounds__(kPartThreads) void spill_tilescan_kernel(PartArgs p
         carry += tot;
     }
     if (threadIdx.x == 0) p.tile_start[p.ntiles] = carry;
-    // parts of at most E = max(kMinPart, 2 * mean) entries: at most ntiles + ntiles / 2
-    // parts in all, and a tile far heavier than the mean (the edge of a dense cluster
-    // just outside the window) is spread over several workgroups
-    const unsigned long long mean2 = 2 * ((carry + p.ntiles - 1) / p.ntiles);
-    const unsigned long long E = mean2 > kMinPart ? mean2 : (unsigned long long)kMinPart;
+    // parts of at most E = max(kMinPart, 1.5 * mean) entries (a tile's entries are split
+    // evenly over its parts): at most ntiles + 2 * ntiles / 3 parts in all, and a tile
+    // heavier than the mean (the edge of a dense cluster just outside the window) is
+    // spread over several workgroups
+    const unsigned long long mean = (carry + p.ntiles - 1) / p.ntiles;
+    const unsigned long long e15 = mean + (mean + 1) / 2;
+    const unsigned long long E = e15 > kMinPart ? e15 : (unsigned long long)kMinPart;
     if (threadIdx.x == 0) *p.part_len = (uint32_t)(E < 0xFFFFFFFFull ? E : 0xFFFFFFFFull);
     unsigned long long pcarry = 0;
     for (int t0 = 0; t0 < p.ntiles; t0 += blockDim.x) {
@@ -840,7 +871,7 @@ __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p)
             rt[k] = 0xFFFFFFFFu;
             rr[k] = 0u;
             if (v < total) {
-                const uint32_t t = rc[k] / p.tile_cells;
+                const uint32_t t = rc[k] >> p.tile_shift;
                 if (t < (uint32_t)p.ntiles) {  // always, for entries the scan wrote
                     rt[k] = t;
                     rr[k] = atomicAdd(&cnt[t], 1u);
@@ -928,8 +959,8 @@ __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay 
     region_init<KEYB>(smem, L, threadIdx.x, kTileThreads);
     __syncthreads();
     const uint32_t first = p.part_start[t], nparts = p.part_start[t + 1] - first;
-    const unsigned long long len = *p.part_len;
     const unsigned long long tb = p.tile_start[t], te = p.tile_start[t + 1];
+    const unsigned long long len = (te - tb + nparts - 1) / nparts;  // an even split
     unsigned long long b = tb + (unsigned long long)(part - first) * len;
     unsigned long long e = b + len;
     b = b < te ? b : te;
@@ -1027,6 +1058,7 @@ struct Plan {
     int32_t max_parts;   // upper bound of the tile parts (grid of the tile kernel)
     bool whole;          // the window is the whole canvas
     int bshift, cw, ch;
+    int tshift;          // tile_cells = 1 << tshift
 };
 
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -1041,16 +1073,16 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     P.ncells = (int64_t)v->width * v->height * ncat;
     if (P.ncells >= (1ll << 32)) return DSX_ERR_SHAPE;
     const uint32_t cb = cell_bytes(P.flags, P.keyb);
-    // tiles: equal runs of cells whose planes fit kTileLds (several tile workgroups share
-    // a CU), or up to the whole LDS when that would make more than kMaxTiles tiles
-    int64_t tcap = (kTileLds - 16) / cb;
-    if ((P.ncells + tcap - 1) / tcap > kMaxTiles) {
-        tcap = (P.ncells + kMaxTiles - 1) / kMaxTiles;
-        if (tcap * (int64_t)cb > kLdsBytes - 16) return DSX_ERR_SHAPE;
-    }
-    P.ntiles = (int32_t)((P.ncells + tcap - 1) / tcap);
+    // tiles: runs of 2^tshift cells (a cell's tile is one shift) whose planes fit kTileLds
+    // (several tile workgroups share a CU), or up to the whole LDS when that would make
+    // more than kMaxTiles tiles
+    P.tshift = 0;
+    while ((int64_t)(2ll << P.tshift) * cb <= kTileLds - 16) ++P.tshift;
+    while ((P.ncells + (1ll << P.tshift) - 1) >> P.tshift > kMaxTiles) ++P.tshift;
+    const int64_t tcap = 1ll << P.tshift;
+    if (tcap * (int64_t)cb > kLdsBytes - 16) return DSX_ERR_SHAPE;
+    P.ntiles = (int32_t)((P.ncells + tcap - 1) >> P.tshift);
     if (P.ntiles < 1) P.ntiles = 1;
-    tcap = (P.ncells + P.ntiles - 1) / P.ntiles;
     P.tl = make_lay(P.flags, P.keyb, (uint32_t)tcap);
     P.tile_lds = P.tl.bytes;
     // scan LDS: window planes, tile histogram, per-wave spill staging, control words.
@@ -1104,7 +1136,7 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     P.o_tile_start = o; o = align_up(o + 8 * (size_t)(P.ntiles + 1), 256);
     P.o_sorted_cell = o; o = align_up(o + 4 * (size_t)n, 256);
     P.o_sorted_val = o; o = align_up(o + (size_t)P.vbytes * n, 256);
-    P.max_parts = P.ntiles + P.ntiles / 2 + 1;
+    P.max_parts = 2 * P.ntiles + 1;
     P.o_part_start = o; o = align_up(o + 4 * (size_t)(P.ntiles + 1), 256);
     P.o_part_len = o; o = align_up(o + 4, 256);
     P.o_part_tile = o; o = align_up(o + 4 * (size_t)P.max_parts, 256);
@@ -1165,6 +1197,7 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     sa.seg_cap = P.seg_cap;
     sa.hist = reinterpret_cast<uint32_t *>(ws + P.o_hist);
     sa.tile_cells = P.tl.cap;
+    sa.tile_shift = P.tshift;
     sa.ntiles = P.ntiles;
     sa.o_thist = P.o_thist;
     sa.o_stage_cell = P.o_stage_cell;
@@ -1214,6 +1247,7 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     pa.seg_cap = P.seg_cap;
     pa.ngroups = P.scan_groups;
     pa.tile_cells = P.tl.cap;
+    pa.tile_shift = P.tshift;
     pa.ntiles = P.ntiles;
     pa.hist = reinterpret_cast<uint32_t *>(ws + P.o_hist);
     pa.tile_start = reinterpret_cast<unsigned long long *>(ws + P.o_tile_start);
