@@ -12,10 +12,14 @@ import os
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libdsx.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # dtype codes
 F32, F64, I8, U8, I16, I32, I64 = 1, 2, 3, 4, 5, 6, 7
+U32, BOOL = 8, 9
+# shade kinds and transforms
+SHADE_CMAP, SHADE_COLOR, SHADE_CATEGORICAL, SHADE_COLOR_KEY = 0, 1, 2, 3
+HOW_CODES = {"linear": 0, "log": 1, "cbrt": 2, "eq_hist": 3}
 # fill modes
 FILL_ZERO32, FILL_ONES32, FILL_ZERO64, FILL_ONES64 = 1, 2, 3, 4
 # finalize modes
@@ -25,7 +29,7 @@ EXPORTS = ("dsx_points_workspace_size", "dsx_points_agg", "dsx_lines_agg",
            "dsx_bounds_workspace_size", "dsx_bounds", "dsx_fill", "dsx_finalize",
            "dsx_grid_index_workspace_size", "dsx_grid_index_build", "dsx_gather_rows",
            "dsx_points_indexed_workspace_size", "dsx_points_agg_indexed",
-           "dsx_abi_version", "dsx_strerror", "dsx_last_hip_error")
+           "dsx_shade_workspace_size", "dsx_shade", "dsx_abi_version", "dsx_strerror", "dsx_last_hip_error")
 
 vp = ctypes.c_void_p
 
@@ -96,6 +100,19 @@ class GatherDesc(ctypes.Structure):
                 ("src", vp * 4), ("dst", vp * 4), ("items", vp), ("nitems", ctypes.c_int64)]
 
 
+class ShadeDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("dtype", ctypes.c_int32),
+                ("H", ctypes.c_int64), ("W", ctypes.c_int64), ("ncat", ctypes.c_int64),
+                ("data", vp), ("out", vp), ("how", ctypes.c_int32), ("ncolors", ctypes.c_int32),
+                ("colors", vp), ("keys", vp),
+                ("alpha", ctypes.c_double), ("min_alpha", ctypes.c_double),
+                ("nalpha", ctypes.c_int32), ("has_span", ctypes.c_int32),
+                ("span_lo", ctypes.c_double), ("span_hi", ctypes.c_double),
+                ("span_delta", ctypes.c_double),
+                ("rescale_discrete_levels", ctypes.c_int32), ("has_baseline", ctypes.c_int32),
+                ("baseline", ctypes.c_double)]
+
+
 class DsxError(RuntimeError):
     """A libdsx call returned a non-zero status."""
 
@@ -132,6 +149,9 @@ def load():
         lib.dsx_points_indexed_workspace_size.restype = sz
         lib.dsx_points_agg_indexed.argtypes = [P(PointsDesc), P(View), P(Planes), vp,
                                                P(GridRef), vp, sz, vp]
+        lib.dsx_shade_workspace_size.argtypes = [P(ShadeDesc)]
+        lib.dsx_shade_workspace_size.restype = sz
+        lib.dsx_shade.argtypes = [P(ShadeDesc), vp, sz, vp]
         lib.dsx_abi_version.argtypes = []
         lib.dsx_strerror.argtypes = [ctypes.c_int]
         lib.dsx_strerror.restype = ctypes.c_char_p
@@ -139,7 +159,7 @@ def load():
         lib.dsx_last_hip_error.restype = ctypes.c_char_p
         for f in ("dsx_points_agg", "dsx_lines_agg", "dsx_bounds", "dsx_fill", "dsx_finalize",
                   "dsx_grid_index_build", "dsx_gather_rows", "dsx_points_agg_indexed",
-                  "dsx_abi_version"):
+                  "dsx_shade", "dsx_abi_version"):
             getattr(lib, f).restype = ctypes.c_int
         if lib.dsx_abi_version() != ABI_VERSION:
             raise RuntimeError("libdsx.so ABI version mismatch; rebuild the library")

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define DSX_ABI_VERSION 3
+#define DSX_ABI_VERSION 4
 
 /* ---- status codes --------------------------------------------------------------- */
 typedef enum {
@@ -266,6 +266,55 @@ typedef struct {
 } dsx_gather_desc;
 
 int dsx_gather_rows(const dsx_gather_desc *g, void *stream);
+
+/* ---- shade: aggregate planes -> RGBA images ---------------------------------------- *
+ * Replaces transfer_functions/__init__.py shade (616-747) for device-resident planes:
+ *   DSX_SHADE_CMAP       _interpolate (251-359) with a list colormap: colour channels
+ *                        interpolated over linspace(span, ncolors), alpha constant
+ *   DSX_SHADE_COLOR      _interpolate with one colour: alpha interpolated over
+ *                        linspace(span, nalpha) onto arange(min_alpha, alpha + 1)
+ *   DSX_SHADE_CATEGORICAL _colorize (360-463) + _interpolate_alpha (466-538): an
+ *                        [H, W, ncat] plane, colours mixed by value, alpha from the total
+ *   DSX_SHADE_COLOR_KEY  _apply_discrete_colorkey (541-613): a 2D plane of category values
+ * how: eq_hist (148-223), log1p, cube root or linear (225-228).  `out` receives H*W pixels
+ * r | g << 8 | b << 16 | a << 24 (the reference's uint32 view of RGBA bytes).
+ * Every scalar the reference derives from the data (offset, span, histogram, cdf) is
+ * computed on the device: a call launches a fixed kernel sequence with no host
+ * synchronisation.  Data dtypes: DSX_U32, DSX_BOOL (u8 0/1), DSX_I32, DSX_I64, DSX_F32,
+ * DSX_F64 (integers exact up to 2^53).                                                  */
+#define DSX_U32 8
+#define DSX_BOOL 9
+
+typedef enum {
+    DSX_SHADE_CMAP = 0, DSX_SHADE_COLOR = 1, DSX_SHADE_CATEGORICAL = 2, DSX_SHADE_COLOR_KEY = 3
+} dsx_shade_kind;
+
+typedef enum {
+    DSX_HOW_LINEAR = 0, DSX_HOW_LOG = 1, DSX_HOW_CBRT = 2, DSX_HOW_EQ_HIST = 3
+} dsx_how;
+
+typedef struct {
+    int32_t kind;            /* dsx_shade_kind */
+    int32_t dtype;           /* dtype code of `data` */
+    int64_t H, W, ncat;      /* ncat: categories (CATEGORICAL), else 1 */
+    const void *data;        /* device, H*W*ncat, C order */
+    uint32_t *out;           /* device, H*W pixels */
+    int32_t how;             /* dsx_how */
+    int32_t ncolors;         /* rows of `colors` (and of `keys` for COLOR_KEY) */
+    const uint8_t *colors;   /* device, ncolors x (r, g, b) */
+    const double *keys;      /* device, COLOR_KEY: the category value of each colour */
+    double alpha, min_alpha; /* as given */
+    int32_t nalpha;          /* COLOR: len(arange(min_alpha, alpha + 1)) */
+    int32_t has_span;        /* span given: clip + fixed normalisation */
+    double span_lo, span_hi; /* span as given */
+    double span_delta;       /* span[1] - span[0] as the caller evaluated it */
+    int32_t rescale_discrete_levels;
+    int32_t has_baseline;    /* CATEGORICAL: color_baseline given; COLOR_KEY: likewise */
+    double baseline;
+} dsx_shade_desc;
+
+size_t dsx_shade_workspace_size(const dsx_shade_desc *d);
+int dsx_shade(const dsx_shade_desc *d, void *workspace, size_t workspace_bytes, void *stream);
 
 /* ---- misc --------------------------------------------------------------------------- */
 int dsx_abi_version(void);

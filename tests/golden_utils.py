@@ -257,3 +257,76 @@ def _merge(kind, a, b):
     if kind == "max":
         return np.fmax(a, b)
     return np.fmin(a, b)
+
+
+# ---- tf.shade cases (golden/shade_cases.json) ---------------------------------------------
+_SHADE = None
+
+
+def shade_cases():
+    global _SHADE
+    if _SHADE is None:
+        with open(os.path.join(HERE, "golden", "shade_cases.json")) as f:
+            _SHADE = json.load(f)["cases"]
+    return _SHADE
+
+
+def _colour(c):
+    return tuple(c["tuple"]) if isinstance(c, dict) else c
+
+
+def shade_case(case):
+    """(plane, shade kwargs, expected uint32 image) of one golden shade case; colours are
+    decoded to names / RGB tuples and dict colour keys to dicts, as the test passed them."""
+    spec = case["data"]
+    vals = [np.nan if v is None else v for v in spec["values"]]
+    data = np.array(vals, dtype=spec["dtype"]).reshape(spec["shape"])
+    kw = dict(case["kwargs"])
+    if "cmap" in kw:
+        cm = kw["cmap"]
+        kw["cmap"] = [_colour(c) for c in cm] if isinstance(cm, list) else _colour(cm)
+    if "color_key" in kw:
+        ck = kw["color_key"]
+        kw["color_key"] = ({k: _colour(v) for k, v in ck["dict"]} if isinstance(ck, dict)
+                           else [_colour(c) for c in ck])
+    expected = np.array(case["expected"], dtype=np.uint32).reshape(data.shape[:2])
+    return data, kw, expected
+
+
+def shade_oracle(data, kw):
+    """The oracle's image for shade(plane, **kw) (names resolved with the reference's
+    colour table as datashader_amd.colors restates it)."""
+    from datashader_amd.colors import Sets1to3, rgb
+    from oracle import shade as osh
+    kw = dict(kw)
+    how = kw.pop("how", "eq_hist")
+    alpha = kw.pop("alpha", 255)
+    min_alpha = kw.pop("min_alpha", 40)
+    span = kw.pop("span", None)
+    rescale = kw.pop("rescale_discrete_levels", False) and how == "eq_hist"
+    baseline = kw.pop("color_baseline", None)
+    if data.ndim == 3:
+        ck = kw.pop("color_key", Sets1to3)
+        ncat = data.shape[2]
+        cols = list(ck.values())[:ncat] if isinstance(ck, dict) else list(ck)[:ncat]
+        return osh.colorize(data, [rgb(c) for c in cols], how=how, alpha=alpha, span=span,
+                            min_alpha=min_alpha, color_baseline=baseline,
+                            rescale_discrete_levels=rescale)
+    ck = kw.pop("color_key", None)
+    if isinstance(ck, dict):
+        return osh.discrete(data, list(ck.keys()), [rgb(c) for c in ck.values()], alpha=alpha,
+                            color_baseline=baseline)
+    cmap = kw.pop("cmap", ["lightblue", "darkblue"])
+    if isinstance(cmap, (str, tuple)):
+        return osh.interpolate(data, rgb(cmap), how=how, alpha=alpha, span=span,
+                               min_alpha=min_alpha, rescale_discrete_levels=rescale,
+                               single=True)
+    return osh.interpolate(data, [rgb(c) for c in cmap], how=how, alpha=alpha, span=span,
+                           min_alpha=min_alpha, rescale_discrete_levels=rescale)
+
+
+def channels_close(a, b, tol):
+    """Per-channel |a - b| <= tol for two uint32 RGBA images."""
+    x = np.asarray(a, dtype=np.uint32).view(np.uint8).astype(int)
+    y = np.asarray(b, dtype=np.uint32).view(np.uint8).astype(int)
+    return x.shape == y.shape and bool(np.all(np.abs(x - y) <= tol))
