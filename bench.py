@@ -9,7 +9,7 @@ Prints ONE JSON line: throughput in Gpoints/s, the aggregation kernel's achieved
 bandwidth (algorithmic bytes per launch / average launch time from HIP events on the
 kernel's stream) against the 8 TB/s peak, and the CPU oracle timed on the host's cores.
 
-Other workloads: --workload points800 | points_skewed | bycat | lines  (see --help).
+Other workloads: --workload points800 | points_skewed | bycat | lines | serving | shade.
 """
 from __future__ import annotations
 
@@ -298,13 +298,103 @@ def run_serving(args, torch):
     print(json.dumps(out), flush=True)
 
 
+def run_shade(args, torch):
+    """tf.shade of the headline's aggregates on the device (eq_hist of the 1920x1080 count
+    plane, linear / log of the mean plane) and of a 1024x1024x16 categorical count plane;
+    the numpy oracle on the host copies is the CPU baseline.  Prints one JSON line."""
+    import numpy as np
+
+    import datashader_amd as dsa
+    import datashader_amd.transfer_functions as tf
+    from datashader_amd import _device as dev
+    from datashader_amd import synthetic
+
+    n = int(100_000_000 * args.scale)
+    x, y, v = synthetic.points_skewed_dev(n, seed=2)
+    frame = dsa.DeviceFrame({"x": x, "y": y, "v": v})
+    xr = dev.maybe_expand_bounds(dev.bounds(x))
+    yr = dev.maybe_expand_bounds(dev.bounds(y))
+    cvs = dsa.Canvas(1920, 1080, x_range=xr, y_range=yr)
+    aggs = cvs.points(frame, "x", "y", dsa.summary(count=dsa.count(), mean=dsa.mean("v")))
+    count, mean = aggs["count"], aggs["mean"]
+    nc = int(50_000_000 * args.scale)
+    cx, cy, cc = synthetic.points_ridges_dev(nc, seed=3)
+    cats = [f"c{k}" for k in range(16)]
+    cframe = dsa.DeviceFrame({"x": cx, "y": cy, "cat": cc}, categories={"cat": cats})
+    ccvs = dsa.Canvas(1024, 1024, x_range=(0, 1), y_range=(0, 1))
+    byc = ccvs.points(cframe, "x", "y", dsa.by("cat", dsa.count()))
+    del x, y, v, frame, cx, cy, cc, cframe
+    torch.cuda.synchronize()
+    cases = [("count eq_hist", count, {}, 4), ("mean linear", mean, {"how": "linear"}, 8),
+             ("mean log", mean, {"how": "log"}, 8), ("by-cat eq_hist", byc, {}, 64)]
+    results = {}
+    for label, agg, kw, in_bytes in cases:
+        for _ in range(args.warmup):
+            tf.shade(agg, **kw)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(torch.cuda.current_stream())
+        for _ in range(args.steps):
+            img = tf.shade(agg, **kw)
+        e1.record(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        px = agg.shape[0] * agg.shape[1]
+        ms = (t1 - t0) * 1e3 / args.steps
+        dev_ms = e0.elapsed_time(e1) / args.steps
+        entry = {"pixels": px, "ms_per_call": ms, "device_ms_per_call": dev_ms,
+                 "mpix_per_s": px / (ms * 1e-3) / 1e6,
+                 "algorithmic_bytes": px * (in_bytes + 4),
+                 "achieved_gbs": px * (in_bytes + 4) / (dev_ms * 1e-3) / 1e9,
+                 "checksum": int(img.values.astype(np.uint64).sum())}
+        if not args.no_cpu:
+            from datashader_amd.colors import Sets1to3, rgb
+            from oracle import shade as osh
+            host = agg.values
+            how = kw.get("how", "eq_hist")
+            times = []
+            for _ in range(3):
+                c0 = time.perf_counter()
+                if host.ndim == 3:
+                    osh.colorize(host, [rgb(c) for c in Sets1to3[:host.shape[2]]], how=how)
+                else:
+                    osh.interpolate(host, [rgb("lightblue"), rgb("darkblue")], how=how)
+                times.append(time.perf_counter() - c0)
+            entry["cpu_ms_per_call"] = statistics.median(times) * 1e3
+        results[label] = entry
+        _log(f"[bench] shade {label}: {ms:.3f} ms/call (device {dev_ms:.3f} ms)")
+    head = results["count eq_hist"]
+    out = {
+        "metric": "Mpixels/s shaded (tf.shade on the device)", "value": head["mpix_per_s"],
+        "unit": "Mpixels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_call"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded, generated on the device)",
+        "config": {"workload": "shade: eq_hist of the 1920x1080 count plane of 100M skewed "
+                               "points (lightblue-darkblue); also the mean plane (linear, log) "
+                               "and a 1024x1024x16 by-category count plane (Sets1to3)"},
+        "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": head["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": None, "kernel_ms": head["device_ms_per_call"],
+                     "algorithmic_bytes": head["algorithmic_bytes"]},
+        "cpu_baseline": None if args.no_cpu else {
+            "value": head["pixels"] / (head["cpu_ms_per_call"] * 1e-3) / 1e6,
+            "unit": "Mpixels/s", "cores": 1, "kind": "port",
+            "sample": "the same planes shaded by the numpy oracle on the host, median of 3"},
+        "cases": results,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="points_skewed",
-                    choices=["points_skewed", "points800", "bycat", "lines", "serving"])
+                    choices=["points_skewed", "points800", "bycat", "lines", "serving",
+                             "shade"])
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload size")
     ap.add_argument("--cpu-sample", type=int, default=20_000_000,
                     help="rows (or line samples) of the CPU oracle baseline sample")
@@ -329,6 +419,9 @@ def main():
     torch.cuda.set_device(0)
     if args.workload == "serving":
         run_serving(args, torch)
+        return
+    if args.workload == "shade":
+        run_shade(args, torch)
         return
     _log(f"[bench] building workload {args.workload} (scale {args.scale})")
     step, rows, algo, cfg, cpu, check = build_workload(args.workload, torch, args.scale)

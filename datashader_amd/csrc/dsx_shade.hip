@@ -21,11 +21,13 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kMaxParts = 1024;   // blocks of a reduction pass (one partial each)
+constexpr int kPlanThreads = 256;   // the single workgroup of a plan kernel
 constexpr int kBins = 65536;      // eq_hist's nbins default (transfer_functions:148)
 constexpr int kHalfBins = 32768;  // bins one histogram workgroup holds in LDS (128 KB)
-constexpr int kCdfThreads = 1024;
-constexpr int kBinsPerThread = kBins / kCdfThreads;
+constexpr int kCdfThreads = 1024;             // bins per cdf workgroup
+constexpr int kCdfBlocks = kBins / kCdfThreads;  // 64
 constexpr int kHistCells = 8192;  // cells per histogram workgroup
+constexpr int kMaxKnots = 4096;   // colours of a list colormap / alpha steps
 
 enum Kind : int32_t { K_INT = 0, K_F32 = 1, K_F64 = 2 };
 
@@ -68,13 +70,19 @@ struct Ws {
     Part *parts;
     uint32_t *hist;
     double *xp, *cdf;
+    int32_t *rank;                   // position of a non-empty bin among the non-empty bins
+    double *eslope;                  // np.interp slopes between consecutive cdf points
+    double *kx, *ks;                 // colour knots and the slopes between them (3 / knot)
+    unsigned long long *chunk_sum;   // per 1024-bin chunk: values
+    uint32_t *chunk_nz;              // per 1024-bin chunk: non-empty bins
     double *val;  // H*W working plane (offset values, then transformed values)
 };
 
 __host__ __device__ inline size_t al256(size_t x) { return (x + 255) / 256 * 256; }
 
 struct Layout {
-    size_t o_params, o_parts, o_hist, o_xp, o_cdf, o_val, total;
+    size_t o_params, o_parts, o_hist, o_xp, o_cdf, o_rank, o_chunk, o_eslope, o_kx, o_ks, o_val,
+        total;
 };
 
 inline Layout layout_of(int64_t pixels) {
@@ -85,6 +93,11 @@ inline Layout layout_of(int64_t pixels) {
     L.o_hist = o; o = al256(o + 4 * (size_t)kBins);
     L.o_xp = o; o = al256(o + 8 * (size_t)kBins);
     L.o_cdf = o; o = al256(o + 8 * (size_t)kBins);
+    L.o_rank = o; o = al256(o + 4 * (size_t)kBins);
+    L.o_chunk = o; o = al256(o + 12 * (size_t)kCdfBlocks);
+    L.o_eslope = o; o = al256(o + 8 * (size_t)kBins);
+    L.o_kx = o; o = al256(o + 8 * (size_t)kMaxKnots);
+    L.o_ks = o; o = al256(o + 24 * (size_t)kMaxKnots);
     L.o_val = o; o = al256(o + 8 * (size_t)pixels);
     L.total = o;
     return L;
@@ -158,13 +171,13 @@ __device__ void block_part(Part p, Part *out) {
 
 // All partials of the previous pass merged (one block, every thread gets the result).
 __device__ Part merge_parts(const Part *parts, int nparts) {
-    __shared__ Part sp[kThreads];
+    __shared__ Part sp[kPlanThreads];
     Part p;
     part_init(p);
-    for (int i = threadIdx.x; i < nparts; i += kThreads) part_merge(p, parts[i]);
+    for (int i = threadIdx.x; i < nparts; i += kPlanThreads) part_merge(p, parts[i]);
     sp[threadIdx.x] = p;
     __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
+    for (int s = kPlanThreads / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) part_merge(sp[threadIdx.x], sp[threadIdx.x + s]);
         __syncthreads();
     }
@@ -198,6 +211,17 @@ __device__ __forceinline__ double knot(const Params &P, int i) {
 __device__ __forceinline__ double interp_between(double x, double xj, double xj1, double fj,
                                                  double fj1) {
     const double slope = (fj1 - fj) / (xj1 - xj);
+    double r = slope * (x - xj) + fj;
+    if (r != r) {
+        r = slope * (x - xj1) + fj1;
+        if (r != r && fj == fj1) r = fj;
+    }
+    return r;
+}
+
+// The same with numpy's precomputed slope of the interval.
+__device__ __forceinline__ double interp_slope(double x, double xj, double xj1, double fj,
+                                               double fj1, double slope) {
     double r = slope * (x - xj) + fj;
     if (r != r) {
         r = slope * (x - xj1) + fj1;
@@ -306,7 +330,7 @@ __global__ __launch_bounds__(kThreads) void raw_stats_kernel(const T *data, int6
 
 // Plan A: the offset (nanmin of the unmasked values, or span[0] cast to the dtype) and
 // the clip bounds.
-__global__ __launch_bounds__(kThreads) void plan_raw_kernel(Ws w, int nparts, int kind,
+__global__ __launch_bounds__(kPlanThreads) void plan_raw_kernel(Ws w, int nparts, int kind,
                                                             dsx_shade_desc d) {
     const Part p = merge_parts(w.parts, nparts);
     if (threadIdx.x != 0) return;
@@ -351,7 +375,7 @@ __global__ __launch_bounds__(kThreads) void offset_kernel(const T *data, int64_t
 }
 
 // Plan B: stats of the offset values and eq_hist's bins (transfer_functions:191-217).
-__global__ __launch_bounds__(kThreads) void plan_val_kernel(Ws w, int nparts, int how) {
+__global__ __launch_bounds__(kPlanThreads) void plan_val_kernel(Ws w, int nparts, int how) {
     const Part p = merge_parts(w.parts, nparts);
     if (threadIdx.x != 0) return;
     Params &P = *w.P;
@@ -415,60 +439,100 @@ __global__ __launch_bounds__(1024) void hist_kernel(int64_t n, Ws w) {
     }
 }
 
-// cumsum / total of the histogram (eq_hist: `cdf = hist.cumsum(); cdf /= cdf[-1]`);
-// the float path keeps only the non-empty bins and their centres.
+// Inclusive scan over a 1024-thread workgroup (wave scans, then the 16 wave totals).
+template <typename T>
+__device__ __forceinline__ T block_scan(T v, T *wave_tot, T &total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        const T y = __shfl_up(v, off, 64);
+        if (lane >= off) v += y;
+    }
+    if (lane == 63) wave_tot[wid] = v;
+    __syncthreads();
+    if (wid == 0) {
+        T t = lane < 16 ? wave_tot[lane] : (T)0;
+        for (int off = 1; off < 16; off <<= 1) {
+            const T y = __shfl_up(t, off, 64);
+            if (lane >= off) t += y;
+        }
+        if (lane < 16) wave_tot[lane] = t;
+    }
+    __syncthreads();
+    const T pre = wid ? wave_tot[wid - 1] : (T)0;
+    total = wave_tot[15];
+    __syncthreads();
+    return v + pre;
+}
+
+// Per 1024-bin chunk of the histogram: number of values and of non-empty bins.
+__global__ __launch_bounds__(kCdfThreads) void chunk_kernel(Ws w) {
+    __shared__ unsigned long long ws[16];
+    __shared__ uint32_t wn[16];
+    const Params &P = *w.P;
+    if (P.vn == 0 || P.nb == 0) return;
+    const int k = blockIdx.x * kCdfThreads + threadIdx.x;
+    const uint32_t c = k < P.nb ? w.hist[k] : 0u;
+    unsigned long long st;
+    uint32_t nt;
+    block_scan<unsigned long long>(c, ws, st);
+    block_scan<uint32_t>(c != 0u, wn, nt);
+    if (threadIdx.x == 0) { w.chunk_sum[blockIdx.x] = st; w.chunk_nz[blockIdx.x] = nt; }
+}
+
+// cumsum / total of the histogram (eq_hist: `cdf = hist.cumsum(); cdf /= cdf[-1]`); the
+// float path keeps only the non-empty bins, their centres and each one's rank.
 __global__ __launch_bounds__(kCdfThreads) void cdf_kernel(Ws w) {
-    __shared__ unsigned long long ss[kCdfThreads];
-    __shared__ uint32_t sn[kCdfThreads];
+    __shared__ unsigned long long ws[16];
+    __shared__ uint32_t wn[16];
+    __shared__ unsigned long long pre_s, tot_s;
+    __shared__ uint32_t pre_n, tot_n;
     Params &P = *w.P;
     const int nb = P.nb;
     if (P.vn == 0 || nb == 0) return;
-    const int t = threadIdx.x;
-    const int k0 = t * kBinsPerThread;
-    const int k1 = k0 + kBinsPerThread < nb ? k0 + kBinsPerThread : nb;
-    unsigned long long s = 0;
-    uint32_t nz = 0;
-    for (int k = k0; k < k1; ++k) {
-        const uint32_t c = w.hist[k];
-        s += c;
-        nz += c != 0u;
+    const int b = blockIdx.x;
+    if (threadIdx.x == 0) {
+        unsigned long long ps = 0, ts = 0;
+        uint32_t pn = 0, tn = 0;
+        for (int i = 0; i < kCdfBlocks; ++i) {
+            if (i < b) { ps += w.chunk_sum[i]; pn += w.chunk_nz[i]; }
+            ts += w.chunk_sum[i];
+            tn += w.chunk_nz[i];
+        }
+        pre_s = ps; tot_s = ts; pre_n = pn; tot_n = tn;
     }
-    ss[t] = s;
-    sn[t] = nz;
     __syncthreads();
-    for (int off = 1; off < kCdfThreads; off <<= 1) {  // inclusive scan
-        unsigned long long a = 0;
-        uint32_t b = 0;
-        if (t >= off) { a = ss[t - off]; b = sn[t - off]; }
-        __syncthreads();
-        ss[t] += a;
-        sn[t] += b;
-        __syncthreads();
-    }
-    const unsigned long long total = ss[kCdfThreads - 1];
-    const uint32_t levels = sn[kCdfThreads - 1];
-    unsigned long long cum = ss[t] - s;
-    uint32_t pos = sn[t] - nz;
-    const double dt = (double)total;
-    for (int k = k0; k < k1; ++k) {
-        const uint32_t c = w.hist[k];
-        cum += c;
+    const int k = b * kCdfThreads + threadIdx.x;
+    const uint32_t c = k < nb ? w.hist[k] : 0u;
+    unsigned long long st;
+    uint32_t nt;
+    const unsigned long long cum = pre_s + block_scan<unsigned long long>(c, ws, st);
+    const uint32_t pos = pre_n + block_scan<uint32_t>(c != 0u, wn, nt);  // inclusive
+    const double dt = (double)tot_s;
+    if (k < nb) {
         if (P.int_path) {
             w.cdf[k] = (double)cum / dt;
         } else if (c) {
             double centre;
             if (P.kind == K_F32) centre = (double)(((float)edge(P, k) + (float)edge(P, k + 1)) / 2.0f);
             else centre = (edge(P, k) + edge(P, k + 1)) / 2.0;
-            w.xp[pos] = centre;
-            w.cdf[pos] = (double)cum / dt;
-            ++pos;
+            w.xp[pos - 1] = centre;
+            w.cdf[pos - 1] = (double)cum / dt;
+            w.rank[k] = (int32_t)pos - 1;
         }
     }
-    if (t == 0) {
-        P.levels = (int32_t)levels;
-        P.L = P.int_path ? nb : (int32_t)levels;
+    if (b == 0 && threadIdx.x == 0) {
+        P.levels = (int32_t)tot_n;
+        P.L = P.int_path ? nb : (int32_t)tot_n;
         P.total = dt;
     }
+}
+
+// Slopes between consecutive (centre, cdf) points, as np.interp precomputes them.
+__global__ __launch_bounds__(kThreads) void eslope_kernel(Ws w) {
+    const Params &P = *w.P;
+    if (P.vn == 0 || P.int_path || P.nb == 0) return;
+    const int j = blockIdx.x * kThreads + threadIdx.x;
+    if (j + 1 < P.L) w.eslope[j] = (w.cdf[j + 1] - w.cdf[j]) / (w.xp[j + 1] - w.xp[j]);
 }
 
 // Pass 3: the transformed values (in place) and their min / max.
@@ -486,9 +550,17 @@ __global__ __launch_bounds__(kThreads) void transform_kernel(int64_t n, Ws w, in
             if (P.int_path) {
                 t = cdf[(int64_t)(x - P.first)];
             } else {
+                // x lies in its own (non-empty) bin k: the largest centre <= x is bin k's
+                // or the previous non-empty bin's -- np.interp's search result in O(1).
                 const int L = P.L;
-                t = interp1(x, L, [&](int k) { return xp[k]; }, [&](int k) { return cdf[k]; },
-                            cdf[0], cdf[L - 1]);
+                const int r = w.rank[bin_of(P, x)];
+                int j = x >= xp[r] ? r : r - 1;
+                if (!((j < 0 || xp[j] <= x) && (j + 1 >= L || xp[j + 1] > x)))
+                    j = search(x, L, [&](int q) { return xp[q]; });
+                if (L == 1 || j < 0) t = cdf[0];
+                else if (j >= L - 1) t = cdf[L - 1];
+                else if (xp[j] == x) t = cdf[j];
+                else t = interp_slope(x, xp[j], xp[j + 1], cdf[j], cdf[j + 1], w.eslope[j]);
             }
         } else {
             t = transform(x, how, P.kind);
@@ -501,57 +573,90 @@ __global__ __launch_bounds__(kThreads) void transform_kernel(int64_t n, Ws w, in
 
 // Plan C: the span of the transformed values (or of the transformed given span) and
 // the knots np.linspace(span[0], span[1], nk) (transfer_functions:293-311, 318-332).
-__global__ __launch_bounds__(kThreads) void plan_span_kernel(Ws w, int nparts, dsx_shade_desc d,
+__global__ __launch_bounds__(kPlanThreads) void plan_span_kernel(Ws w, int nparts, dsx_shade_desc d,
                                                              int nk, int knots_f64) {
     const Part p = merge_parts(w.parts, nparts);
-    if (threadIdx.x != 0) return;
-    Params &P = *w.P;
-    P.tmin = p.mn; P.tmax = p.mx;
-    double s0, s1;
-    int f32;
-    if (d.has_span) {
-        s0 = transform(0.0, d.how, K_F64);
-        s1 = transform(d.span_delta, d.how, K_F64);
-        f32 = 0;
-    } else {
-        s0 = p.mn;
-        s1 = p.mx;
-        // nanmin / nanmax of a float32 plane are float32 scalars: linspace stays float32
-        f32 = !knots_f64 && P.kind == K_F32 && d.how != DSX_HOW_EQ_HIST;
-        if (d.rescale_discrete_levels && d.how == DSX_HOW_EQ_HIST) {
-            // _rescale_discrete_levels (transfer_functions:236-248)
-            const double m = -0.5 / 98.0;
-            const double c = 1.5 - 2.0 * m;
-            const double multiple = m * (double)P.levels + c;
-            if (multiple > 1.0) {
-                const double lower = s1 - multiple * (s1 - s0);
-                s0 = lower > 0.0 ? lower : 0.0;
-                s1 = 1.0;
+    __shared__ Params sP;
+    if (threadIdx.x == 0) {
+        Params &P = sP;
+        P = *w.P;
+        P.tmin = p.mn; P.tmax = p.mx;
+        double s0, s1;
+        int f32;
+        if (d.has_span) {
+            s0 = transform(0.0, d.how, K_F64);
+            s1 = transform(d.span_delta, d.how, K_F64);
+            f32 = 0;
+        } else {
+            s0 = p.mn;
+            s1 = p.mx;
+            // nanmin / nanmax of a float32 plane are float32 scalars: linspace stays float32
+            f32 = !knots_f64 && P.kind == K_F32 && d.how != DSX_HOW_EQ_HIST;
+            if (d.rescale_discrete_levels && d.how == DSX_HOW_EQ_HIST) {
+                // _rescale_discrete_levels (transfer_functions:236-248)
+                const double m = -0.5 / 98.0;
+                const double c = 1.5 - 2.0 * m;
+                const double multiple = m * (double)P.levels + c;
+                if (multiple > 1.0) {
+                    const double lower = s1 - multiple * (s1 - s0);
+                    s0 = lower > 0.0 ? lower : 0.0;
+                    s1 = 1.0;
+                }
             }
         }
-    }
-    P.kf32 = f32;
-    P.nk = nk;
-    if (f32) {
-        const float a = (float)s0, b = (float)s1;
-        const float delta = b - a;
-        P.kstart = a; P.kstop = b; P.kdelta = delta;
-        if (nk > 1) {
-            const float step = delta / (float)(nk - 1);
-            P.kstep = step;
-            P.kmode = step == 0.0f ? 1 : 0;
+        P.kf32 = f32;
+        P.nk = nk;
+        if (f32) {
+            const float a = (float)s0, b = (float)s1;
+            const float delta = b - a;
+            P.kstart = a; P.kstop = b; P.kdelta = delta;
+            if (nk > 1) {
+                const float step = delta / (float)(nk - 1);
+                P.kstep = step;
+                P.kmode = step == 0.0f ? 1 : 0;
+            } else {
+                P.kmode = 2;
+            }
         } else {
-            P.kmode = 2;
+            const double delta = s1 - s0;
+            P.kstart = s0; P.kstop = s1; P.kdelta = delta;
+            if (nk > 1) {
+                const double step = delta / (double)(nk - 1);
+                P.kstep = step;
+                P.kmode = step == 0.0 ? 1 : 0;
+            } else {
+                P.kmode = 2;
+            }
         }
-    } else {
-        const double delta = s1 - s0;
-        P.kstart = s0; P.kstop = s1; P.kdelta = delta;
-        if (nk > 1) {
-            const double step = delta / (double)(nk - 1);
-            P.kstep = step;
-            P.kmode = step == 0.0 ? 1 : 0;
-        } else {
-            P.kmode = 2;
+        *w.P = P;
+    }
+    __syncthreads();
+    const Params &Q = sP;
+    // knots and the slopes np.interp precomputes: colour channels (list colormap), the
+    // alpha steps (one colour) or the two-point alpha ramp (categorical)
+    if (d.kind == DSX_SHADE_CATEGORICAL) {
+        if (threadIdx.x == 0) {
+            w.kx[0] = Q.kstart;
+            w.kx[1] = Q.kstop;
+            w.ks[0] = (d.alpha - d.min_alpha) / (Q.kstop - Q.kstart);
+        }
+        return;
+    }
+    const double d1 = (d.min_alpha + 1.0) - d.min_alpha;
+    auto fa = [&](int k) {
+        return k == 0 ? d.min_alpha : (k == 1 ? d.min_alpha + 1.0 : d.min_alpha + (double)k * d1);
+    };
+    for (int k = threadIdx.x; k < Q.nk; k += kPlanThreads) {
+        const double xk = knot(Q, k);
+        w.kx[k] = xk;
+        if (k + 1 < Q.nk) {
+            const double dx = knot(Q, k + 1) - xk;
+            if (d.kind == DSX_SHADE_CMAP) {
+                for (int c = 0; c < 3; ++c)
+                    w.ks[3 * k + c] = ((double)d.colors[3 * (k + 1) + c] - (double)d.colors[3 * k + c]) / dx;
+            } else {
+                w.ks[k] = (fa(k + 1) - fa(k)) / dx;
+            }
         }
     }
 }
@@ -563,22 +668,19 @@ __device__ __forceinline__ uint32_t pack(uint32_t r, uint32_t g, uint32_t b, uin
 // Pass 4 (list colormap): r, g, b = interp(t, knots, channel, left=255), alpha constant.
 __global__ __launch_bounds__(kThreads) void cmap_kernel(int64_t n, Ws w, const uint8_t *colors,
                                                         double alpha, uint32_t *out) {
-    const Params P = *w.P;
-    const int nk = P.nk;
+    const int nk = w.P->nk;
+    const int all_masked = w.P->all_masked;
+    const double *kx = w.kx, *ks = w.ks;
     const uint32_t a_const = u8_of(alpha);
-    auto xp = [&](int k) { return knot(P, k); };
+    auto xp = [&](int k) { return kx[k]; };
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads) {
-        if (P.all_masked) { out[i] = 0u; continue; }
+        if (all_masked) { out[i] = 0u; continue; }
         const double t = w.val[i];
         uint32_t ch[3];
-        if (nk == 1) {
-            const double x0 = xp(0);
-            for (int c = 0; c < 3; ++c) {
-                const double f = colors[c];
-                const double r = t < x0 ? 255.0 : (t > x0 ? f : f);
-                ch[c] = u8_of(r);
-            }
+        if (nk == 1) {  // numpy's single-point branch: NaN -> the colour
+            const double x0 = kx[0];
+            for (int c = 0; c < 3; ++c) ch[c] = t < x0 ? 255u : colors[c];
         } else if (t != t) {
             ch[0] = ch[1] = ch[2] = 0u;  // nan_to_num(NaN) = 0
         } else {
@@ -588,14 +690,15 @@ __global__ __launch_bounds__(kThreads) void cmap_kernel(int64_t n, Ws w, const u
             } else if (j >= nk - 1) {
                 for (int c = 0; c < 3; ++c) ch[c] = colors[3 * (nk - 1) + c];
             } else {
-                const double xj = xp(j);
+                const double xj = kx[j];
                 if (xj == t) {
                     for (int c = 0; c < 3; ++c) ch[c] = colors[3 * j + c];
                 } else {
-                    const double xj1 = xp(j + 1);
+                    const double xj1 = kx[j + 1];
                     for (int c = 0; c < 3; ++c)
-                        ch[c] = u8_of(interp_between(t, xj, xj1, (double)colors[3 * j + c],
-                                                     (double)colors[3 * (j + 1) + c]));
+                        ch[c] = u8_of(interp_slope(t, xj, xj1, (double)colors[3 * j + c],
+                                                   (double)colors[3 * (j + 1) + c],
+                                                   ks[3 * j + c]));
                 }
             }
         }
@@ -607,19 +710,32 @@ __global__ __launch_bounds__(kThreads) void cmap_kernel(int64_t n, Ws w, const u
 // left=0, right=255).
 __global__ __launch_bounds__(kThreads) void color_kernel(int64_t n, Ws w, const uint8_t *colors,
                                                          double min_alpha, uint32_t *out) {
-    const Params P = *w.P;
+    const int nk = w.P->nk;
+    const int all_masked = w.P->all_masked;
+    const double *kx = w.kx, *ks = w.ks;
     const uint32_t rgb = pack(colors[0], colors[1], colors[2], 0u);
     // np.arange(min_alpha, alpha + 1): start, start + 1, then start + k * (second - start)
     const double d1 = (min_alpha + 1.0) - min_alpha;
     auto fp = [&](int k) {
         return k == 0 ? min_alpha : (k == 1 ? min_alpha + 1.0 : min_alpha + (double)k * d1);
     };
-    auto xp = [&](int k) { return knot(P, k); };
+    auto xp = [&](int k) { return kx[k]; };
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads) {
-        if (P.all_masked) { out[i] = 0u; continue; }
+        if (all_masked) { out[i] = 0u; continue; }
         const double t = w.val[i];
-        double a = interp1(t, P.nk, xp, fp, 0.0, 255.0);
+        double a;
+        if (nk == 1) {
+            a = t < kx[0] ? 0.0 : (t > kx[0] ? 255.0 : fp(0));
+        } else if (t != t) {
+            a = 0.0;
+        } else {
+            const int j = search(t, nk, xp);
+            if (j < 0) a = 0.0;
+            else if (j >= nk) a = 255.0;
+            else if (j == nk - 1 || kx[j] == t) a = fp(j);
+            else a = interp_slope(t, kx[j], kx[j + 1], fp(j), fp(j + 1), ks[j]);
+        }
         if (a != a) a = 0.0;
         out[i] = rgb | (u8_of(a) << 24);
     }
@@ -646,7 +762,7 @@ __global__ __launch_bounds__(kThreads) void cat_min_kernel(const T *data, int64_
     block_part(p, w.parts);
 }
 
-__global__ __launch_bounds__(kThreads) void plan_baseline_kernel(Ws w, int nparts,
+__global__ __launch_bounds__(kPlanThreads) void plan_baseline_kernel(Ws w, int nparts,
                                                                  dsx_shade_desc d, int kind) {
     const Part p = merge_parts(w.parts, nparts);
     if (threadIdx.x != 0) return;
@@ -739,7 +855,7 @@ __global__ __launch_bounds__(kThreads) void cat_pixel_kernel(const T *data, int6
 
 // Plan for the alpha of a categorical plane (_interpolate_alpha, transfer_functions:
 // 474-518): offset, the masking of zero totals of unsigned planes, clip bounds.
-__global__ __launch_bounds__(kThreads) void plan_total_kernel(Ws w, int nparts, dsx_shade_desc d,
+__global__ __launch_bounds__(kPlanThreads) void plan_total_kernel(Ws w, int nparts, dsx_shade_desc d,
                                                               int data_kind, int unsigned_data) {
     const Part p = merge_parts(w.parts, nparts);
     if (threadIdx.x != 0) return;
@@ -796,13 +912,17 @@ __global__ __launch_bounds__(kThreads) void total_offset_kernel(int64_t npix, Ws
 // Alpha of a categorical pixel: interp(a_scaled, norm_span, [min_alpha, alpha], 0, 255).
 __global__ __launch_bounds__(kThreads) void cat_alpha_kernel(int64_t npix, Ws w, double min_alpha,
                                                              double alpha, uint32_t *out) {
-    const Params P = *w.P;
-    auto xp = [&](int k) { return k == 0 ? P.kstart : P.kstop; };
-    auto fp = [&](int k) { return k == 0 ? min_alpha : alpha; };
+    const double x0 = w.kx[0], x1 = w.kx[1], slope = w.ks[0];
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < npix;
          i += (int64_t)gridDim.x * kThreads) {
         const double t = w.val[i];
-        double a = interp1(t, 2, xp, fp, 0.0, 255.0);
+        double a;
+        if (t != t) a = 0.0;
+        else if (t > x1) a = 255.0;
+        else if (t < x0) a = 0.0;
+        else if (t >= x1) a = alpha;        // search result j == 1 (the last point)
+        else if (t == x0) a = min_alpha;
+        else a = interp_slope(t, x0, x1, min_alpha, alpha, slope);
         if (a != a) a = 0.0;
         out[i] = (out[i] & 0x00FFFFFFu) | (u8_of(a) << 24);
     }
@@ -849,10 +969,12 @@ int check(const dsx_shade_desc *d) {
     if (d->how < DSX_HOW_LINEAR || d->how > DSX_HOW_EQ_HIST) return DSX_ERR_MODE;
     switch (d->kind) {
     case DSX_SHADE_CMAP:
-        if (d->ncat != 1 || d->ncolors < 1 || !d->colors) return d->colors ? DSX_ERR_SHAPE : DSX_ERR_NULL;
+        if (!d->colors) return DSX_ERR_NULL;
+        if (d->ncat != 1 || d->ncolors < 1 || d->ncolors > kMaxKnots) return DSX_ERR_SHAPE;
         break;
     case DSX_SHADE_COLOR:
-        if (d->ncat != 1 || d->ncolors != 1 || d->nalpha < 1) return DSX_ERR_SHAPE;
+        if (d->ncat != 1 || d->ncolors != 1 || d->nalpha < 1 || d->nalpha > kMaxKnots)
+            return DSX_ERR_SHAPE;
         if (!d->colors) return DSX_ERR_NULL;
         break;
     case DSX_SHADE_CATEGORICAL:
@@ -877,7 +999,7 @@ int data_kind(int dtype) {
 int tail(const dsx_shade_desc *d, Ws w, int64_t npix, int nk, int knots_f64, hipStream_t st) {
     const int g = grid_parts(npix);
     int s;
-    hipLaunchKernelGGL(plan_val_kernel, dim3(1), dim3(kThreads), 0, st, w, g, d->how);
+    hipLaunchKernelGGL(plan_val_kernel, dim3(1), dim3(kPlanThreads), 0, st, w, g, d->how);
     if ((s = launch_status())) return s;
     if (d->how == DSX_HOW_EQ_HIST) {
         hipLaunchKernelGGL(zero_hist_kernel, dim3(kBins / kThreads), dim3(kThreads), 0, st, w);
@@ -885,12 +1007,16 @@ int tail(const dsx_shade_desc *d, Ws w, int64_t npix, int nk, int knots_f64, hip
         const int64_t hb = (npix + kHistCells - 1) / kHistCells;
         hipLaunchKernelGGL(hist_kernel, dim3((unsigned)hb, 2), dim3(1024), 0, st, npix, w);
         if ((s = launch_status())) return s;
-        hipLaunchKernelGGL(cdf_kernel, dim3(1), dim3(kCdfThreads), 0, st, w);
+        hipLaunchKernelGGL(chunk_kernel, dim3(kCdfBlocks), dim3(kCdfThreads), 0, st, w);
+        if ((s = launch_status())) return s;
+        hipLaunchKernelGGL(cdf_kernel, dim3(kCdfBlocks), dim3(kCdfThreads), 0, st, w);
+        if ((s = launch_status())) return s;
+        hipLaunchKernelGGL(eslope_kernel, dim3(kBins / kThreads), dim3(kThreads), 0, st, w);
         if ((s = launch_status())) return s;
     }
     hipLaunchKernelGGL(transform_kernel, dim3(g), dim3(kThreads), 0, st, npix, w, d->how);
     if ((s = launch_status())) return s;
-    hipLaunchKernelGGL(plan_span_kernel, dim3(1), dim3(kThreads), 0, st, w, g, *d, nk, knots_f64);
+    hipLaunchKernelGGL(plan_span_kernel, dim3(1), dim3(kPlanThreads), 0, st, w, g, *d, nk, knots_f64);
     return launch_status();
 }
 
@@ -902,7 +1028,7 @@ int run2d(const dsx_shade_desc *d, Ws w, hipStream_t st) {
     int s;
     hipLaunchKernelGGL((raw_stats_kernel<T, BOOLEAN>), dim3(g), dim3(kThreads), 0, st, data, n, w);
     if ((s = launch_status())) return s;
-    hipLaunchKernelGGL(plan_raw_kernel, dim3(1), dim3(kThreads), 0, st, w, g,
+    hipLaunchKernelGGL(plan_raw_kernel, dim3(1), dim3(kPlanThreads), 0, st, w, g,
                        data_kind(d->dtype), *d);
     if ((s = launch_status())) return s;
     hipLaunchKernelGGL((offset_kernel<T, BOOLEAN>), dim3(g), dim3(kThreads), 0, st, data, n, w,
@@ -928,13 +1054,13 @@ int run3d(const dsx_shade_desc *d, Ws w, hipStream_t st) {
     const int gn = grid_parts(n), gp = grid_parts(npix);
     hipLaunchKernelGGL((cat_min_kernel<T>), dim3(gn), dim3(kThreads), 0, st, data, n, w);
     if ((s = launch_status())) return s;
-    hipLaunchKernelGGL(plan_baseline_kernel, dim3(1), dim3(kThreads), 0, st, w, gn, *d,
+    hipLaunchKernelGGL(plan_baseline_kernel, dim3(1), dim3(kPlanThreads), 0, st, w, gn, *d,
                        data_kind(d->dtype));
     if ((s = launch_status())) return s;
     hipLaunchKernelGGL((cat_pixel_kernel<T>), dim3(gp), dim3(kThreads), 0, st, data, npix,
                        (int)d->ncat, d->colors, d->has_baseline, w, d->out);
     if ((s = launch_status())) return s;
-    hipLaunchKernelGGL(plan_total_kernel, dim3(1), dim3(kThreads), 0, st, w, gp, *d,
+    hipLaunchKernelGGL(plan_total_kernel, dim3(1), dim3(kPlanThreads), 0, st, w, gp, *d,
                        data_kind(d->dtype), (int)(d->dtype == DSX_U32));
     if ((s = launch_status())) return s;
     hipLaunchKernelGGL(total_offset_kernel, dim3(gp), dim3(kThreads), 0, st, npix, w, d->has_span);
@@ -997,6 +1123,12 @@ extern "C" int dsx_shade(const dsx_shade_desc *d, void *workspace, size_t worksp
     w.hist = reinterpret_cast<uint32_t *>(ws + L.o_hist);
     w.xp = reinterpret_cast<double *>(ws + L.o_xp);
     w.cdf = reinterpret_cast<double *>(ws + L.o_cdf);
+    w.rank = reinterpret_cast<int32_t *>(ws + L.o_rank);
+    w.chunk_sum = reinterpret_cast<unsigned long long *>(ws + L.o_chunk);
+    w.chunk_nz = reinterpret_cast<uint32_t *>(ws + L.o_chunk + 8 * kCdfBlocks);
+    w.eslope = reinterpret_cast<double *>(ws + L.o_eslope);
+    w.kx = reinterpret_cast<double *>(ws + L.o_kx);
+    w.ks = reinterpret_cast<double *>(ws + L.o_ks);
     w.val = reinterpret_cast<double *>(ws + L.o_val);
     if (d->kind == DSX_SHADE_CATEGORICAL) {
         switch (d->dtype) {

@@ -193,6 +193,8 @@ def _interpolate(agg, cmap, how, alpha, span, min_alpha, name, rescale_discrete_
         colors = [rgb(c) for c in cmap]
         if not colors:
             raise ValueError("not enough values to unpack (expected 3, got 0)")
+        if len(colors) > 4096:
+            raise ValueError("the device colormap takes at most 4096 colours")
         kind = _abi.SHADE_CMAP
     elif isinstance(cmap, (str, tuple)):
         colors = [rgb(cmap)]
