@@ -9,7 +9,8 @@ Prints ONE JSON line: throughput in Gpoints/s, the aggregation kernel's achieved
 bandwidth (algorithmic bytes per launch / average launch time from HIP events on the
 kernel's stream) against the 8 TB/s peak, and the CPU oracle timed on the host's cores.
 
-Other workloads: --workload points800 | points_skewed | bycat | lines | serving | shade.
+Other workloads: --workload points800 | points_skewed | bycat | lines | serving | shade |
+stream.
 """
 from __future__ import annotations
 
@@ -100,6 +101,51 @@ def build_workload(name, torch, scale=1.0):
         def check(res):
             c = res["count"].data
             return int(c.sum().item())
+
+        return step, n, algo, cfg, cpu, check
+
+    if name == "stream":
+        import pandas as pd
+        n = int(400_000_000 * scale)
+        W, H = 1920, 1080
+        chunk = 1 << 25
+        x, y, v = synthetic.points_skewed_dev(n, seed=2)
+        xr = dev.maybe_expand_bounds(dev.bounds(x))
+        yr = dev.maybe_expand_bounds(dev.bounds(y))
+        host = {"x": x.cpu().numpy(), "y": y.cpu().numpy(), "v": v.cpu().numpy()}
+        del x, y, v
+        torch.cuda.empty_cache()
+        source = dsa.ChunkedFrame(pd.DataFrame(host), chunk_rows=chunk)
+        canvas = dsa.Canvas(W, H, x_range=xr, y_range=yr)
+        agg = dsa.summary(count=dsa.count(), mean=dsa.mean("v"))
+
+        def step():
+            return canvas.points(source, "x", "y", agg)
+
+        nchunks = -(-n // chunk)
+        algo = 12 * n // nchunks + (4 + 8) * W * H  # per aggregation launch (one chunk)
+        cfg = {"workload": "points 1920x1080 count+mean streamed from host memory "
+                           "(ChunkedFrame, 2^25-row chunks, PCIe included): 400M float32 "
+                           "rows (90% in a 0.01-sigma Gaussian, 10% uniform), value "
+                           "lognormal(0,2)", "rows": n, "chunk_rows": chunk,
+               "canvas": [W, H], "aggs": ["count()", "mean('v')"]}
+
+        def cpu(sample_rows):
+            import oracle
+            m = min(sample_rows, n)
+            xs, ys, vs = (host[k][:m].astype(np.float64) for k in ("x", "y", "v"))
+            sx, tx = canvas.x_axis.compute_scale_and_translate(xr, W)
+            sy, ty = canvas.y_axis.compute_scale_and_translate(yr, H)
+            view = oracle.View(sx, tx, sy, ty, xr[0], xr[1], yr[0], yr[1])
+            times = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                oracle.points(xs, ys, view, W, H, ("count", "countv", "sumz"), val=vs)
+                times.append(time.perf_counter() - t0)
+            return m, statistics.median(times)
+
+        def check(res):
+            return int(res["count"].values.sum(dtype=np.int64))
 
         return step, n, algo, cfg, cpu, check
 
@@ -394,7 +440,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="points_skewed",
                     choices=["points_skewed", "points800", "bycat", "lines", "serving",
-                             "shade"])
+                             "shade", "stream"])
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload size")
     ap.add_argument("--cpu-sample", type=int, default=20_000_000,
                     help="rows (or line samples) of the CPU oracle baseline sample")

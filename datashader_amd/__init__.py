@@ -17,4 +17,5 @@ from .reductions import (any, by, category_binning, category_codes,  # noqa: F40
                          category_modulo, count, count_cat, max, mean, min, std, sum,
                          summary, var)
 from . import backend as _backend  # noqa: F401  (registers the pipelines)
+from .streaming import ChunkedFrame  # noqa: F401  (registers the chunked pipeline)
 from .xrlite import DataArray, Dataset  # noqa: F401

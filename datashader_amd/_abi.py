@@ -29,7 +29,7 @@ EXPORTS = ("dsx_points_workspace_size", "dsx_points_agg", "dsx_lines_agg",
            "dsx_bounds_workspace_size", "dsx_bounds", "dsx_fill", "dsx_finalize",
            "dsx_grid_index_workspace_size", "dsx_grid_index_build", "dsx_gather_rows",
            "dsx_points_indexed_workspace_size", "dsx_points_agg_indexed",
-           "dsx_shade_workspace_size", "dsx_shade", "dsx_abi_version", "dsx_strerror", "dsx_last_hip_error")
+           "dsx_shade_workspace_size", "dsx_shade", "dsx_combine", "dsx_abi_version", "dsx_strerror", "dsx_last_hip_error")
 
 vp = ctypes.c_void_p
 
@@ -100,6 +100,11 @@ class GatherDesc(ctypes.Structure):
                 ("src", vp * 4), ("dst", vp * 4), ("items", vp), ("nitems", ctypes.c_int64)]
 
 
+class CombineDesc(ctypes.Structure):
+    _fields_ = [("ncells", ctypes.c_int64), ("key_bytes", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("into", Planes), ("part", Planes)]
+
+
 class ShadeDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("dtype", ctypes.c_int32),
                 ("H", ctypes.c_int64), ("W", ctypes.c_int64), ("ncat", ctypes.c_int64),
@@ -149,6 +154,7 @@ def load():
         lib.dsx_points_indexed_workspace_size.restype = sz
         lib.dsx_points_agg_indexed.argtypes = [P(PointsDesc), P(View), P(Planes), vp,
                                                P(GridRef), vp, sz, vp]
+        lib.dsx_combine.argtypes = [P(CombineDesc), vp]
         lib.dsx_shade_workspace_size.argtypes = [P(ShadeDesc)]
         lib.dsx_shade_workspace_size.restype = sz
         lib.dsx_shade.argtypes = [P(ShadeDesc), vp, sz, vp]
@@ -159,7 +165,7 @@ def load():
         lib.dsx_last_hip_error.restype = ctypes.c_char_p
         for f in ("dsx_points_agg", "dsx_lines_agg", "dsx_bounds", "dsx_fill", "dsx_finalize",
                   "dsx_grid_index_build", "dsx_gather_rows", "dsx_points_agg_indexed",
-                  "dsx_shade", "dsx_abi_version"):
+                  "dsx_shade", "dsx_combine", "dsx_abi_version"):
             getattr(lib, f).restype = ctypes.c_int
         if lib.dsx_abi_version() != ABI_VERSION:
             raise RuntimeError("libdsx.so ABI version mismatch; rebuild the library")

@@ -194,7 +194,61 @@ def _points_call(lib, d, vw, planes_struct, mean, device, stream, grid=None):
                "dsx_points_agg")
 
 
-def run_pass(lib, frame, schema, glyph, view, p, H, W):
+def _pass_desc(frame, schema, glyph, view, p, device, catc):
+    """The kernel descriptor of one pass over ``frame``: PointsDesc or LinesDesc, plus
+    the (kept-alive) tensors it points to."""
+    t = dev.torch()
+    val = dev.as_float_column(frame[p.val]) if p.val is not None else None
+    vdt = (_abi.F32 if val.dtype == t.float32 else _abi.F64) if val is not None else 0
+    if isinstance(glyph, Point):
+        x = dev.as_float_column(frame[glyph.x])
+        y = dev.as_float_column(frame[glyph.y])
+        if x.dtype != y.dtype:  # x and y share one dtype in the kernel
+            x, y = x.to(t.float64), y.to(t.float64)
+        d = _abi.PointsDesc()
+        d.n = len(frame)
+        d.x, d.y = dev.ptr(x), dev.ptr(y)
+        d.xy_dtype = dev.dtype_code(x)
+        d.val, d.val_dtype = dev.ptr(val), vdt
+        d.cat = catc
+        return d, (x, y, val)
+    lay = glyph.vertex_layout(frame)
+    d = _abi.LinesDesc()
+    d.nlines, d.nverts = lay["nlines"], lay["nverts"]
+    d.x, d.y = dev.ptr(lay["x"]), dev.ptr(lay["y"])
+    d.x_dtype, d.y_dtype = dev.dtype_code(lay["x"]), dev.dtype_code(lay["y"])
+    d.x_line_stride, d.x_vert_stride = lay["xls"], lay["xvs"]
+    d.y_line_stride, d.y_vert_stride = lay["yls"], lay["yvs"]
+    d.axis = lay["axis"]
+    d.plot_start = 1
+    d.val, d.val_dtype = dev.ptr(val), vdt
+    d.cat = catc
+    return d, (lay, val)
+
+
+def m2_pass(lib, frame, schema, glyph, view, p, ncells, mean, grid=None):
+    """var / std: a second pass that accumulates every row's squared deviation from its
+    cell's mean (``mean``: float64 plane); returns the m2 plane."""
+    device = frame.device
+    catc, catcol, _ = _categorizer(p.cat, frame, schema)
+    d, keep = _pass_desc(frame, schema, glyph, view, p, device, catc)
+    vw = _view_struct(view)
+    stream = dev.stream_handle()
+    m2 = _alloc(ncells, "f64", device)
+    m2s = _abi.Planes()
+    m2s.sum = dev.ptr(m2)
+    if isinstance(glyph, Point):
+        _points_call(lib, d, vw, m2s, mean, device, stream, grid)
+    else:
+        _fill(lib, m2, _abi.FILL_ZERO64)
+        d.mean = dev.ptr(mean)
+        _abi.check(lib.dsx_lines_agg(d, vw, m2s, stream), "dsx_lines_agg")
+    return m2
+
+
+def run_pass(lib, frame, schema, glyph, view, p, H, W, with_m2=True):
+    """One kernel pass over ``frame``: its planes (and, for var / std, the m2 plane
+    against the pass's own means unless ``with_m2`` is False)."""
     t = dev.torch()
     device = frame.device
     catc, catcol, ncat = _categorizer(p.cat, frame, schema)
@@ -202,7 +256,6 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W):
     val = dev.as_float_column(frame[p.val]) if p.val is not None else None
     f32 = val is not None and val.dtype == t.float32
     keyb = 4 if f32 else 8
-    vdt = (_abi.F32 if f32 else _abi.F64) if val is not None else 0
     vw = _view_struct(view)
     stream = dev.stream_handle()
     points = isinstance(glyph, Point)
@@ -223,17 +276,9 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W):
             if not points:
                 _fill(lib, planes[name], init)
             setattr(ps, name, dev.ptr(planes[name]))
+    d, keep = _pass_desc(frame, schema, glyph, view, p, device, catc)
+    grid = None
     if points:
-        x = dev.as_float_column(frame[glyph.x])
-        y = dev.as_float_column(frame[glyph.y])
-        if x.dtype != y.dtype:  # x and y share one dtype in the kernel
-            x, y = x.to(t.float64), y.to(t.float64)
-        d = _abi.PointsDesc()
-        d.n = len(frame)
-        d.x, d.y = dev.ptr(x), dev.ptr(y)
-        d.xy_dtype = dev.dtype_code(x)
-        d.val, d.val_dtype = dev.ptr(val), vdt
-        d.cat = catc
         grid = _grid_ref(frame, glyph, p, view)
         if grid is None and getattr(frame, "persistent", False):
             key = (glyph.x, glyph.y, p.val, None if p.cat is None else p.cat._key(),
@@ -242,36 +287,12 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W):
             win, reuse = _window_slot(frame, key, device)
             d.window, d.reuse_window = dev.ptr(win), reuse
         _points_call(lib, d, vw, ps, None, device, stream, grid)
-        if p.m2:
-            mean = t.empty(ncells, dtype=t.float64, device=device)
-            _finalize(lib, _abi.FIN_MEAN, keyb, ncells, planes, mean)
-            planes["m2"] = _alloc(ncells, "f64", device)
-            m2s = _abi.Planes()
-            m2s.sum = dev.ptr(planes["m2"])
-            _points_call(lib, d, vw, m2s, mean, device, stream, grid)
     else:
-        lay = glyph.vertex_layout(frame)
-        d = _abi.LinesDesc()
-        d.nlines, d.nverts = lay["nlines"], lay["nverts"]
-        d.x, d.y = dev.ptr(lay["x"]), dev.ptr(lay["y"])
-        d.x_dtype, d.y_dtype = dev.dtype_code(lay["x"]), dev.dtype_code(lay["y"])
-        d.x_line_stride, d.x_vert_stride = lay["xls"], lay["xvs"]
-        d.y_line_stride, d.y_vert_stride = lay["yls"], lay["yvs"]
-        d.axis = lay["axis"]
-        d.plot_start = 1
-        d.val, d.val_dtype = dev.ptr(val), vdt
-        d.cat = catc
         _abi.check(_timed(lambda: lib.dsx_lines_agg(d, vw, ps, stream)), "dsx_lines_agg")
-        if p.m2:
-            # var / std: squared deviations of every pixel's value from the cell mean
-            mean = t.empty(ncells, dtype=t.float64, device=device)
-            _finalize(lib, _abi.FIN_MEAN, keyb, ncells, planes, mean)
-            planes["m2"] = _alloc(ncells, "f64", device)
-            _fill(lib, planes["m2"], _abi.FILL_ZERO64)
-            m2s = _abi.Planes()
-            m2s.sum = dev.ptr(planes["m2"])
-            d.mean = dev.ptr(mean)
-            _abi.check(lib.dsx_lines_agg(d, vw, m2s, stream), "dsx_lines_agg")
+    if p.m2 and with_m2:
+        mean = t.empty(ncells, dtype=t.float64, device=device)
+        _finalize(lib, _abi.FIN_MEAN, keyb, ncells, planes, mean)
+        planes["m2"] = m2_pass(lib, frame, schema, glyph, view, p, ncells, mean, grid)
     return PassResult(ncat, keyb, planes)
 
 
@@ -388,6 +409,12 @@ def pipeline(source, schema, canvas, glyph, agg, *, antialias=False, to_host=Non
         frame = source
         to_host = False if to_host is None else to_host
     res, view, x_range, y_range = aggregate(frame, schema, canvas, glyph, agg)
+    return wrap_results(agg, res, canvas, glyph, view, x_range, y_range, schema, to_host)
+
+
+def wrap_results(agg, res, canvas, glyph, view, x_range, y_range, schema, to_host):
+    """The DataArray (or, for ``summary``, Dataset) of finalized device tensors
+    (``pandas.py:61-66``, ``compiler.py:510-536``)."""
     if isinstance(agg, rd.summary):
         data = {name: _wrap(red, t_, canvas, glyph, view, x_range, y_range, schema, to_host,
                             name=name) for name, red, t_ in res}

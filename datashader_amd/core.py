@@ -255,10 +255,11 @@ def bypixel(source, canvas, glyph, agg, *, antialias=False):
         if len(cols) < len(source.columns):
             source = source[cols]
         schema = Schema.from_pandas(source)
-    elif isinstance(source, DeviceFrame):
+    elif isinstance(source, DeviceFrame) or getattr(source, "streams_in_chunks", False):
         schema = source.schema
     else:
-        raise ValueError("source must be a pandas DataFrame or a datashader_amd.DeviceFrame")
+        raise ValueError("source must be a pandas DataFrame, a datashader_amd.DeviceFrame "
+                         "or a datashader_amd.ChunkedFrame")
     glyph.validate(schema)
     agg.validate(schema)
     canvas.validate()

@@ -211,6 +211,46 @@ extern "C" int dsx_finalize(const dsx_finalize_desc *f, void *stream) {
     return launch_status();
 }
 
+// Partition combine (compiler.py:478-507 with the bases' _combine): counts and float64
+// sums add, min / max keys take the unsigned min / max (empty cells hold the identity).
+template <typename K>
+__global__ __launch_bounds__(256) void combine_kernel(dsx_combine_desc c) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.ncells; i += stride) {
+        if (c.into.count) c.into.count[i] += c.part.count[i];
+        if (c.into.countv) c.into.countv[i] += c.part.countv[i];
+        if (c.into.sum) c.into.sum[i] += c.part.sum[i];
+        if (c.into.min) {
+            K *a = reinterpret_cast<K *>(c.into.min);
+            const K b = reinterpret_cast<const K *>(c.part.min)[i];
+            if (b < a[i]) a[i] = b;
+        }
+        if (c.into.max) {
+            K *a = reinterpret_cast<K *>(c.into.max);
+            const K b = reinterpret_cast<const K *>(c.part.max)[i];
+            if (b > a[i]) a[i] = b;
+        }
+    }
+}
+
+extern "C" int dsx_combine(const dsx_combine_desc *c, void *stream) {
+    if (!c) return DSX_ERR_NULL;
+    if (c->ncells < 0) return DSX_ERR_SHAPE;
+    if (c->key_bytes != 4 && c->key_bytes != 8) return DSX_ERR_DTYPE;
+    // the same planes must be present in `into` and `part`
+    if ((c->into.count == nullptr) != (c->part.count == nullptr) ||
+        (c->into.countv == nullptr) != (c->part.countv == nullptr) ||
+        (c->into.sum == nullptr) != (c->part.sum == nullptr) ||
+        (c->into.min == nullptr) != (c->part.min == nullptr) ||
+        (c->into.max == nullptr) != (c->part.max == nullptr))
+        return DSX_ERR_ACCUM;
+    if (c->ncells == 0) return DSX_OK;
+    const int grid = grid_for(c->ncells, 256, 4);
+    if (c->key_bytes == 4) combine_kernel<uint32_t><<<grid, 256, 0, as_stream(stream)>>>(*c);
+    else combine_kernel<uint64_t><<<grid, 256, 0, as_stream(stream)>>>(*c);
+    return launch_status();
+}
+
 extern "C" int dsx_abi_version(void) { return DSX_ABI_VERSION; }
 
 extern "C" const char *dsx_strerror(int status) {

@@ -187,6 +187,23 @@ typedef enum {
 /* Fill ncells elements with a kernel (no hipMemset, so it is safe under graph capture). */
 int dsx_fill(void *plane, int32_t mode, int64_t ncells, void *stream);
 
+/* ---- partition combine ---------------------------------------------------------------- *
+ * into[p] (+)= part[p] for every plane p present in both (the planes present must be the
+ * same): count / countv / sum add, min / max keys take the unsigned min / max.  Replaces
+ * the combine step of compiler.py:478-507 (count._combine, _sum_zero._combine, min/max
+ * _combine, reductions.py:652-654, 985-996, 1197-1205, 1252-1260) that dask uses to merge
+ * partition aggregates (data_libraries/dask.py:168-217); here it merges the planes of the
+ * chunks of a host-resident table streamed through the GPU.                              */
+typedef struct {
+    int64_t ncells;
+    int32_t key_bytes;            /* width of the min / max keys: 4 or 8                  */
+    int32_t reserved;
+    dsx_planes into;              /* accumulated in place                                 */
+    dsx_planes part;              /* one chunk's planes                                   */
+} dsx_combine_desc;
+
+int dsx_combine(const dsx_combine_desc *c, void *stream);
+
 typedef enum {
     DSX_FIN_ANY = 1,          /* out u8  = count > 0                                      */
     DSX_FIN_SUM = 2,          /* out f64 = countv > 0 ? sum : NaN                         */

@@ -41,7 +41,7 @@ STRUCTS = {
     "dsx_points_desc": _abi.PointsDesc, "dsx_lines_desc": _abi.LinesDesc,
     "dsx_finalize_desc": _abi.FinalizeDesc, "dsx_grid_index_desc": _abi.GridIndexDesc,
     "dsx_gather_desc": _abi.GatherDesc, "dsx_grid_ref": _abi.GridRef,
-    "dsx_shade_desc": _abi.ShadeDesc,
+    "dsx_shade_desc": _abi.ShadeDesc, "dsx_combine_desc": _abi.CombineDesc,
 }
 
 
