@@ -74,12 +74,25 @@ class Pass:
 
 
 def plan(agg):
-    """Group the leaves of ``agg`` into passes; returns (passes, [(name, red, pass)])."""
+    """Group the leaves of ``agg`` into passes; returns (passes, [(name, red, pass)]).
+    Row-index leaves (first, last, where, the row-index selectors) get the key
+    ``("rowindex", i)``: they run their own kernel (``row_index_leaf``), after the passes
+    (a where over max / min adds that plane to the pass of its selector's column)."""
     passes = {}
     leaves = []
-    for name, red in rd.leaves(agg):
+    for i, (name, red) in enumerate(rd.leaves(agg)):
         cat = red.categorizer if isinstance(red, rd.by) else None
         inner = red.reduction if isinstance(red, rd.by) else red
+        if getattr(inner, "row_index", False):
+            sel = inner.selector if isinstance(inner, rd.where) else inner
+            if isinstance(sel, (rd.max, rd.min)):
+                key = (sel.column, cat)
+                p = passes.get(key)
+                if p is None:
+                    p = passes[key] = Pass(sel.column, cat)
+                p.planes.add("max" if isinstance(sel, rd.max) else "min")
+            leaves.append((name, red, ("rowindex", i)))
+            continue
         key = (inner.column, cat)
         p = passes.get(key)
         if p is None:
@@ -88,7 +101,7 @@ def plan(agg):
         p.m2 |= inner.needs_m2
         leaves.append((name, red, key))
     # a pass with no value column rides along with a value pass of the same categorizer
-    for key in [k for k in passes if k[0] is None]:
+    for key in [k for k in passes if k[0] is None and k in passes]:
         host = next((k for k in passes if k[0] is not None and k[1] == key[1]), None)
         if host is not None:
             passes[host].planes.update(passes.pop(key).planes)
@@ -362,6 +375,62 @@ def make_view(canvas, x_range, y_range):
                 xlog=canvas.x_axis.log, ylog=canvas.y_axis.log, x_st=x_st, y_st=y_st)
 
 
+def row_index_leaf(lib, frame, schema, glyph, view, red, passes, results, H, W):
+    """The finalized array of a row-index leaf: the selected row of every cell
+    (dsx_points_rowindex) and, with a lookup column, that row's value (dsx_take)."""
+    t = dev.torch()
+    if not isinstance(glyph, Point):
+        raise NotImplementedError(f"{type(red).__name__} is only supported for points on "
+                                  "the MI355X backend")
+    cat = red.categorizer if isinstance(red, rd.by) else None
+    inner = red.reduction if isinstance(red, rd.by) else red
+    sel = inner.selector if isinstance(inner, rd.where) else inner
+    if isinstance(inner, rd.where):
+        lookup = inner.column
+    elif isinstance(inner, rd._first_or_last):
+        lookup = inner.column
+    else:
+        lookup = None  # the row-index selectors return the index itself
+    device = frame.device
+    catc, catcol, ncat = _categorizer(cat, frame, schema)
+    ncells = H * W * ncat
+    r = _abi.RowIndexDesc()
+    key = None
+    valcol = sel.column
+    if isinstance(sel, (rd.max, rd.min)):
+        r.mode = _abi.RI_ARGMAX if isinstance(sel, rd.max) else _abi.RI_ARGMIN
+        res = results[(sel.column, cat)]
+        key = res.planes["max" if isinstance(sel, rd.max) else "min"]
+        r.key = dev.ptr(key)
+        r.nan_check = 1
+    else:
+        r.mode = sel.ri_mode
+        r.nan_check = 1 if valcol is not None else 0
+    index = t.empty(ncells, dtype=t.int64, device=device)
+    r.index = dev.ptr(index)
+    p = Pass(valcol, cat)
+    d, keep = _pass_desc(frame, schema, glyph, view, p, device, catc)
+    if key is not None and d.val_dtype != (_abi.F32 if res.keyb == 4 else _abi.F64):
+        raise _abi.DsxError("key plane width does not match the value column")
+    _abi.check(lib.dsx_points_rowindex(d, _view_struct(view), r, dev.stream_handle()),
+               "dsx_points_rowindex")
+    shape = (H, W, ncat) if isinstance(red, rd.by) else (H, W)
+    if lookup is None:
+        return index.reshape(shape)
+    col = frame[lookup]
+    if col.dtype == t.bool:
+        col = col.to(t.uint8)
+    tk = _abi.TakeDesc()
+    tk.ncells = ncells
+    tk.index = dev.ptr(index)
+    tk.col = dev.ptr(col)
+    tk.dtype = dev.dtype_code(col)
+    out = t.empty(ncells, dtype=t.float64, device=device)
+    tk.out = dev.ptr(out)
+    _abi.check(lib.dsx_take(tk, dev.stream_handle()), "dsx_take")
+    return out.reshape(shape)
+
+
 def aggregate(frame, schema, canvas, glyph, agg):
     """Run the aggregation on the device; returns (results, view, x_range, y_range)
     where ``results`` is ``[(name, reduction, device tensor)]``."""
@@ -375,8 +444,13 @@ def aggregate(frame, schema, canvas, glyph, agg):
     passes, leaves = plan(agg)
     results = {key: run_pass(lib, frame, schema, glyph, view, p, H, W)
                for key, p in passes.items()}
-    out = [(name, red, finalize_leaf(lib, red, results[key], H, W))
-           for (name, red, key) in leaves]
+    out = []
+    for name, red, key in leaves:
+        if key[0] == "rowindex":
+            arr = row_index_leaf(lib, frame, schema, glyph, view, red, passes, results, H, W)
+        else:
+            arr = finalize_leaf(lib, red, results[key], H, W)
+        out.append((name, red, arr))
     return out, view, x_range, y_range
 
 

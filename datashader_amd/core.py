@@ -239,8 +239,12 @@ def _cols_to_keep(columns, glyph, agg):
             for c in a.columns:
                 if c is not None:
                     keep[c] = True
-        elif a.column is not None:
-            keep[a.column] = True
+            recurse(a.reduction)
+        else:
+            if a.column is not None:
+                keep[a.column] = True
+            if isinstance(a, rd.where):
+                recurse(a.selector)
 
     recurse(agg)
     return [c for c, k in keep.items() if k]

@@ -251,6 +251,38 @@ extern "C" int dsx_combine(const dsx_combine_desc *c, void *stream) {
     return launch_status();
 }
 
+// where()'s lookup: the value of the selected row, NaN where no row was selected.
+template <typename T>
+__global__ __launch_bounds__(256) void take_kernel(dsx_take_desc t) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const T *col = reinterpret_cast<const T *>(t.col);
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < t.ncells; c += stride) {
+        const int64_t i = t.index[c];
+        t.out[c] = i < 0 ? __builtin_nan("") : (double)col[i];
+    }
+}
+
+extern "C" int dsx_take(const dsx_take_desc *t, void *stream) {
+    if (!t || !t->index || !t->out) return DSX_ERR_NULL;
+    if (t->ncells < 0) return DSX_ERR_SHAPE;
+    if (t->ncells == 0) return DSX_OK;
+    if (!t->col) return DSX_ERR_NULL;
+    const int grid = grid_for(t->ncells, 256, 4);
+    hipStream_t s = as_stream(stream);
+    switch (t->dtype) {
+    case DSX_F32: take_kernel<float><<<grid, 256, 0, s>>>(*t); break;
+    case DSX_F64: take_kernel<double><<<grid, 256, 0, s>>>(*t); break;
+    case DSX_I8: take_kernel<int8_t><<<grid, 256, 0, s>>>(*t); break;
+    case DSX_U8: take_kernel<uint8_t><<<grid, 256, 0, s>>>(*t); break;
+    case DSX_I16: take_kernel<int16_t><<<grid, 256, 0, s>>>(*t); break;
+    case DSX_I32: take_kernel<int32_t><<<grid, 256, 0, s>>>(*t); break;
+    case DSX_I64: take_kernel<int64_t><<<grid, 256, 0, s>>>(*t); break;
+    case DSX_U32: take_kernel<uint32_t><<<grid, 256, 0, s>>>(*t); break;
+    default: return DSX_ERR_DTYPE;
+    }
+    return launch_status();
+}
+
 extern "C" int dsx_abi_version(void) { return DSX_ABI_VERSION; }
 
 extern "C" const char *dsx_strerror(int status) {

@@ -1629,6 +1629,86 @@ extern "C" int dsx_points_agg(const dsx_points_desc *d, const dsx_view *v, const
     return dispatch_val<double>(d, a, P, out, mean, ws, s);
 }
 
+// ---- row-index reductions ---------------------------------------------------------------
+// One pass over the rows with a global atomic per candidate row; a plain read of the cell
+// first skips rows that cannot improve it (FIRST / ARG* see rows in increasing order,
+// LAST in decreasing order, so after the first sweep almost every row is skipped).
+
+struct RiArgs {
+    PointArgs a;
+    int32_t mode, nan_check;
+    const void *key;
+    unsigned long long *index;
+};
+
+template <typename TC, typename TV>
+__global__ __launch_bounds__(256) void rowindex_kernel(RiArgs r) {
+    const PointArgs &a = r.a;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
+        const int64_t i = r.mode == DSX_RI_LAST ? a.n - 1 - k : k;
+        int32_t px, py, pc;
+        if (!locate<TC>(a, i, reinterpret_cast<const TC *>(a.x)[i],
+                        reinterpret_cast<const TC *>(a.y)[i], px, py, pc))
+            continue;
+        const int64_t cell = ((int64_t)py * a.W + px) * a.cat.ncat + pc;
+        if (r.nan_check) {
+            const TV v = reinterpret_cast<const TV *>(a.val)[i];
+            if (v != v) continue;
+            if (r.mode == DSX_RI_ARGMAX || r.mode == DSX_RI_ARGMIN) {
+                if constexpr (sizeof(TV) == 4) {
+                    if (fkey32((float)v) != reinterpret_cast<const uint32_t *>(r.key)[cell]) continue;
+                } else {
+                    if (fkey64((double)v) != reinterpret_cast<const uint64_t *>(r.key)[cell]) continue;
+                }
+            }
+        }
+        unsigned long long *slot = r.index + cell;
+        if (r.mode == DSX_RI_LAST) {
+            const long long cur = *reinterpret_cast<volatile long long *>(slot);
+            if ((long long)i > cur) atomicMax(reinterpret_cast<long long *>(slot), (long long)i);
+        } else {
+            // -1 (empty) is the largest unsigned value, so min keeps the smallest row
+            const unsigned long long cur = *reinterpret_cast<volatile unsigned long long *>(slot);
+            if ((unsigned long long)i < cur) atomicMin(slot, (unsigned long long)i);
+        }
+    }
+}
+
+template <typename TC>
+static int rowindex_launch(const RiArgs &r, int val_dtype, hipStream_t st) {
+    const int grid = grid_for(r.a.n < 1 ? 1 : r.a.n, 256, 8);
+    if (val_dtype == DSX_F32)
+        hipLaunchKernelGGL((rowindex_kernel<TC, float>), dim3(grid), dim3(256), 0, st, r);
+    else
+        hipLaunchKernelGGL((rowindex_kernel<TC, double>), dim3(grid), dim3(256), 0, st, r);
+    return launch_status();
+}
+
+extern "C" int dsx_points_rowindex(const dsx_points_desc *d, const dsx_view *v,
+                                   const dsx_rowindex_desc *r, void *stream) {
+    int st = check_points(d, v);
+    if (st) return st;
+    if (!r || !r->index) return DSX_ERR_NULL;
+    if (r->mode < DSX_RI_FIRST || r->mode > DSX_RI_ARGMIN) return DSX_ERR_MODE;
+    const bool arg = r->mode == DSX_RI_ARGMAX || r->mode == DSX_RI_ARGMIN;
+    if ((r->nan_check || arg) && !has_value(d)) return DSX_ERR_ACCUM;
+    if (arg && !r->key) return DSX_ERR_NULL;
+    const int64_t ncells = (int64_t)v->width * v->height * cat_of(d->cat).ncat;
+    hipStream_t s = as_stream(stream);
+    if ((st = fill_async(r->index, DSX_FILL_ONES64, ncells, s))) return st;
+    if (d->n == 0) return DSX_OK;
+    RiArgs ra;
+    ra.a = args_of(d, v);
+    ra.mode = r->mode;
+    ra.nan_check = (r->nan_check || arg) ? 1 : 0;
+    ra.key = r->key;
+    ra.index = reinterpret_cast<unsigned long long *>(r->index);
+    const int vdt = has_value(d) ? d->val_dtype : DSX_F64;
+    if (d->xy_dtype == DSX_F32) return rowindex_launch<float>(ra, vdt, s);
+    return rowindex_launch<double>(ra, vdt, s);
+}
+
 extern "C" size_t dsx_points_indexed_workspace_size(const dsx_points_desc *d, const dsx_view *v,
                                                     const dsx_planes *p, const dsx_grid_ref *g) {
     if (idx_check(d, v, p, nullptr, g)) return 0;

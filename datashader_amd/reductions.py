@@ -249,6 +249,89 @@ class std(Reduction):
     needs_m2 = True
 
 
+class _first_or_last(FloatingReduction):
+    """first / last (reductions.py:1340-1456): the value of the first / last row (in row
+    order) whose value is not NaN; computed as the reference's GPU path does, through
+    the smallest / largest row index of the bin and a lookup of that row."""
+
+    row_index = True
+    ri_mode = None
+
+    def planes(self):
+        return ()
+
+
+class first(_first_or_last):
+    ri_mode = _abi.RI_FIRST
+
+
+class last(_first_or_last):
+    ri_mode = _abi.RI_LAST
+
+
+class _max_or_min_row_index(OptionalFieldReduction):
+    """The largest / smallest row index in each bin, int64, -1 where empty
+    (reductions.py:2236-2372)."""
+
+    row_index = True
+    ri_mode = None
+    out_dtype = np.int64
+
+    def __init__(self):
+        super().__init__(None)
+
+    def planes(self):
+        return ()
+
+    def __repr__(self):
+        return f"{type(self).__name__}()"
+
+
+class _min_row_index(_max_or_min_row_index):
+    ri_mode = _abi.RI_FIRST
+
+
+class _max_row_index(_max_or_min_row_index):
+    ri_mode = _abi.RI_LAST
+
+
+class where(FloatingReduction):
+    """Values of ``lookup_column`` at the row a ``selector`` (first, last, max, min or a
+    row-index selector) picks in each bin; with no lookup column the row index itself
+    (int64, -1 where empty) (reductions.py:1842-2166).  Among rows tied for the max / min
+    the first in row order is picked, as the reference's row-order appends do."""
+
+    row_index = True
+
+    def __init__(self, selector, lookup_column=None):
+        if not isinstance(selector, (first, last, max, min, _max_or_min_row_index)):
+            raise TypeError("selector can only be a first, first_n, last, last_n, "
+                            "max, max_n, min or min_n reduction")
+        super().__init__(lookup_column)
+        self.selector = selector
+        self.columns = (selector.column, lookup_column)
+
+    @property
+    def out_dtype(self):
+        return np.int64 if self.column is None else np.float64
+
+    def planes(self):
+        return ()
+
+    def validate(self, schema):
+        if self.column is not None:
+            super().validate(schema)
+        self.selector.validate(schema)
+        if self.column is not None and self.column == self.selector.column:
+            raise ValueError("where and its contained reduction cannot use the same column")
+
+    def _key(self):
+        return ("where", self.selector._key(), self.column)
+
+    def __repr__(self):
+        return f"where(selector={self.selector!r}, lookup_column={self.column!r})"
+
+
 class by(Reduction):
     """Apply ``reduction`` separately per category (reductions.py:696-823)."""
 

@@ -187,6 +187,42 @@ typedef enum {
 /* Fill ncells elements with a kernel (no hipMemset, so it is safe under graph capture). */
 int dsx_fill(void *plane, int32_t mode, int64_t ncells, void *stream);
 
+/* ---- row-index reductions (first / last / where) ----------------------------------- *
+ * For every cell the row the reference's selector keeps (reductions.py:1340-1456,
+ * 2236-2372, where 1842-2166): FIRST the smallest row index (_min_row_index; first()),
+ * LAST the largest (_max_row_index; last()), ARGMAX / ARGMIN the smallest row index among
+ * the rows whose value equals the cell's max / min (max._append / min._append keep the
+ * first row that reaches the extreme).  Rows are positions 0..n-1 of the points desc;
+ * rows with a NaN value are skipped when nan_check is set (always for ARGMAX / ARGMIN).
+ * The categorizer of the desc splits cells by category as in dsx_points_agg.           */
+typedef enum { DSX_RI_FIRST = 1, DSX_RI_LAST = 2, DSX_RI_ARGMAX = 3, DSX_RI_ARGMIN = 4 }
+    dsx_ri_mode;
+
+typedef struct {
+    int32_t mode;                 /* dsx_ri_mode                                          */
+    int32_t nan_check;            /* FIRST / LAST: skip rows whose value is NaN           */
+    const void *key;              /* ARGMAX / ARGMIN: the max / min key plane of the value
+                                     column as dsx_points_agg wrote it (key width = value
+                                     width)                                               */
+    int64_t *index;               /* out, H*W*ncat: the selected row, -1 where none       */
+} dsx_rowindex_desc;
+
+int dsx_points_rowindex(const dsx_points_desc *d, const dsx_view *v,
+                        const dsx_rowindex_desc *r, void *stream);
+
+/* out[c] = col[index[c]] as float64, NaN where index[c] < 0 (where()'s lookup,
+ * reductions.py:1914-1925).  col dtype: DSX_F32, DSX_F64, DSX_I8..DSX_I64, DSX_U32.     */
+typedef struct {
+    int64_t ncells;
+    const int64_t *index;
+    const void *col;
+    int32_t dtype;
+    int32_t reserved;
+    double *out;
+} dsx_take_desc;
+
+int dsx_take(const dsx_take_desc *t, void *stream);
+
 /* ---- partition combine ---------------------------------------------------------------- *
  * into[p] (+)= part[p] for every plane p present in both (the planes present must be the
  * same): count / countv / sum add, min / max keys take the unsigned min / max.  Replaces

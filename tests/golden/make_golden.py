@@ -169,6 +169,48 @@ def point_cases(frame):
         i32.reshape((2, 2, 5)).sum(axis=2, dtype="f8").T, pick="i32_sum")
     add("test_multiple_aggregates/i32_count", cite, summ, [[5, 5], [5, 5]], pick="i32_count")
 
+    # row-index reductions: first / last / where / _min_row_index / _max_row_index
+    nan = float("nan")
+    lo_rows, hi_rows = [[0, 10], [5, 15]], [[4, 14], [9, 19]]
+    for c in ("i32", "i64", "f32", "f64"):
+        add(f"test_first/first({c})", TP + ":991-996", {"kind": "first", "column": c}, lo_rows)
+        add(f"test_last/last({c})", TP + ":999-1004", {"kind": "last", "column": c}, hi_rows)
+        for sel, rev, rows, cite in (("first", [[20, 10], [15, 5]], lo_rows, ":448-462"),
+                                     ("last", [[16, 6], [11, 1]], hi_rows, ":465-479"),
+                                     ("max", [[16, 6], [11, 1]], hi_rows, ":482-494"),
+                                     ("min", [[20, 10], [15, 5]], lo_rows, ":497-509")):
+            s_ = {"kind": sel, "column": c}
+            add(f"test_where_{sel}/where({sel}({c}),reverse)", TP + cite,
+                {"kind": "where", "selector": s_, "lookup": "reverse"}, rev)
+            add(f"test_where_{sel}/where({sel}({c}))", TP + cite,
+                {"kind": "where", "selector": s_, "lookup": None}, rows, dtype="int64")
+    add("test_min_row_index", TP + ":1029-1033", {"kind": "_min_row_index"}, lo_rows,
+        dtype="int64")
+    add("test_max_row_index", TP + ":1036-1040", {"kind": "_max_row_index"}, hi_rows,
+        dtype="int64")
+    add("test_where_min_row_index", TP + ":404-406",
+        {"kind": "where", "selector": {"kind": "_min_row_index"}, "lookup": "plusminus"},
+        [[0, 10], [-5, -15]])
+    add("test_where_max_row_index", TP + ":409-411",
+        {"kind": "where", "selector": {"kind": "_max_row_index"}, "lookup": "plusminus"},
+        [[4, 14], [-9, -19]])
+    add("test_categorical_min_row_index", TP + ":316-319",
+        {"kind": "by", "cat": "cat2", "reduction": {"kind": "_min_row_index"}},
+        [[[0, 1, 2, 3], [12, 13, 10, 11]], [[8, 5, 6, 7], [16, 17, 18, 15]]],
+        cat_coords=["a", "b", "c", "d"], cat_dim="cat2", dtype="int64")
+    add("test_categorical_max_row_index", TP + ":322-325",
+        {"kind": "by", "cat": "cat2", "reduction": {"kind": "_max_row_index"}},
+        [[[4, 1, 2, 3], [12, 13, 14, 11]], [[8, 9, 6, 7], [16, 17, 18, 19]]],
+        cat_coords=["a", "b", "c", "d"], cat_dim="cat2", dtype="int64")
+    add("test_categorical_first", TP + ":356-363",
+        {"kind": "by", "cat": "cat2", "reduction": {"kind": "first", "column": "plusminus"}},
+        [[[0, -1, nan, -3], [12, -13, 10, -11]], [[8, -5, 6, -7], [16, -17, 18, -15]]],
+        cat_coords=["a", "b", "c", "d"], cat_dim="cat2")
+    add("test_categorical_last", TP + ":366-373",
+        {"kind": "by", "cat": "cat2", "reduction": {"kind": "last", "column": "plusminus"}},
+        [[[4, -1, nan, -3], [12, -13, 14, -11]], [[8, -9, 6, -7], [16, -17, 18, -19]]],
+        cat_coords=["a", "b", "c", "d"], cat_dim="cat2")
+
     # test_auto_range_points, test_pandas.py:1080-1129 (integer x, y; autorange)
     frames = {}
     for n, mult, extra, cite in ((10, 1, 0, ":1080-1092"), (10, 1, 1, ":1094-1101"),

@@ -113,8 +113,19 @@ def _leaf(spec, run):
     return _leaf_inner(spec, run, None, 1)
 
 
+_ROW_MODES = {"first": "first", "last": "last", "_min_row_index": "first",
+              "_max_row_index": "last", "max": "max", "min": "min"}
+
+
 def _leaf_inner(spec, run, codes, ncat):
     kind, col = spec["kind"], spec.get("column")
+    if kind in ("first", "last", "_min_row_index", "_max_row_index", "where"):
+        if kind == "where":
+            sel, lookup = spec["selector"], spec["lookup"]
+        else:
+            sel, lookup = spec, spec.get("column")
+        return run.row_index(_ROW_MODES[sel["kind"]], sel.get("column"), lookup, codes, ncat,
+                             codes is None)
     val = None if col is None else run.column(col)
     squeeze = codes is None
     if kind == "count":
@@ -152,6 +163,18 @@ class PointRun:
 
     def cat_codes(self, cat):
         return _cat_codes(self.df, cat)
+
+    def row_index(self, mode, selcol, lookup, codes, ncat, squeeze):
+        """first / last / where / row-index selectors with the sequential oracle."""
+        from oracle import closed_forms as cf
+        from oracle import rowindex as ri
+        bins = cf.point_bins(self.df[self.x].to_numpy(np.float64),
+                             self.df[self.y].to_numpy(np.float64), self.view, self.W, self.H)
+        cell = ri.cells(bins, codes, ncat)
+        val = None if selcol is None else self.column(selcol)
+        idx = ri.select_rows(cell, self.W * self.H * ncat, mode, val)
+        out = idx if lookup is None else ri.take(idx, self.column(lookup))
+        return out.reshape((self.H, self.W) if squeeze else (self.H, self.W, ncat))
 
     def __call__(self, bases, val, codes, ncat, squeeze):
         return oracle.points(self.df[self.x].to_numpy(np.float64),

@@ -42,6 +42,7 @@ STRUCTS = {
     "dsx_finalize_desc": _abi.FinalizeDesc, "dsx_grid_index_desc": _abi.GridIndexDesc,
     "dsx_gather_desc": _abi.GatherDesc, "dsx_grid_ref": _abi.GridRef,
     "dsx_shade_desc": _abi.ShadeDesc, "dsx_combine_desc": _abi.CombineDesc,
+    "dsx_rowindex_desc": _abi.RowIndexDesc, "dsx_take_desc": _abi.TakeDesc,
 }
 
 

@@ -45,8 +45,12 @@ def _red(spec):
                 cat = dsa.category_binning(cat["column"], cat["lower"], cat["upper"],
                                            cat["nbins"])
         return dsa.by(cat, _red(spec["reduction"]))
+    if k == "where":
+        return dsa.where(_red(spec["selector"]), spec["lookup"])
     cls = getattr(dsa, k)
     col = spec.get("column")
+    if k in ("_min_row_index", "_max_row_index"):
+        return cls()
     return cls() if col is None and k in ("count", "any") else cls(col)
 
 
@@ -80,9 +84,13 @@ def test_points_golden(case, source):
     if "x_coords" in case:
         assert agg.coords["x"].values.tolist() == case["x_coords"]
         assert agg.coords["y"].values.tolist() == case["y_coords"]
-    k = case["agg"]["kind"] if "pick" not in case else case["agg"]["items"][case["pick"]]["kind"]
-    exp_dtype = {"count": np.uint32, "any": np.bool_, "count_cat": np.uint32,
-                 "by": np.uint32}.get(k, np.float64)
+    spec = case["agg"] if "pick" not in case else case["agg"]["items"][case["pick"]]
+    k = spec["kind"]
+    if k == "by":
+        k = spec["reduction"]["kind"]
+    exp_dtype = {"count": np.uint32, "any": np.bool_, "count_cat": np.uint32}.get(k, np.float64)
+    if "dtype" in case:
+        exp_dtype = np.dtype(case["dtype"])
     assert agg.values.dtype == exp_dtype
 
 

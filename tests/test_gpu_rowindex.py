@@ -1,0 +1,88 @@
+"""GPU parity of the row-index reductions (first, last, where, _min_row_index,
+_max_row_index) against the oracle, bit-exact: the selected row of every cell, and the
+looked-up value at it.  Random skewed points (hot cells hold thousands of rows), values
+with ties and NaNs, float32 and float64 coordinates and values, plain and by-category."""
+from __future__ import annotations
+
+import numpy as np
+import pandas as pd
+import pytest
+
+import datashader_amd as dsa
+from oracle import closed_forms as cf
+from oracle import rowindex as ri
+import golden_utils as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _frame(n, xdt, vdt, seed):
+    rng = np.random.default_rng(seed)
+    hot = rng.random(n) < 0.7
+    x = np.where(hot, rng.normal(0.5, 0.02, n), rng.uniform(0, 1, n)).astype(xdt)
+    y = np.where(hot, rng.normal(0.4, 0.02, n), rng.uniform(0, 1, n)).astype(xdt)
+    v = rng.integers(0, 50, n).astype(vdt)  # many ties per cell
+    v[rng.random(n) < 0.05] = np.nan
+    w = rng.normal(size=n)
+    cat = pd.Categorical(rng.choice(list("pqrs"), n), categories=list("pqrs"))
+    return pd.DataFrame({"x": x, "y": y, "v": v, "w": w, "cat": cat})
+
+
+MODES = {"first": "first", "last": "last", "max": "max", "min": "min"}
+
+
+@pytest.mark.parametrize("by", [False, True], ids=["plain", "by-cat"])
+@pytest.mark.parametrize("vdt", ["f4", "f8"])
+@pytest.mark.parametrize("xdt", ["f4", "f8"])
+def test_row_index_reductions_vs_oracle(xdt, vdt, by):
+    df = _frame(300_000, xdt, vdt, 3)
+    W, H = 123, 77
+    cvs = dsa.Canvas(W, H, x_range=(0, 1), y_range=(0, 1))
+    src = dsa.DeviceFrame.from_pandas(df)
+    view = G.view_for(W, H, (0, 1), (0, 1))
+    bins = cf.point_bins(df["x"].to_numpy(np.float64), df["y"].to_numpy(np.float64), view, W, H)
+    ncat = 4 if by else 1
+    cell = ri.cells(bins, df["cat"].cat.codes.to_numpy() if by else None, ncat)
+    v, w = df["v"].to_numpy(np.float64), df["w"].to_numpy()
+    shape = (H, W, ncat) if by else (H, W)
+
+    def wrap(r):
+        return dsa.by("cat", r) if by else r
+
+    aggs = {}
+    for name, sel in (("first", dsa.first("v")), ("last", dsa.last("v")),
+                      ("max", dsa.max("v")), ("min", dsa.min("v"))):
+        aggs[f"{name}_w"] = wrap(dsa.where(sel, "w"))
+        aggs[f"{name}_i"] = wrap(dsa.where(sel))
+    aggs["first"] = wrap(dsa.first("v"))
+    aggs["last"] = wrap(dsa.last("v"))
+    aggs["rmin"] = wrap(dsa._min_row_index())
+    aggs["rmax"] = wrap(dsa._max_row_index())
+    aggs["count"] = wrap(dsa.count())
+    got = cvs.points(src, "x", "y", dsa.summary(**aggs))
+    for name in MODES:
+        idx = ri.selected_rows_closed_form(cell, W * H * ncat, MODES[name], v).reshape(shape)
+        np.testing.assert_array_equal(got[f"{name}_i"].values, idx, err_msg=name)
+        assert got[f"{name}_i"].values.dtype == np.int64
+        G.assert_same(got[f"{name}_w"].values, ri.take(idx, w))
+    for name in ("first", "last"):
+        idx = ri.selected_rows_closed_form(cell, W * H * ncat, name, v).reshape(shape)
+        G.assert_same(got[name].values, ri.take(idx, v))
+    for name, mode in (("rmin", "first"), ("rmax", "last")):
+        idx = ri.selected_rows_closed_form(cell, W * H * ncat, mode).reshape(shape)
+        np.testing.assert_array_equal(got[name].values, idx)
+    assert int(got["count"].values.sum()) == int((cell >= 0).sum())
+
+
+def test_where_validation():
+    with pytest.raises(TypeError, match="selector can only be"):
+        dsa.where(dsa.sum("v"), "w")
+    df = _frame(100, "f8", "f8", 1)
+    with pytest.raises(ValueError, match="same column"):
+        dsa.Canvas(4, 4).points(df, "x", "y", dsa.where(dsa.max("v"), "v"))
+
+
+def test_row_index_lines_not_supported():
+    df = pd.DataFrame({"x": np.arange(5.0), "y": np.arange(5.0), "v": np.arange(5.0)})
+    with pytest.raises(NotImplementedError):
+        dsa.Canvas(4, 4).line(df, "x", "y", agg=dsa.first("v"))

@@ -232,3 +232,18 @@ def test_line_f32_delta_flag():
     a = oracle.line_axis0(xs, ys, v64, W, H)["count"]
     b = oracle.line_axis0(xs, ys, v32, W, H)["count"]
     np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("mode", ["first", "last", "max", "min"])
+def test_row_index_loop_matches_closed_form(mode):
+    """The sequential row-index restatement against its vectorised statement, on random
+    cells with ties (integer values) and NaNs."""
+    from oracle import rowindex as ri
+    rng = np.random.default_rng(17)
+    n, ncells = 20_000, 97
+    cell = rng.integers(-1, ncells, n)
+    val = rng.integers(0, 6, n).astype(float)
+    val[rng.random(n) < 0.1] = np.nan
+    for v in (val, None) if mode in ("first", "last") else (val,):
+        np.testing.assert_array_equal(ri.select_rows(cell, ncells, mode, v),
+                                      ri.selected_rows_closed_form(cell, ncells, mode, v))
