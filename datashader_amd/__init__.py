@@ -13,9 +13,10 @@ __version__ = "0.1.0"
 
 from .core import Canvas, LinearAxis, LogAxis, bypixel  # noqa: F401
 from .frame import DeviceFrame, Schema  # noqa: F401
-from .reductions import (_max_row_index, _min_row_index, any, by,  # noqa: F401,A004
-                         category_binning, category_codes, category_modulo, count,
-                         count_cat, first, last, max, mean, min, std, sum, summary, var,
+from .reductions import (_max_n_row_index, _max_row_index,  # noqa: F401,A004
+                         _min_n_row_index, _min_row_index, any, by, category_binning,
+                         category_codes, category_modulo, count, count_cat, first, first_n,
+                         last, last_n, max, max_n, mean, min, min_n, std, sum, summary, var,
                          where)
 from . import backend as _backend  # noqa: F401  (registers the pipelines)
 from .streaming import ChunkedFrame  # noqa: F401  (registers the chunked pipeline)

@@ -19,6 +19,7 @@ F32, F64, I8, U8, I16, I32, I64 = 1, 2, 3, 4, 5, 6, 7
 U32, BOOL = 8, 9
 # row-index reduction modes
 RI_FIRST, RI_LAST, RI_ARGMAX, RI_ARGMIN = 1, 2, 3, 4
+NS_FIRST, NS_LAST, NS_MAX, NS_MIN = 1, 2, 3, 4
 # shade kinds and transforms
 SHADE_CMAP, SHADE_COLOR, SHADE_CATEGORICAL, SHADE_COLOR_KEY = 0, 1, 2, 3
 HOW_CODES = {"linear": 0, "log": 1, "cbrt": 2, "eq_hist": 3}
@@ -32,7 +33,8 @@ EXPORTS = ("dsx_points_workspace_size", "dsx_points_agg", "dsx_lines_agg",
            "dsx_grid_index_workspace_size", "dsx_grid_index_build", "dsx_gather_rows",
            "dsx_points_indexed_workspace_size", "dsx_points_agg_indexed",
            "dsx_shade_workspace_size", "dsx_shade", "dsx_combine", "dsx_points_rowindex",
-           "dsx_take", "dsx_abi_version", "dsx_strerror", "dsx_last_hip_error")
+           "dsx_take", "dsx_points_nselect_workspace_size", "dsx_points_nselect",
+           "dsx_abi_version", "dsx_strerror", "dsx_last_hip_error")
 
 vp = ctypes.c_void_p
 
@@ -108,6 +110,11 @@ class RowIndexDesc(ctypes.Structure):
                 ("index", vp)]
 
 
+class NSelectDesc(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("n", ctypes.c_int32), ("nan_check", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("index", vp)]
+
+
 class TakeDesc(ctypes.Structure):
     _fields_ = [("ncells", ctypes.c_int64), ("index", vp), ("col", vp),
                 ("dtype", ctypes.c_int32), ("reserved", ctypes.c_int32), ("out", vp)]
@@ -170,6 +177,9 @@ def load():
         lib.dsx_combine.argtypes = [P(CombineDesc), vp]
         lib.dsx_points_rowindex.argtypes = [P(PointsDesc), P(View), P(RowIndexDesc), vp]
         lib.dsx_take.argtypes = [P(TakeDesc), vp]
+        lib.dsx_points_nselect_workspace_size.argtypes = [P(PointsDesc), P(View), P(NSelectDesc)]
+        lib.dsx_points_nselect_workspace_size.restype = sz
+        lib.dsx_points_nselect.argtypes = [P(PointsDesc), P(View), P(NSelectDesc), vp, sz, vp]
         lib.dsx_shade_workspace_size.argtypes = [P(ShadeDesc)]
         lib.dsx_shade_workspace_size.restype = sz
         lib.dsx_shade.argtypes = [P(ShadeDesc), vp, sz, vp]
@@ -181,7 +191,7 @@ def load():
         for f in ("dsx_points_agg", "dsx_lines_agg", "dsx_bounds", "dsx_fill", "dsx_finalize",
                   "dsx_grid_index_build", "dsx_gather_rows", "dsx_points_agg_indexed",
                   "dsx_shade", "dsx_combine", "dsx_points_rowindex", "dsx_take",
-                  "dsx_abi_version"):
+                  "dsx_points_nselect", "dsx_abi_version"):
             getattr(lib, f).restype = ctypes.c_int
         if lib.dsx_abi_version() != ABI_VERSION:
             raise RuntimeError("libdsx.so ABI version mismatch; rebuild the library")

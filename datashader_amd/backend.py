@@ -387,13 +387,16 @@ def row_index_leaf(lib, frame, schema, glyph, view, red, passes, results, H, W):
     sel = inner.selector if isinstance(inner, rd.where) else inner
     if isinstance(inner, rd.where):
         lookup = inner.column
-    elif isinstance(inner, rd._first_or_last):
-        lookup = inner.column
+    elif isinstance(inner, (rd._first_or_last, rd.FloatingNReduction)):
+        lookup = inner.column  # None for the row-index selectors: the index itself
     else:
         lookup = None  # the row-index selectors return the index itself
     device = frame.device
     catc, catcol, ncat = _categorizer(cat, frame, schema)
     ncells = H * W * ncat
+    if isinstance(sel, rd.FloatingNReduction):
+        return _n_select_leaf(lib, frame, schema, glyph, view, red, sel, lookup, cat, catc,
+                              ncat, H, W)
     r = _abi.RowIndexDesc()
     key = None
     valcol = sel.column
@@ -431,6 +434,49 @@ def row_index_leaf(lib, frame, schema, glyph, view, red, passes, results, H, W):
     return out.reshape(shape)
 
 
+def _take(lib, index, col, device):
+    """float64 values of ``col`` at the rows of ``index`` (NaN where -1)."""
+    t = dev.torch()
+    if col.dtype == t.bool:
+        col = col.to(t.uint8)
+    tk = _abi.TakeDesc()
+    tk.ncells = int(index.numel())
+    tk.index = dev.ptr(index)
+    tk.col = dev.ptr(col)
+    tk.dtype = dev.dtype_code(col)
+    out = t.empty(index.numel(), dtype=t.float64, device=device)
+    tk.out = dev.ptr(out)
+    _abi.check(lib.dsx_take(tk, dev.stream_handle()), "dsx_take")
+    return out
+
+
+def _n_select_leaf(lib, frame, schema, glyph, view, red, sel, lookup, cat, catc, ncat, H, W):
+    """max_n / min_n / first_n / last_n / the n row-index selectors (and where over
+    them): the rows of each cell's n-slot list (dsx_points_nselect), then the lookup."""
+    t = dev.torch()
+    device = frame.device
+    n = sel.n
+    r = _abi.NSelectDesc()
+    r.mode = sel.ns_mode
+    r.n = n
+    r.nan_check = 1 if sel.column is not None else 0
+    index = t.empty(H * W * ncat * n, dtype=t.int64, device=device)
+    r.index = dev.ptr(index)
+    p = Pass(sel.column, cat)
+    d, keep = _pass_desc(frame, schema, glyph, view, p, device, catc)
+    vw = _view_struct(view)
+    ws = int(lib.dsx_points_nselect_workspace_size(d, vw, r))
+    if ws == 0:
+        raise _abi.DsxError("dsx_points_nselect_workspace_size rejected the call")
+    work = t.empty(ws, dtype=t.uint8, device=device)
+    _abi.check(lib.dsx_points_nselect(d, vw, r, dev.ptr(work), ws, dev.stream_handle()),
+               "dsx_points_nselect")
+    shape = (H, W, ncat, n) if isinstance(red, rd.by) else (H, W, n)
+    if lookup is None:
+        return index.reshape(shape)
+    return _take(lib, index, frame[lookup], device).reshape(shape)
+
+
 def aggregate(frame, schema, canvas, glyph, agg):
     """Run the aggregation on the device; returns (results, view, x_range, y_range)
     where ``results`` is ``[(name, reduction, device tensor)]``."""
@@ -462,6 +508,12 @@ def _wrap(red, data, canvas, glyph, view, x_range, y_range, schema, to_host, nam
     if isinstance(red, rd.by):
         dims.append(red.cat_column)
         coords[red.cat_column] = red.categorizer.categories(schema)
+    inner = red.reduction if isinstance(red, rd.by) else red
+    sel = inner.selector if isinstance(inner, rd.where) else inner
+    if isinstance(sel, rd.FloatingNReduction):
+        # FloatingNReduction._add_finalize_kwargs (reductions.py:1467-1478)
+        dims.append("n")
+        coords["n"] = np.arange(sel.n)
     attrs = dict(x_range=x_range, y_range=y_range)
     if to_host:
         data = data.cpu().numpy()

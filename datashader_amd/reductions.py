@@ -295,6 +295,68 @@ class _max_row_index(_max_or_min_row_index):
     ri_mode = _abi.RI_LAST
 
 
+class FloatingNReduction(OptionalFieldReduction):
+    """Reductions that keep ``n`` values per bin, along a trailing ``n`` dimension
+    (reductions.py:1457-1494); float64 with NaN padding (int64 with -1 for the row-index
+    forms).  Computed from the rows of each bin sorted in the reduction's order."""
+
+    row_index = True
+    ns_mode = None
+
+    def __init__(self, column=None, n=1):
+        super().__init__(column)
+        self.n = n if n >= 1 else 1
+
+    def planes(self):
+        return ()
+
+    def _key(self):
+        return (type(self).__name__, self.column, self.n)
+
+    def __repr__(self):
+        return f"{type(self).__name__}(column={self.column!r}, n={self.n!r})"
+
+
+class max_n(FloatingNReduction):
+    """The ``n`` largest values of the bin, descending (reductions.py:1629-1713)."""
+    ns_mode = _abi.NS_MAX
+
+
+class min_n(FloatingNReduction):
+    """The ``n`` smallest values of the bin, ascending (reductions.py:1716-1803)."""
+    ns_mode = _abi.NS_MIN
+
+
+class first_n(FloatingNReduction):
+    """The first ``n`` non-NaN values of the bin in row order (reductions.py:1535-1588)."""
+    ns_mode = _abi.NS_FIRST
+
+
+class last_n(FloatingNReduction):
+    """The last ``n`` non-NaN values of the bin, latest first (reductions.py:1591-1626)."""
+    ns_mode = _abi.NS_LAST
+
+
+class _max_n_or_min_n_row_index(FloatingNReduction):
+    out_dtype = np.int64
+
+    def __init__(self, n=1):
+        super().__init__(None, n)
+
+    def __repr__(self):
+        return f"{type(self).__name__}(n={self.n!r})"
+
+
+class _max_n_row_index(_max_n_or_min_n_row_index):
+    """The ``n`` largest row indices of the bin, descending (reductions.py:2398-2475)."""
+    ns_mode = _abi.NS_LAST
+
+
+class _min_n_row_index(_max_n_or_min_n_row_index):
+    """The ``n`` smallest row indices of the bin, ascending (reductions.py:2478-2562)."""
+    ns_mode = _abi.NS_FIRST
+
+
 class where(FloatingReduction):
     """Values of ``lookup_column`` at the row a ``selector`` (first, last, max, min or a
     row-index selector) picks in each bin; with no lookup column the row index itself
@@ -304,7 +366,8 @@ class where(FloatingReduction):
     row_index = True
 
     def __init__(self, selector, lookup_column=None):
-        if not isinstance(selector, (first, last, max, min, _max_or_min_row_index)):
+        if not isinstance(selector, (first, last, max, min, _max_or_min_row_index,
+                                     FloatingNReduction)):
             raise TypeError("selector can only be a first, first_n, last, last_n, "
                             "max, max_n, min or min_n reduction")
         super().__init__(lookup_column)

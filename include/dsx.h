@@ -210,6 +210,29 @@ typedef struct {
 int dsx_points_rowindex(const dsx_points_desc *d, const dsx_view *v,
                         const dsx_rowindex_desc *r, void *stream);
 
+/* ---- n-value reductions (max_n / min_n / first_n / last_n) ------------------------ *
+ * For every cell the rows of the reference's n-slot list (reductions.py:1457-1805,
+ * 2375-2562), in its order: FIRST the first n rows (first_n, _min_n_row_index), LAST the
+ * last n, latest first (last_n, _max_n_row_index), MAX / MIN the rows of the n largest /
+ * smallest values, rows that tie keeping row order (max_n / min_n insert only before a
+ * strictly smaller / larger value).  index[cell * n + k]; -1 pads cells with fewer rows.
+ * Rows with a NaN value are skipped when nan_check is set (always for MAX / MIN).        */
+typedef enum { DSX_NS_FIRST = 1, DSX_NS_LAST = 2, DSX_NS_MAX = 3, DSX_NS_MIN = 4 }
+    dsx_ns_mode;
+
+typedef struct {
+    int32_t mode;                 /* dsx_ns_mode                                          */
+    int32_t n;                    /* rows kept per cell, 1..4096                          */
+    int32_t nan_check;
+    int32_t reserved;
+    int64_t *index;               /* out, H*W*ncat*n                                      */
+} dsx_nselect_desc;
+
+size_t dsx_points_nselect_workspace_size(const dsx_points_desc *d, const dsx_view *v,
+                                         const dsx_nselect_desc *r);
+int dsx_points_nselect(const dsx_points_desc *d, const dsx_view *v, const dsx_nselect_desc *r,
+                       void *workspace, size_t workspace_bytes, void *stream);
+
 /* out[c] = col[index[c]] as float64, NaN where index[c] < 0 (where()'s lookup,
  * reductions.py:1914-1925).  col dtype: DSX_F32, DSX_F64, DSX_I8..DSX_I64, DSX_U32.     */
 typedef struct {

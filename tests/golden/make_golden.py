@@ -211,6 +211,113 @@ def point_cases(frame):
         [[[4, -1, nan, -3], [12, -13, 14, -11]], [[8, -9, 6, -7], [16, -17, 18, -19]]],
         cat_coords=["a", "b", "c", "d"], cat_dim="cat2")
 
+    # n-value reductions (max_n / min_n / first_n / last_n / row-index forms / where)
+    pm_min = np.array([[[-3, -1, 0, 4, nan, nan], [-13, -11, 10, 12, 14, nan]],
+                       [[-9, -7, -5, 6, 8, nan], [-19, -17, -15, 16, 18, nan]]])
+    pm_max = np.array([[[4, 0, -1, -3, nan, nan], [14, 12, 10, -11, -13, nan]],
+                       [[8, 6, -5, -7, -9, nan], [18, 16, -15, -17, -19, nan]]])
+    pm_first = np.array([[[0, -1, -3, 4, nan, nan], [10, -11, 12, -13, 14, nan]],
+                         [[-5, 6, -7, 8, -9, nan], [-15, 16, -17, 18, -19, nan]]])
+    pm_last = np.array([[[4, -3, -1, 0, nan, nan], [14, -13, 12, -11, 10, nan]],
+                        [[-9, 8, -7, 6, -5, nan], [-19, 18, -17, 16, -15, nan]]])
+    ri_min = np.array([[[0, 1, 2, 3, 4, -1], [10, 11, 12, 13, 14, -1]],
+                       [[5, 6, 7, 8, 9, -1], [15, 16, 17, 18, 19, -1]]])
+    ri_max = np.array([[[4, 3, 2, 1, 0, -1], [14, 13, 12, 11, 10, -1]],
+                       [[9, 8, 7, 6, 5, -1], [19, 18, 17, 16, 15, -1]]])
+    w_min_ri = np.array([[[0, -1, nan, -3, 4, nan], [10, -11, 12, -13, 14, nan]],
+                         [[-5, 6, -7, 8, -9, nan], [-15, 16, -17, 18, -19, nan]]])
+    w_max_ri = np.array([[[4, -3, nan, -1, 0, nan], [14, -13, 12, -11, 10, nan]],
+                         [[-9, 8, -7, 6, -5, nan], [-19, 18, -17, 16, -15, nan]]])
+    sel_rows = {  # test_where_{first,last,max,min}_n (512-621): selected row indices
+        "first_n": np.array([[[0, 1, 3, 4, -1, -1], [10, 11, 12, 13, 14, -1]],
+                             [[5, 6, 7, 8, 9, -1], [15, 16, 17, 18, 19, -1]]]),
+        "last_n": np.array([[[4, 3, 1, 0, -1, -1], [14, 13, 12, 11, 10, -1]],
+                            [[9, 8, 7, 6, 5, -1], [19, 18, 17, 16, 15, -1]]]),
+        "max_n": np.array([[[4, 0, 1, 3, -1, -1], [14, 12, 10, 11, 13, -1]],
+                           [[8, 6, 5, 7, 9, -1], [18, 16, 15, 17, 19, -1]]]),
+        "min_n": np.array([[[3, 1, 0, 4, -1, -1], [13, 11, 10, 12, 14, -1]],
+                           [[9, 7, 5, 6, 8, -1], [19, 17, 15, 16, 18, -1]]]),
+    }
+    cites = {"first_n": ":512-536", "last_n": ":539-563", "max_n": ":566-589",
+             "min_n": ":592-615"}
+    for n in (1, 3, 6):
+        for kind, sol, cite in (("min_n", pm_min, ":246-254"), ("max_n", pm_max, ":257-265"),
+                                ("first_n", pm_first, ":1007-1015"),
+                                ("last_n", pm_last, ":1018-1026")):
+            add(f"test_{kind}/n={n}", TP + cite, {"kind": kind, "column": "plusminus", "n": n},
+                sol[:, :, :n], n_dim=True)
+        add(f"test_min_n_row_index/n={n}", TP + ":1043-1052",
+            {"kind": "_min_n_row_index", "n": n}, ri_min[:, :, :n], dtype="int64", n_dim=True)
+        add(f"test_max_n_row_index/n={n}", TP + ":1055-1064",
+            {"kind": "_max_n_row_index", "n": n}, ri_max[:, :, :n], dtype="int64", n_dim=True)
+        add(f"test_where_min_n_row_index/n={n}", TP + ":414-428",
+            {"kind": "where", "selector": {"kind": "_min_n_row_index", "n": n},
+             "lookup": "plusminus"}, w_min_ri[:, :, :n], n_dim=True)
+        add(f"test_where_max_n_row_index/n={n}", TP + ":431-445",
+            {"kind": "where", "selector": {"kind": "_max_n_row_index", "n": n},
+             "lookup": "plusminus"}, w_max_ri[:, :, :n], n_dim=True)
+        for kind, rows in sel_rows.items():
+            sel = {"kind": kind, "column": "plusminus", "n": n}
+            add(f"test_where_{kind}/rows/n={n}", TP + cites[kind],
+                {"kind": "where", "selector": sel, "lookup": None}, rows[:, :, :n],
+                dtype="int64", n_dim=True)
+            rev = np.where((rows < 0) | (rows == 6), nan, 20 - rows)
+            add(f"test_where_{kind}/reverse/n={n}", TP + cites[kind],
+                {"kind": "where", "selector": sel, "lookup": "reverse"}, rev[:, :, :n],
+                n_dim=True)
+    cat_sols = {
+        "min_n": ("f32", [[[[0, 4], [1, nan], [nan, nan], [3, nan]],
+                           [[12, nan], [13, nan], [10, 14], [11, nan]]],
+                          [[[8, nan], [5, 9], [6, nan], [7, nan]],
+                           [[16, nan], [17, nan], [18, nan], [15, 19]]]], ":288-299"),
+        "max_n": ("f32", [[[[4, 0], [1, nan], [nan, nan], [3, nan]],
+                           [[12, nan], [13, nan], [14, 10], [11, nan]]],
+                          [[[8, nan], [9, 5], [6, nan], [7, nan]],
+                           [[16, nan], [17, nan], [18, nan], [19, 15]]]], ":302-313"),
+        "first_n": ("plusminus", [[[[0, 4], [-1, nan], [nan, nan], [-3, nan]],
+                                   [[12, nan], [-13, nan], [10, 14], [-11, nan]]],
+                                  [[[8, nan], [-5, -9], [6, nan], [-7, nan]],
+                                   [[16, nan], [-17, nan], [18, nan], [-15, -19]]]], ":376-387"),
+        "last_n": ("plusminus", [[[[4, 0], [-1, nan], [nan, nan], [-3, nan]],
+                                  [[12, nan], [-13, nan], [14, 10], [-11, nan]]],
+                                 [[[8, nan], [-9, -5], [6, nan], [-7, nan]],
+                                  [[16, nan], [-17, nan], [18, nan], [-19, -15]]]], ":390-401"),
+    }
+    cat_ri = {
+        "_min_n_row_index": ([[[[0, 4], [1, -1], [2, -1], [3, -1]],
+                               [[12, -1], [13, -1], [10, 14], [11, -1]]],
+                              [[[8, -1], [5, 9], [6, -1], [7, -1]],
+                               [[16, -1], [17, -1], [18, -1], [15, 19]]]], ":328-339"),
+        "_max_n_row_index": ([[[[4, 0], [1, -1], [2, -1], [3, -1]],
+                               [[12, -1], [13, -1], [14, 10], [11, -1]]],
+                              [[[8, -1], [9, 5], [6, -1], [7, -1]],
+                               [[16, -1], [17, -1], [18, -1], [19, 15]]]], ":342-353"),
+    }
+    for n in (1, 2):
+        for kind, (column, sol, cite) in cat_sols.items():
+            add(f"test_categorical_{kind}/n={n}", TP + cite,
+                {"kind": "by", "cat": "cat2",
+                 "reduction": {"kind": kind, "column": column, "n": n}},
+                np.array(sol)[..., :n], cat_coords=["a", "b", "c", "d"], cat_dim="cat2",
+                n_dim=True)
+        for kind, (sol, cite) in cat_ri.items():
+            add(f"test_categorical{kind}/n={n}", TP + cite,
+                {"kind": "by", "cat": "cat2", "reduction": {"kind": kind, "n": n}},
+                np.array(sol)[..., :n], cat_coords=["a", "b", "c", "d"], cat_dim="cat2",
+                dtype="int64", n_dim=True)
+    summ_n = {"kind": "summary", "items": {
+        "count": {"kind": "count", "column": None},
+        "min_n": {"kind": "where", "selector": {"kind": "min_n", "column": "plusminus", "n": 5},
+                  "lookup": None},
+        "max_n": {"kind": "where", "selector": {"kind": "max_n", "column": "plusminus", "n": 5},
+                  "lookup": "reverse"}}}
+    add("test_summary_where_n/min_n", TP + ":648-676", summ_n, sel_rows["min_n"][:, :, :5],
+        pick="min_n", dtype="int64", n_dim=True)
+    mx = sel_rows["max_n"][:, :, :5]
+    add("test_summary_where_n/max_n", TP + ":648-676", summ_n,
+        np.where((mx < 0) | (mx == 6), nan, 20 - mx), pick="max_n", n_dim=True)
+    add("test_summary_where_n/count", TP + ":648-676", summ_n, [[5, 5], [5, 5]], pick="count")
+
     # test_auto_range_points, test_pandas.py:1080-1129 (integer x, y; autorange)
     frames = {}
     for n, mult, extra, cite in ((10, 1, 0, ":1080-1092"), (10, 1, 1, ":1094-1101"),

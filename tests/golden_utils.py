@@ -117,13 +117,20 @@ _ROW_MODES = {"first": "first", "last": "last", "_min_row_index": "first",
               "_max_row_index": "last", "max": "max", "min": "min"}
 
 
+_N_MODES = {"max_n": "max", "min_n": "min", "first_n": "first", "last_n": "last",
+            "_min_n_row_index": "first", "_max_n_row_index": "last"}
+
+
 def _leaf_inner(spec, run, codes, ncat):
     kind, col = spec["kind"], spec.get("column")
-    if kind in ("first", "last", "_min_row_index", "_max_row_index", "where"):
+    if kind in _ROW_MODES or kind in _N_MODES or kind == "where":
         if kind == "where":
             sel, lookup = spec["selector"], spec["lookup"]
         else:
             sel, lookup = spec, spec.get("column")
+        if sel["kind"] in _N_MODES:
+            return run.n_select(_N_MODES[sel["kind"]], sel.get("column"), lookup, sel["n"],
+                                codes, ncat, codes is None)
         return run.row_index(_ROW_MODES[sel["kind"]], sel.get("column"), lookup, codes, ncat,
                              codes is None)
     val = None if col is None else run.column(col)
@@ -175,6 +182,18 @@ class PointRun:
         idx = ri.select_rows(cell, self.W * self.H * ncat, mode, val)
         out = idx if lookup is None else ri.take(idx, self.column(lookup))
         return out.reshape((self.H, self.W) if squeeze else (self.H, self.W, ncat))
+
+    def n_select(self, mode, selcol, lookup, n, codes, ncat, squeeze):
+        """The n-value reductions with the sequential oracle."""
+        from oracle import closed_forms as cf
+        from oracle import rowindex as ri
+        bins = cf.point_bins(self.df[self.x].to_numpy(np.float64),
+                             self.df[self.y].to_numpy(np.float64), self.view, self.W, self.H)
+        cell = ri.cells(bins, codes, ncat)
+        val = None if selcol is None else self.column(selcol)
+        idx = ri.select_n(cell, self.W * self.H * ncat, n, mode, val)
+        out = idx if lookup is None else ri.take(idx, self.column(lookup))
+        return out.reshape((self.H, self.W, n) if squeeze else (self.H, self.W, ncat, n))
 
     def __call__(self, bases, val, codes, ncat, squeeze):
         return oracle.points(self.df[self.x].to_numpy(np.float64),

@@ -51,6 +51,10 @@ def _red(spec):
     col = spec.get("column")
     if k in ("_min_row_index", "_max_row_index"):
         return cls()
+    if k in ("_min_n_row_index", "_max_n_row_index"):
+        return cls(n=spec["n"])
+    if "n" in spec:
+        return cls(col, n=spec["n"])
     return cls() if col is None and k in ("count", "any") else cls(col)
 
 
@@ -76,11 +80,14 @@ def test_points_golden(case, source):
     if "x_range" in case:
         assert np.allclose(agg.attrs["x_range"], case["x_range"])
         assert np.allclose(agg.attrs["y_range"], case["y_range"])
+    ndim = ["n"] if case.get("n_dim") else []
     if "cat_dim" in case:
-        assert list(agg.dims) == [case["y"], case["x"], case["cat_dim"]]
+        assert list(agg.dims) == [case["y"], case["x"], case["cat_dim"]] + ndim
         assert list(agg.coords[case["cat_dim"]].values) == case["cat_coords"]
     else:
-        assert list(agg.dims) == [case["y"], case["x"]]
+        assert list(agg.dims) == [case["y"], case["x"]] + ndim
+    if ndim:
+        assert agg.coords["n"].values.tolist() == list(range(agg.shape[-1]))
     if "x_coords" in case:
         assert agg.coords["x"].values.tolist() == case["x_coords"]
         assert agg.coords["y"].values.tolist() == case["y_coords"]

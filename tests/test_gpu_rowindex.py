@@ -86,3 +86,46 @@ def test_row_index_lines_not_supported():
     df = pd.DataFrame({"x": np.arange(5.0), "y": np.arange(5.0), "v": np.arange(5.0)})
     with pytest.raises(NotImplementedError):
         dsa.Canvas(4, 4).line(df, "x", "y", agg=dsa.first("v"))
+
+
+N_MODES = {"max_n": "max", "min_n": "min", "first_n": "first", "last_n": "last"}
+
+
+@pytest.mark.parametrize("n", [1, 4])
+@pytest.mark.parametrize("by", [False, True], ids=["plain", "by-cat"])
+@pytest.mark.parametrize("vdt", ["f4", "f8"])
+def test_n_reductions_vs_oracle(vdt, by, n):
+    df = _frame(400_000, "f4", vdt, 9)
+    W, H = 97, 61
+    cvs = dsa.Canvas(W, H, x_range=(0, 1), y_range=(0, 1))
+    src = dsa.DeviceFrame.from_pandas(df)
+    view = G.view_for(W, H, (0, 1), (0, 1))
+    bins = cf.point_bins(df["x"].to_numpy(np.float64), df["y"].to_numpy(np.float64), view, W, H)
+    ncat = 4 if by else 1
+    cell = ri.cells(bins, df["cat"].cat.codes.to_numpy() if by else None, ncat)
+    v, w = df["v"].to_numpy(np.float64), df["w"].to_numpy()
+    shape = (H, W, ncat, n) if by else (H, W, n)
+
+    def wrap(r):
+        return dsa.by("cat", r) if by else r
+
+    aggs = {}
+    for name in N_MODES:
+        red = getattr(dsa, name)
+        aggs[name] = wrap(red("v", n=n))
+        aggs[name + "_i"] = wrap(dsa.where(red("v", n=n)))
+        aggs[name + "_w"] = wrap(dsa.where(red("v", n=n), "w"))
+    aggs["rmin"] = wrap(dsa._min_n_row_index(n=n))
+    aggs["rmax"] = wrap(dsa._max_n_row_index(n=n))
+    got = cvs.points(src, "x", "y", dsa.summary(**aggs))
+    ncells = W * H * ncat
+    for name, mode in N_MODES.items():
+        idx = ri.select_n_closed_form(cell, ncells, n, mode, v).reshape(shape)
+        np.testing.assert_array_equal(got[name + "_i"].values, idx, err_msg=name)
+        G.assert_same(got[name].values, ri.take(idx, v))
+        G.assert_same(got[name + "_w"].values, ri.take(idx, w))
+        assert got[name].dims[-1] == "n"
+    for name, mode in (("rmin", "first"), ("rmax", "last")):
+        idx = ri.select_n_closed_form(cell, ncells, n, mode).reshape(shape)
+        np.testing.assert_array_equal(got[name].values, idx)
+        assert got[name].values.dtype == np.int64
