@@ -10,7 +10,7 @@ bandwidth (algorithmic bytes per launch / average launch time from HIP events on
 kernel's stream) against the 8 TB/s peak, and the CPU oracle timed on the host's cores.
 
 Other workloads: --workload points800 | points_skewed | bycat | lines | serving | shade |
-stream.
+stream | rowindex.
 """
 from __future__ import annotations
 
@@ -433,6 +433,71 @@ def run_shade(args, torch):
     print(json.dumps(out), flush=True)
 
 
+def run_rowindex(args, torch):
+    """Row-index reductions over the headline's 100M rows (1920x1080): first, last,
+    where(max), where(min) (one atomic pass each after the max / min pass) and max_n(3)
+    (a device sort of the kept rows); the sequential oracle on a sample is the CPU
+    baseline.  Prints one JSON line."""
+    import numpy as np
+
+    import datashader_amd as dsa
+    from datashader_amd import _device as dev
+    from datashader_amd import synthetic
+
+    n = int(100_000_000 * args.scale)
+    W, H = 1920, 1080
+    x, y, v = synthetic.points_skewed_dev(n, seed=2)
+    w = torch.arange(n, device="cuda", dtype=torch.float32)
+    frame = dsa.DeviceFrame({"x": x, "y": y, "v": v, "w": w})
+    xr = dev.maybe_expand_bounds(dev.bounds(x))
+    yr = dev.maybe_expand_bounds(dev.bounds(y))
+    cvs = dsa.Canvas(W, H, x_range=xr, y_range=yr)
+    cases = [("first", dsa.first("v")), ("last", dsa.last("v")),
+             ("where_max", dsa.where(dsa.max("v"), "w")),
+             ("where_min", dsa.where(dsa.min("v"))), ("max_n3", dsa.max_n("v", n=3))]
+    results = {}
+    for label, red in cases:
+        for _ in range(args.warmup):
+            cvs.points(frame, "x", "y", red)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = cvs.points(frame, "x", "y", red)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        results[label] = {"ms_per_call": ms, "gpoints_per_s": n / (ms * 1e-3) / 1e9,
+                          "checksum": float(np.nansum(res.values.astype(np.float64)))}
+        _log(f"[bench] rowindex {label}: {ms:.3f} ms/call")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import closed_forms as cf
+        from oracle import rowindex as ri
+        m = min(args.cpu_sample, 5_000_000)
+        xs, ys, vs = (t_[:m].double().cpu().numpy() for t_ in (x, y, v))
+        sx, tx = cvs.x_axis.compute_scale_and_translate(xr, W)
+        sy, ty = cvs.y_axis.compute_scale_and_translate(yr, H)
+        import oracle
+        view = oracle.View(sx, tx, sy, ty, xr[0], xr[1], yr[0], yr[1])
+        t0 = time.perf_counter()
+        cell = cf.point_bins(xs, ys, view, W, H)
+        ri.select_rows(cell, W * H, "first", vs)
+        sec = time.perf_counter() - t0
+        cpu = {"value": m / sec / 1e9, "unit": "Gpoints/s", "cores": 1, "kind": "port",
+               "sample": f"first {m} rows, first('v') by the sequential Python oracle"}
+    head = results["where_max"]
+    out = {
+        "metric": "Gpoints/s aggregated (row-index reductions)", "value": head["gpoints_per_s"],
+        "unit": "Gpoints/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_call"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded, generated on the device)",
+        "config": {"workload": "row-index reductions over 100M float32 skewed points into "
+                               "1920x1080: where(max('v'), 'w') (value), and first, last, "
+                               "where(min('v')), max_n('v', 3)", "rows": n},
+        "cpu_baseline": cpu, "cases": results,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -440,7 +505,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="points_skewed",
                     choices=["points_skewed", "points800", "bycat", "lines", "serving",
-                             "shade", "stream"])
+                             "shade", "stream", "rowindex"])
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload size")
     ap.add_argument("--cpu-sample", type=int, default=20_000_000,
                     help="rows (or line samples) of the CPU oracle baseline sample")
@@ -468,6 +533,9 @@ def main():
         return
     if args.workload == "shade":
         run_shade(args, torch)
+        return
+    if args.workload == "rowindex":
+        run_rowindex(args, torch)
         return
     _log(f"[bench] building workload {args.workload} (scale {args.scale})")
     step, rows, algo, cfg, cpu, check = build_workload(args.workload, torch, args.scale)
