@@ -18,7 +18,7 @@ import numpy as np
 
 from . import reductions as rd
 from .glyphs import (LineAxis0, LineAxis0Multi, LinesAxis1, LinesAxis1XConstant,
-                     LinesAxis1YConstant, Point)
+                     LinesAxis1YConstant, LinesAxis1Ragged, Point)
 
 
 class Axis:
@@ -166,8 +166,7 @@ See docstring for more information on valid usage""")
             elif isinstance(x, (list, tuple)) and isinstance(y, np.ndarray):
                 glyph = LinesAxis1YConstant(tuple(x), y)
             elif isinstance(x, (Number, str)) and isinstance(y, (Number, str)):
-                raise NotImplementedError("ragged-array lines are not supported by the "
-                                          "MI355X backend")
+                glyph = LinesAxis1Ragged(x, y)
             else:
                 raise ValueError(f"""
 Invalid combination of x and y arguments to Canvas.line when axis=1.

@@ -29,6 +29,69 @@ def _torch():
     return torch
 
 
+def is_ragged(series):
+    """A pandas column whose rows are variable-length sequences: the reference's
+    ``RaggedArray`` (anything exposing ``start_indices`` / ``flat_array``) or an object
+    column of arrays / lists."""
+    arr = getattr(series, "array", None)
+    if hasattr(arr, "start_indices") and hasattr(arr, "flat_array"):
+        return True
+    if series.dtype != object or len(series) == 0:
+        return False
+    first = series.iloc[0]
+    return isinstance(first, (list, tuple, np.ndarray))
+
+
+def ragged_parts(series):
+    """(start indices with the total appended, flat values) of a ragged column."""
+    arr = getattr(series, "array", None)
+    if hasattr(arr, "start_indices") and hasattr(arr, "flat_array"):
+        flat = np.asarray(arr.flat_array)
+        starts = np.append(np.asarray(arr.start_indices, np.int64), len(flat))
+        return starts, flat
+    rows = [np.asarray(r) for r in series]
+    lens = np.array([len(r) for r in rows], np.int64)
+    starts = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    dt = np.float32 if rows and all(r.dtype == np.float32 for r in rows) else np.float64
+    flat = np.concatenate([r.astype(dt) for r in rows]) if rows else np.zeros(0, dt)
+    return starts, flat
+
+
+class RaggedColumn:
+    """A ragged column on the device: row r holds ``flat[starts[r]:starts[r+1]]``
+    (``starts`` has one entry more than rows).  Counts as a 1-D column of ``len(starts)-1``
+    rows in a DeviceFrame."""
+
+    def __init__(self, starts, flat):
+        self.starts = starts
+        self.flat = flat
+
+    @property
+    def shape(self):
+        return (int(self.starts.numel()) - 1,)
+
+    def dim(self):
+        return 1
+
+    @property
+    def dtype(self):
+        return self.flat.dtype
+
+    @property
+    def device(self):
+        return self.flat.device
+
+    def __len__(self):
+        return self.shape[0]
+
+    @classmethod
+    def from_series(cls, series, device="cuda"):
+        torch = _torch()
+        starts, flat = ragged_parts(series)
+        return cls(torch.from_numpy(starts).to(device),
+                   torch.from_numpy(np.ascontiguousarray(flat)).to(device))
+
+
 class Schema:
     """Column name -> kind ('f4', 'f8', 'i4', ..., 'category'), plus categories."""
 
@@ -56,12 +119,16 @@ class Schema:
             if isinstance(dt, pd.CategoricalDtype):
                 kinds[name] = "category"
                 cats[name] = list(dt.categories)
+            elif is_ragged(df[name]):
+                kinds[name] = "ragged"
             else:
                 kinds[name] = _KIND.get(np.dtype(dt), "O") if dt != object else "O"
         return cls(kinds, cats)
 
 
 def _torch_kind(t):
+    if isinstance(t, RaggedColumn):
+        return "ragged"
     torch = _torch()
     m = {torch.float32: "f4", torch.float64: "f8", torch.int8: "i1", torch.int16: "i2",
          torch.int32: "i4", torch.int64: "i8", torch.uint8: "u1", torch.bool: "b1"}
@@ -99,6 +166,10 @@ class DeviceFrame:
                 codes = np.ascontiguousarray(df[name].cat.codes.to_numpy())
                 cols[name] = torch.from_numpy(codes).to(device)
                 cats[name] = list(dt.categories)
+            elif is_ragged(df[name]):
+                cols[name] = RaggedColumn.from_series(df[name], device)
+            elif not isinstance(dt, np.dtype):
+                continue  # other extension types are never aggregated
             elif np.dtype(dt).kind == "f":
                 groups.setdefault(np.dtype(dt), []).append(name)
             elif np.dtype(dt).kind in "iub":

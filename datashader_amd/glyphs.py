@@ -217,3 +217,49 @@ class LinesAxis1YConstant(LinesAxis1):
         yt = _const_tensor(self.y, xb.device)
         return dict(nlines=len(frame), nverts=len(self.x), x=xb, y=yt, xls=xrs, xvs=xcs,
                     yls=0, yvs=1, axis=1)
+
+
+class LinesAxis1Ragged(_PointLike, _LineLike):
+    """One line per row of two ragged columns (``glyphs/line.py:457-523``): row i joins
+    the first ``min(len(x_i), len(y_i))`` vertices (``line.py:1569-1586``).  The rows are
+    laid out as one NaN-padded ``[rows, longest]`` matrix per coordinate on the device; a
+    NaN vertex ends a line's segments, so the padding draws nothing."""
+
+    def validate(self, schema):
+        if schema[str(self.x)] != "ragged":
+            raise ValueError("x must be a RaggedArray")
+        elif schema[str(self.y)] != "ragged":
+            raise ValueError("y must be a RaggedArray")
+
+    def required_columns(self):
+        return [self.x, self.y]
+
+    def compute_x_bounds(self, frame):
+        # over every stored value, as the reference bounds the flat array
+        return self.maybe_expand_bounds(dev.bounds(frame[self.x].flat))
+
+    def compute_y_bounds(self, frame):
+        return self.maybe_expand_bounds(dev.bounds(frame[self.y].flat))
+
+    def vertex_layout(self, frame):
+        t = dev.torch()
+        xc, yc = frame[self.x], frame[self.y]
+        lx = xc.starts[1:] - xc.starts[:-1]
+        ly = yc.starts[1:] - yc.starts[:-1]
+        lens = t.minimum(lx, ly)
+        nrows = int(lens.numel())
+        width = max(2, int(lens.max().item()) if nrows else 2)
+        dtype = xc.flat.dtype if xc.flat.dtype == yc.flat.dtype == t.float32 else t.float64
+
+        def padded(col, lengths):
+            out = t.full((nrows, width), float("nan"), dtype=dtype, device=col.flat.device)
+            if col.flat.numel():
+                row = t.repeat_interleave(t.arange(nrows, device=col.flat.device), lengths)
+                pos = t.arange(col.flat.numel(), device=col.flat.device) - col.starts[row]
+                keep = pos < lens[row]
+                out[row[keep], pos[keep]] = col.flat[keep].to(dtype)
+            return out
+
+        X, Y = padded(xc, lx), padded(yc, ly)
+        return dict(nlines=nrows, nverts=width, x=X, y=Y, xls=width, xvs=1, yls=width, yvs=1,
+                    axis=1)

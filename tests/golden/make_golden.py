@@ -525,6 +525,19 @@ def api_line_cases():
                      [2, 0, 0, 0, 0, 0, 0, 0, 2], [0, 1, 0, 0, 0, 0, 0, 1, 0],
                      [0, 0, 1, 0, 0, 0, 1, 0, 0], [0, 0, 0, 1, 0, 1, 0, 0, 0],
                      [0, 0, 0, 0, 1, 0, 0, 0, 0]])
+    add("test_line_manual_range/axis1_ragged", TP + ":1703-1707",
+        {"x": {"ragged": [[4, 0], [0, -4, 0, 4]], "dtype": "f4"},
+         "y": {"ragged": [[0, -4], [-4, 0, 4, 0]], "dtype": "f4"}},
+        {"x": "x", "y": "y", "axis": 1, "agg": {"kind": "count", "column": None}}, c7, sol7)
+    add("test_line_autorange_axis1_ragged", TP + ":1935-1967",
+        {"x": {"ragged": [[4, 0], [0, -4, 0, 4]], "dtype": "f4"},
+         "y": {"ragged": [[0, -4], [-4, 0, 4, 0]], "dtype": "f4"}},
+        {"x": "x", "y": "y", "axis": 1, "agg": {"kind": "count", "column": None}}, c9,
+        np.array([[0, 0, 0, 0, 2, 0, 0, 0, 0], [0, 0, 0, 1, 0, 1, 0, 0, 0],
+                  [0, 0, 1, 0, 0, 0, 1, 0, 0], [0, 1, 0, 0, 0, 0, 0, 1, 0],
+                  [1, 0, 0, 0, 0, 0, 0, 0, 2], [0, 1, 0, 0, 0, 0, 0, 1, 0],
+                  [0, 0, 1, 0, 0, 0, 1, 0, 0], [0, 0, 0, 1, 0, 1, 0, 0, 0],
+                  [0, 0, 0, 0, 1, 0, 0, 0, 0]]), x_range=[-4, 4], y_range=[-4, 4])
     add("test_line_autorange_axis1_x_constant", TP + ":1863-1895",
         {"y0": [0, 0], "y1": [-4, 4], "y2": [0, 0]},
         {"x": {"ndarray": [-4, 0, 4]}, "y": ["y0", "y1", "y2"], "axis": 1,

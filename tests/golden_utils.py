@@ -45,7 +45,15 @@ def frame(name):
 
 
 def line_frame(spec):
-    cols = {k: dec(v, float) for k, v in spec.items()}
+    """The frame of a golden line case; ``{"ragged": rows, "dtype": d}`` columns become
+    object columns of per-row arrays (one ragged value per row)."""
+    cols = {}
+    for k, v in spec.items():
+        if isinstance(v, dict) and "ragged" in v:
+            cols[k] = pd.Series([np.asarray(r, dtype=v["dtype"]) for r in v["ragged"]],
+                                dtype=object)
+        else:
+            cols[k] = dec(v, float)
     return pd.DataFrame(cols)
 
 
@@ -241,6 +249,14 @@ def line_polylines(case_kwargs, df):
         y = [y] * len(x)
     if isinstance(y, list) and isinstance(x, str):
         x = [x] * len(y)
+    if axis == 1 and isinstance(x, str) and isinstance(y, str):
+        # ragged rows: row i joins its first min(len x_i, len y_i) vertices
+        xs, ys = [], []
+        for xr_, yr_ in zip(df[x], df[y]):
+            m = min(len(xr_), len(yr_))
+            xs.append(np.asarray(xr_[:m], float))
+            ys.append(np.asarray(yr_[:m], float))
+        return xs, ys, "ragged"
     if axis == 0:
         if isinstance(x, str):
             return [df[x].to_numpy(float)], [df[y].to_numpy(float)], "axis0"
@@ -267,12 +283,17 @@ def line_case_oracle(case):
         mins, maxs = zip(*[oracle.bounds(a) for a in arrs])
         return maybe_expand((min(mins), max(maxs)))
 
+    if layout == "ragged":
+        xs_all = [np.concatenate([np.asarray(r, float) for r in df[kw["x"]]])]
+        ys_all = [np.concatenate([np.asarray(r, float) for r in df[kw["y"]]])]
+    else:
+        xs_all, ys_all = xs, ys
     xr_ = cv["x_range"] or (maybe_expand((np.nanmin(kw["x"]["ndarray"]),
                                          np.nanmax(kw["x"]["ndarray"])))
-                            if isinstance(kw["x"], dict) else bnd(xs))
+                            if isinstance(kw["x"], dict) else bnd(xs_all))
     yr_ = cv["y_range"] or (maybe_expand((np.nanmin(kw["y"]["ndarray"]),
                                          np.nanmax(kw["y"]["ndarray"])))
-                            if isinstance(kw["y"], dict) else bnd(ys))
+                            if isinstance(kw["y"], dict) else bnd(ys_all))
     view = view_for(W, H, xr_, yr_)
     agg = kw["agg"]
     col = agg.get("column")
@@ -284,6 +305,13 @@ def line_case_oracle(case):
         total = None
         for xl, yl in zip(xs, ys):
             r = oracle.line_axis0(xl, yl, view, W, H, (base,), plot_start=True, val=val)[base]
+            total = r.astype(float) if total is None else _merge(kind, total, r)
+        return total, (xr_, yr_)
+    if layout == "ragged":
+        total = None
+        for i, (xl, yl) in enumerate(zip(xs, ys)):
+            v1 = None if val is None else val[i:i + 1]
+            r = oracle.lines_axis1(xl[None, :], yl[None, :], view, W, H, (base,), val=v1)[base]
             total = r.astype(float) if total is None else _merge(kind, total, r)
         return total, (xr_, yr_)
     r = oracle.lines_axis1(np.array(xs), np.array(ys), view, W, H, (base,), val=val)[base]

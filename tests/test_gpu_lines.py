@@ -251,3 +251,45 @@ def test_line_value_reductions_vs_oracle():
     G.assert_same(agg["mn"].values, r["min"])
     G.assert_same(agg["mx"].values, r["max"])
     assert has.sum() > 100
+
+
+@pytest.mark.parametrize("dt", ["f4", "f8"])
+def test_ragged_lines_vs_oracle(dt):
+    """Ragged rows of random lengths (x and y lengths differ per row, NaN gaps inside):
+    count and max against the oracle drawing each row's first min(len x, len y)
+    vertices, with autorange over every stored value."""
+    rng = np.random.default_rng(21)
+    rows = 1500
+    xs, ys, vs = [], [], rng.uniform(0, 1, rows)
+    for _ in range(rows):
+        lx = int(rng.integers(1, 40))
+        ly = lx + int(rng.integers(-3, 4))
+        ly = max(1, ly)
+        x = np.cumsum(rng.normal(0, 0.3, lx)).astype(dt)
+        y = np.cumsum(rng.normal(0, 0.3, ly)).astype(dt)
+        if lx > 5 and rng.random() < 0.2:
+            x[int(rng.integers(1, lx - 1))] = np.nan
+        xs.append(x)
+        ys.append(y)
+    df = pd.DataFrame({"x": pd.Series(xs, dtype=object), "y": pd.Series(ys, dtype=object),
+                       "v": vs})
+    W, H = 211, 157
+    for agg, kind in ((dsa.count(), "count"), (dsa.max("v"), "max")):
+        got = dsa.Canvas(W, H).line(df, "x", "y", agg=agg, axis=1)
+        case = {"frame": None, "kwargs": {"x": "x", "y": "y", "axis": 1,
+                                          "agg": {"kind": kind,
+                                                  "column": None if kind == "count" else "v"}},
+                "canvas": {"W": W, "H": H, "x_range": None, "y_range": None}}
+        want, (xr_, yr_) = _oracle_frame_case(case, df)
+        assert np.allclose(got.attrs["x_range"], xr_) and np.allclose(got.attrs["y_range"], yr_)
+        G.assert_same(got.values, want)
+
+
+def _oracle_frame_case(case, df):
+    """G.line_case_oracle for an in-memory frame."""
+    orig = G.line_frame
+    G.line_frame = lambda spec: df
+    try:
+        return G.line_case_oracle(case)
+    finally:
+        G.line_frame = orig
