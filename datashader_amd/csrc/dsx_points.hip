@@ -456,24 +456,19 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
             int32_t px = 0, py = 0, pc = 0;
             bool keep;
             if constexpr (LIN) {
-                // locate_xy without branches: every test is evaluated and and-ed; the
-                // projection is clamped before the conversion so that a NaN or
-                // out-of-range coordinate converts to a defined value.  The clamp keeps
-                // every value that can survive the tests below (wraparound included:
-                // anything under -W is dropped either way)
+                // locate_xy without branches, for a view whose translation is exactly
+                // -xmin*sx, -ymin*sy (checked on the host): an in-range row then projects
+                // into [0, W] -- x >= xmin gives fl(x*sx) >= fl(xmin*sx) -- so the
+                // wraparound and the lower bound never apply; a row out of range (NaN
+                // included) converts a harmless 0 and is dropped by `inr`
                 const double x = (double)cur.x[it], y = (double)cur.y[it];
                 const bool inr = (a.xmin <= x) & (x <= a.xmax) & (a.ymin <= y) & (y <= a.ymax);
-                const double fx = fmin(fmax(__dadd_rn(__dmul_rn(x, a.sx), a.tx), -(double)a.W - 4.0), 2.0e9);
-                const double fy = fmin(fmax(__dadd_rn(__dmul_rn(y, a.sy), a.ty), -(double)a.H - 4.0), 2.0e9);
-                int32_t xx = (int32_t)fx, yy = (int32_t)fy;
-                xx = xx >= a.W ? a.W - 1 : xx;
-                yy = yy >= a.H ? a.H - 1 : yy;
-                xx += xx < 0 ? a.W : 0;  // numba's negative-index wraparound
-                yy += yy < 0 ? a.H : 0;
-                keep = (j < rem) & inr & ((uint32_t)xx < (uint32_t)a.W) &
-                       ((uint32_t)yy < (uint32_t)a.H);
-                px = xx;
-                py = yy;
+                const double fx = inr ? __dadd_rn(__dmul_rn(x, a.sx), a.tx) : 0.0;
+                const double fy = inr ? __dadd_rn(__dmul_rn(y, a.sy), a.ty) : 0.0;
+                const int32_t xx = (int32_t)fx, yy = (int32_t)fy;
+                px = xx < a.W - 1 ? xx : a.W - 1;
+                py = yy < a.H - 1 ? yy : a.H - 1;
+                keep = (j < rem) & inr;
             } else {
                 keep = j < rem && locate_xy<TC>(a, cur.x[it], cur.y[it], px, py);
             }
@@ -485,7 +480,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
             const bool vnan = HAS_VAL ? (v != v) : true;
             const bool inw = keep && in_window(w, px, py) && !(HAS_VAL && s.spill_nan && vnan);
             if (inw) {
-                const uint32_t lc = (uint32_t)(((py - w.y0) * w.w + (px - w.x0)) * ncat + pc);
+                // 24-bit multiplies (full rate): window and canvas sides are < 2^24
+                const uint32_t lc = CK == CK_NONE
+                    ? __umul24((uint32_t)(py - w.y0), (uint32_t)w.w) + (uint32_t)(px - w.x0)
+                    : (uint32_t)(((py - w.y0) * w.w + (px - w.x0)) * ncat + pc);
                 double add = v;
                 if constexpr (SQDEV) {
                     const int64_t gc = ((int64_t)py * a.W + px) * ncat + pc;
@@ -499,8 +497,9 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
             const unsigned long long m = __ballot(spill);
             if (spill) {
                 // cells < 2^32 (checked on the host), so 32-bit arithmetic is exact
-                const uint32_t cell =
-                    ((uint32_t)py * (uint32_t)a.W + (uint32_t)px) * (uint32_t)ncat + (uint32_t)pc;
+                const uint32_t cell = CK == CK_NONE
+                    ? __umul24((uint32_t)py, (uint32_t)a.W) + (uint32_t)px
+                    : ((uint32_t)py * (uint32_t)a.W + (uint32_t)px) * (uint32_t)ncat + (uint32_t)pc;
                 const uint32_t slot = staged + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                 st_cell[slot] = cell;
                 if constexpr (HAS_VAL) st_val[slot] = cur.v[it];
@@ -1153,7 +1152,11 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     else if (a.cat.dtype == DSX_I8 || a.cat.dtype == DSX_U8)
         scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_BYTE>;
     // the common plane sets over linear axes, without categorizer: compile-time planes
-    if (a.cat.kind == DSX_CAT_NONE && !a.xlog && !a.ylog) {
+    // (the linear fast path also needs the reference's own translation, t = -start * s,
+    // under which in-range rows cannot project below pixel 0)
+    const bool origin = a.sx > 0 && a.sy > 0 && a.tx == -(a.xmin * a.sx) &&
+                        a.ty == -(a.ymin * a.sy);
+    if (a.cat.kind == DSX_CAT_NONE && !a.xlog && !a.ylog && origin) {
         const uint32_t wf = P.wl.flags;
         if constexpr (!HAS_VAL) {
             if (wf == PL_CNT) scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE, true, PL_CNT>;

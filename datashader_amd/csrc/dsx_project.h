@@ -52,6 +52,7 @@ __device__ __forceinline__ bool locate(const PointArgs &a, int64_t i, TC xc, TC 
 static int check_points(const dsx_points_desc *d, const dsx_view *v) {
     if (!d || !v) return DSX_ERR_NULL;
     if (d->n < 0 || v->width <= 0 || v->height <= 0) return DSX_ERR_SHAPE;
+    if (v->width >= (1 << 24) || v->height >= (1 << 24)) return DSX_ERR_SHAPE;  // 24-bit math
     if (d->n >= (1ll << 32)) return DSX_ERR_SHAPE;  // spill positions are 32-bit per tile
     if (d->n > 0 && (!d->x || !d->y)) return DSX_ERR_NULL;
     if (d->xy_dtype != DSX_F32 && d->xy_dtype != DSX_F64) return DSX_ERR_DTYPE;
