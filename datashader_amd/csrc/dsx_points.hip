@@ -34,7 +34,7 @@ constexpr int kItems = 4;
 constexpr int kChunk = kScanThreads * kItems;  // rows per workgroup iteration
 constexpr int kLdsBytes = 160 * 1024;           // LDS of one gfx950 workgroup
 constexpr int kCtrlBytes = 64;
-constexpr int kFlush = 32;                      // spill entries a wave writes per store
+constexpr int kFlush = 64;                      // spill entries a wave writes per store
 constexpr int kStage = kFlush + 64;             // staged spill entries per wave in LDS
 static_assert(kStage >= (kFlush - 1) + 64, "a wave's staging holds a partial line + one item");
 constexpr int kPartThreads = 1024;
