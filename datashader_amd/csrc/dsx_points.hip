@@ -38,12 +38,14 @@ constexpr int kFlush = 64;                      // spill entries a wave writes p
 constexpr int kStage = kFlush + 64;             // staged spill entries per wave in LDS
 static_assert(kStage >= (kFlush - 1) + 64, "a wave's staging holds a partial line + one item");
 constexpr int kPartThreads = 1024;
-constexpr int kTileThreads = 256;
-// target LDS of one tile workgroup: a quarter of the CU's LDS less one 512-byte
-// allocation granule, so that four tile workgroups fit on a CU
-constexpr int kTileLds = 39 * 1024;
+constexpr int kTileThreads = 512;
+// largest LDS of one tile workgroup: half of the CU's LDS less a 2 KB margin, so that two
+// tile workgroups always fit on a CU (three when the power-of-two tile takes <= 52 KB).
+// Larger tiles mean fewer tiles, so the scatter writes longer runs per tile.
+constexpr int kTileLds = 78 * 1024;
 constexpr int kMaxTiles = 4096;
 constexpr int kMinPart = 4096;                  // fewest spill entries a tile part is given
+constexpr int kScatterPad = 8192;               // slack entries after the spill sub-segments
 constexpr int kSelBins = 8192;                  // coarse density histogram bins
 constexpr int kSampleRuns = 256;
 constexpr int kSampleRun = 4096;
@@ -449,7 +451,8 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
     auto step = [&](R &cur, R &far) {
         far.load(a, ck + 2 * (int64_t)gridDim.x, tid);
         const int64_t rem64 = a.n - ck * kChunk;  // rows of this chunk, uniform
-        const uint32_t rem = rem64 >= kChunk ? (uint32_t)kChunk : (uint32_t)rem64;
+        const uint32_t rem = rem64 >= kChunk ? (uint32_t)kChunk
+                                             : (rem64 > 0 ? (uint32_t)rem64 : 0u);
 #pragma unroll
         for (int it = 0; it < kItems; ++it) {
             const uint32_t j = (uint32_t)(it * kScanThreads + tid);
@@ -534,11 +537,12 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
         }
         ck += gridDim.x;
     };
+    // no exit between the three steps: a loop edge that skips steps makes the compiler's
+    // wait counts assume fewer loads in flight, and it then drains the prefetch once per
+    // iteration.  Steps past the last chunk load the last row and keep nothing (rem = 0).
     while (ck < nchunks) {
         step(ra, rc);
-        if (ck >= nchunks) break;
         step(rb, ra);
-        if (ck >= nchunks) break;
         step(rc, rb);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -652,8 +656,9 @@ struct PartArgs {
     int32_t ntiles;
     uint32_t *hist;                   // [G][ntiles]
     unsigned long long *tile_start;   // [ntiles + 1]
-    uint32_t *sorted_cell;
-    void *sorted_val;
+    uint32_t *sorted_cell;            // [sink + 64 * G]: the sorted spill, then one
+    void *sorted_val;                 // 64-slot sink line per scatter workgroup
+    size_t sink;
     // tile work split: tile t is processed by parts [part_start[t], part_start[t + 1]),
     // over which tile t's entries are split evenly (*part_len: the largest part size);
     // done[t] counts finished parts of a split tile
@@ -750,127 +755,144 @@ __global__ __launch_bounds__(kPartThreads) void spill_tilescan_kernel(PartArgs p
     if (threadIdx.x == 0) p.part_start[p.ntiles] = (uint32_t)pcarry;
 }
 
-// Scatter: partition group g walks the kWaves sub-segments of scan workgroup g as one
-// concatenated sequence in sub-chunks, counting-sorts each sub-chunk by tile in LDS and
-// writes it out run by run (coalesced within runs).
-template <typename TV>
-struct ScatterSize {  // entries per LDS round: staging fits LDS next to the tile arrays
-    static constexpr int value = sizeof(TV) == 8 ? 8192 : 12288;
+// Scatter: workgroup g sorts the spill of scan workgroup g by tile.  Wave w reads
+// sub-segment w (written by scan wave w), kWv entries a round; each round is counting-
+// sorted by tile in LDS (LDS counters rank the entries, one block scan over the tiles
+// places them) and written out in sorted order, so a tile's entries of the round leave
+// as one run.  All rounds are alike: every load is issued (past the end of a sub-segment
+// it re-reads the last entry) and every thread writes exactly kPer entries (slots past
+// the round's count go to the workgroup's own sink line after the sorted spill), so the
+// compiler counts the loads and stores in flight exactly, and the next round's loads (two
+// register sets alternate) stay in flight behind this round's stores.
+template <typename TV, bool HAS_VAL>
+struct ScatterCfg {  // entries per round: cell + value staging fits LDS next to the tile arrays
+    static constexpr int kSub = !HAS_VAL ? 16384 : (sizeof(TV) == 8 ? 8192 : 12288);
+    static constexpr int kPer = kSub / kPartThreads;  // per thread
+    static constexpr int kWv = kSub / kWaves;         // per wave
+};
+constexpr int kTpt = kMaxTiles / kPartThreads;         // tiles per thread in the block scan
+
+template <typename TV, bool HAS_VAL>
+struct ScatterRound {
+    uint32_t c[ScatterCfg<TV, HAS_VAL>::kPer];
+    TV v[ScatterCfg<TV, HAS_VAL>::kPer];
 };
 
 template <typename TV, bool HAS_VAL>
 __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p) {
-    constexpr int kSub = HAS_VAL ? ScatterSize<TV>::value : 12288;
-    __shared__ uint32_t cur[kMaxTiles];    // next slot of this group in each tile
-    __shared__ uint32_t cnt[kMaxTiles];    // sub-chunk counts, then sub-chunk starts
+    using C = ScatterCfg<TV, HAS_VAL>;
+    constexpr int kSub = C::kSub, kPer = C::kPer, kWv = C::kWv;
+    static_assert(kPer * kPartThreads == kSub && kWv % 64 == 0, "round shape");
+    static_assert(kTpt * kPartThreads == kMaxTiles, "tile split");
+    static_assert(3 * kWv <= kScatterPad, "loads past a sub-segment stay in the slack");
+    __shared__ uint32_t cnt[kMaxTiles];  // round counts, then the round start of each tile
+    __shared__ uint32_t wb[kMaxTiles];   // sorted position of round slot 0, per tile
     __shared__ uint32_t s_cell[kSub];
-    __shared__ uint16_t s_tile[kSub];
     __shared__ TV s_val[HAS_VAL ? kSub : 1];
     __shared__ unsigned long long sh[kPartThreads / 64 + 1];
-    __shared__ unsigned long long pre[kWaves + 1];  // sub-segment prefix lengths
-    const int g = blockIdx.x;
-    // cur[t]: where this group's next entry of tile t goes in the sorted spill (spill
-    // positions fit 32 bits: the host rejects inputs of 2^32 rows or more)
-    for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x) {
-        cur[t] = (uint32_t)(p.tile_start[t] + p.hist[(size_t)g * p.ntiles + t]);
+    __shared__ uint32_t s_rounds;
+    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar bases
+    // tiles [tid * kTpt, tid * kTpt + kTpt): where group g's next entry of each goes in
+    // the sorted spill (positions fit 32 bits: the host rejects 2^32 rows or more)
+    uint32_t cur[kTpt];
+#pragma unroll
+    for (int i = 0; i < kTpt; ++i) {
+        const int t = tid * kTpt + i;
+        cur[i] = t < p.ntiles ? (uint32_t)(p.tile_start[t] + p.hist[(size_t)g * p.ntiles + t]) : 0u;
         cnt[t] = 0u;
     }
-    if (threadIdx.x == 0) {
-        unsigned long long run = 0;
-        for (int wv = 0; wv < kWaves; ++wv) {
-            pre[wv] = run;
-            unsigned long long len = p.seg_count[(size_t)g * kWaves + wv];
-            if (len > (unsigned long long)p.seg_cap) len = (unsigned long long)p.seg_cap;
-            run += len;
-        }
-        pre[kWaves] = run;
-    }
+    uint32_t len = p.seg_count[(size_t)g * kWaves + wave];  // my wave's sub-segment
+    if ((int64_t)len > p.seg_cap) len = (uint32_t)p.seg_cap;
+    if (tid == 0) s_rounds = 0u;
     __syncthreads();
-    const unsigned long long total = pre[kWaves];
-    if (total == 0) return;
-    const unsigned long long base = (unsigned long long)g * kWaves * (unsigned long long)p.seg_cap;
-    const TV *val = reinterpret_cast<const TV *>(p.val);
-    constexpr int kPer = kSub / kPartThreads;
-    // entries of one round in registers; every load is issued (an index past the end
-    // re-reads the last entry), so the next round's loads stay in flight behind this one
-    struct Round {
-        uint32_t rc[kPer];
-        TV rv[kPer];
-    };
-    auto load_round = [&](Round &r, unsigned long long c0) {
+    if (lane == 0) atomicMax(&s_rounds, (len + kWv - 1) / (uint32_t)kWv);
+    __syncthreads();
+    const uint32_t rounds = s_rounds;
+    if (rounds == 0) return;  // uniform: this group spilled nothing
+    // Loads run past the end of a sub-segment into the next one, and past the last one
+    // into kScatterPad slack entries (never used: masked by len).  Sub-segments are
+    // shorter than 2^31 entries (rows per wave), so int32 round arithmetic is exact.
+    const size_t seg = ((size_t)g * kWaves + wave) * (size_t)p.seg_cap;
+    const uint32_t *cellp = p.cell + seg;
+    const TV *valp = reinterpret_cast<const TV *>(p.val) + (HAS_VAL ? seg : 0);
+    TV *sval = reinterpret_cast<TV *>(p.sorted_val);
+    const size_t sink = p.sink + (size_t)g * 64 + lane;
+    using R = ScatterRound<TV, HAS_VAL>;
+    auto load = [&](R &x, int32_t r) {  // one vector address, immediate offsets
+        const uint32_t *q = cellp + ((size_t)r * kWv + lane);
+        const TV *qv = valp + (HAS_VAL ? (size_t)r * kWv + lane : 0);
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            unsigned long long v = c0 + (unsigned long long)(k * kPartThreads + threadIdx.x);
-            v = v < total ? v : total - 1;
-            int wv = 0;  // the last sub-segment starting at or before v (pre is sorted)
-#pragma unroll
-            for (int step = kWaves / 2; step >= 1; step >>= 1)
-                if (pre[wv + step] <= v) wv += step;
-            const unsigned long long j = base + (unsigned long long)wv * p.seg_cap + (v - pre[wv]);
-            r.rc[k] = p.cell[j];
-            if constexpr (HAS_VAL) r.rv[k] = val[j];
-            else r.rv[k] = TV(0);
+            x.c[k] = q[k * 64];
+            if constexpr (HAS_VAL) x.v[k] = qv[k * 64];
+            else x.v[k] = TV(0);
         }
     };
-    Round nxt;
-    load_round(nxt, 0);
-    for (unsigned long long c0 = 0; c0 < total; c0 += kSub) {
-        Round r = nxt;
-        if (c0 + kSub < total) load_round(nxt, c0 + kSub);
-        uint32_t rt[kPer], rr[kPer];
-        const uint32_t *rc = r.rc;
-        const TV *rv = r.rv;
+    auto round = [&](const R &x, int32_t r) {
+        const int32_t lim = (int32_t)len - r * kWv;  // entries of this round left to my wave
+        uint32_t rr[kPer];
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const unsigned long long v = c0 + (unsigned long long)(k * kPartThreads + threadIdx.x);
-            rt[k] = 0xFFFFFFFFu;
-            rr[k] = 0u;
-            if (v < total) {
-                const uint32_t t = rc[k] >> p.tile_shift;
-                if (t < (uint32_t)p.ntiles) {  // always, for entries the scan wrote
-                    rt[k] = t;
-                    rr[k] = atomicAdd(&cnt[t], 1u);
-                }
+        for (int k = 0; k < kPer; ++k) {  // rank within the tile
+            rr[k] = 0xFFFFFFFFu;
+            const uint32_t t = x.c[k] >> p.tile_shift;
+            if (lane < lim - k * 64 && t < (uint32_t)p.ntiles)  // always in range, for entries the scan wrote
+                rr[k] = atomicAdd(&cnt[t], 1u);
+        }
+        __syncthreads();
+        uint32_t v[kTpt], sum = 0u;
+#pragma unroll
+        for (int i = 0; i < kTpt; ++i) {
+            v[i] = cnt[tid * kTpt + i];  // 0 past the last tile
+            sum += v[i];
+        }
+        unsigned long long total;
+        uint32_t run = (uint32_t)block_exscan(sum, sh, total);
+#pragma unroll
+        for (int i = 0; i < kTpt; ++i) {
+            const int t = tid * kTpt + i;
+            cnt[t] = run;
+            wb[t] = cur[i] - run;  // modular: wb[t] + slot is the entry's sorted position
+            cur[i] += v[i];
+            run += v[i];
+        }
+        const uint32_t nsub = (uint32_t)total;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {  // counting sort into LDS
+            if (rr[k] != 0xFFFFFFFFu) {
+                const uint32_t slot = cnt[x.c[k] >> p.tile_shift] + rr[k];
+                s_cell[slot] = x.c[k];
+                if constexpr (HAS_VAL) s_val[slot] = x.v[k];
             }
         }
         __syncthreads();
-        // exclusive scan of the sub-chunk counts -> sub-chunk start of each tile
-        unsigned long long carry = 0;
-        for (int t0 = 0; t0 < p.ntiles; t0 += blockDim.x) {
-            const int t = t0 + threadIdx.x;
-            const unsigned long long v = t < p.ntiles ? cnt[t] : 0ull;
-            unsigned long long tot;
-            const unsigned long long ex = block_exscan(v, sh, tot);
-            if (t < p.ntiles) cnt[t] = (uint32_t)(carry + ex);
-            carry += tot;
-        }
-        __syncthreads();
+        // a fresh copy of tid per round: without it the compiler hoists all kPer slot
+        // indices out of the round loop and spills registers to hold them
+        int32_t me;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(me) : "v"(tid));
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            if (rt[k] != 0xFFFFFFFFu) {
-                const uint32_t slot = cnt[rt[k]] + rr[k];
-                s_cell[slot] = rc[k];
-                s_tile[slot] = (uint16_t)rt[k];
-                if constexpr (HAS_VAL) s_val[slot] = rv[k];
-            }
+        for (int k = 0; k < kPer; ++k) {  // write out: kPer stores per thread, always
+            const uint32_t j = (uint32_t)(k * kPartThreads + me);
+            const bool ok = me < (int32_t)nsub - k * kPartThreads;
+            const uint32_t c = s_cell[j];
+            const uint32_t t = ok ? c >> p.tile_shift : 0u;
+            const size_t dst = ok ? (size_t)(wb[t] + j) : sink;
+            p.sorted_cell[dst] = c;
+            if constexpr (HAS_VAL) sval[dst] = s_val[j];
+            __builtin_amdgcn_sched_barrier(0);  // one entry at a time: bounded registers
         }
+#pragma unroll
+        for (int i = 0; i < kTpt; ++i) cnt[tid * kTpt + i] = 0u;
         __syncthreads();
-        const uint32_t nsub = (uint32_t)((total - c0) < (unsigned long long)kSub ? (total - c0) : kSub);
-        for (uint32_t j = threadIdx.x; j < nsub; j += blockDim.x) {
-            const uint32_t t = s_tile[j];
-            const uint32_t dst = cur[t] + (j - cnt[t]);
-            p.sorted_cell[dst] = s_cell[j];
-            if constexpr (HAS_VAL) reinterpret_cast<TV *>(p.sorted_val)[dst] = s_val[j];
-        }
-        __syncthreads();
-        // advance the group's cursors by this sub-chunk's counts; reset the counts
-        for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x) {
-            const uint32_t next = (t + 1 < p.ntiles) ? cnt[t + 1] : nsub;
-            cur[t] += next - cnt[t];
-        }
-        __syncthreads();
-        for (int t = threadIdx.x; t < p.ntiles; t += blockDim.x) cnt[t] = 0u;
-        __syncthreads();
+    };
+    R ra, rb;
+    load(ra, 0);
+    for (int32_t r = 0; r < (int32_t)rounds; r += 2) {  // an odd count ends with an empty round
+        load(rb, r + 1);
+        round(ra, r);
+        load(ra, r + 2);
+        round(rb, r + 1);
     }
 }
 
@@ -949,11 +971,9 @@ __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay 
             }
             j0 += kStep;
         };
-        while (j0 < e) {
+        while (j0 < e) {  // steps past e load entry b and add nothing
             step(ra, rc);
-            if (j0 >= e) break;
             step(rb, ra);
-            if (j0 >= e) break;
             step(rc, rb);
         }
     }
@@ -1084,13 +1104,14 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     P.o_hist_sel = o; o = align_up(o + 4 * (size_t)kSelBins, 256);
     P.o_count = o; o = align_up(o + 4 * (size_t)P.scan_groups * kWaves, 256);  // sub-segment lengths
     P.o_partials = o; o = align_up(o + (size_t)P.scan_groups * P.wl.bytes, 256);
-    P.o_spill_cell = o; o = align_up(o + 4 * nseg, 256);
-    P.o_spill_val = o; o = align_up(o + (size_t)P.vbytes * nseg, 256);
+    P.o_spill_cell = o; o = align_up(o + 4 * (nseg + kScatterPad), 256);
+    P.o_spill_val = o; o = align_up(o + (size_t)P.vbytes * (nseg + kScatterPad), 256);
     P.o_hist = o; o = align_up(o + 4 * (size_t)P.scan_groups * P.ntiles, 256);
     P.o_totals = o; o = align_up(o + 8 * (size_t)P.ntiles, 256);
     P.o_tile_start = o; o = align_up(o + 8 * (size_t)(P.ntiles + 1), 256);
-    P.o_sorted_cell = o; o = align_up(o + 4 * (size_t)n, 256);
-    P.o_sorted_val = o; o = align_up(o + (size_t)P.vbytes * n, 256);
+    const size_t nsorted = (size_t)n + 64 * (size_t)P.scan_groups;  // + the scatter's sink lines
+    P.o_sorted_cell = o; o = align_up(o + 4 * nsorted, 256);
+    P.o_sorted_val = o; o = align_up(o + (size_t)P.vbytes * nsorted, 256);
     P.max_parts = 2 * P.ntiles + 1;
     P.o_part_start = o; o = align_up(o + 4 * (size_t)(P.ntiles + 1), 256);
     P.o_part_len = o; o = align_up(o + 4, 256);
@@ -1193,6 +1214,7 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     pa.tile_start = reinterpret_cast<unsigned long long *>(ws + P.o_tile_start);
     pa.sorted_cell = reinterpret_cast<uint32_t *>(ws + P.o_sorted_cell);
     pa.sorted_val = ws + P.o_sorted_val;
+    pa.sink = (size_t)(d->n < 1 ? 1 : d->n);
     pa.part_start = reinterpret_cast<uint32_t *>(ws + P.o_part_start);
     pa.part_len = reinterpret_cast<uint32_t *>(ws + P.o_part_len);
     pa.part_tile = reinterpret_cast<uint32_t *>(ws + P.o_part_tile);
