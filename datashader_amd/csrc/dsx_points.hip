@@ -398,10 +398,11 @@ struct Rows {
         for (int it = 0; it < kItems; ++it) {
             uint32_t j = (uint32_t)(it * kScanThreads + tid);
             j = j < lastj ? j : lastj;
-            x[it] = X[j];
-            y[it] = Y[j];
-            if constexpr (HAS_VAL) v[it] = V[j];
+            x[it] = stream_load(X + j);
+            y[it] = stream_load(Y + j);
+            if constexpr (HAS_VAL) v[it] = stream_load(V + j);
             else v[it] = TV(0);
+            // (a byte column's 64-byte wave reads share cache lines between waves: cached)
             if constexpr (CK == CK_BYTE) c[it] = reinterpret_cast<const uint8_t *>(a.cat.col)[b + j];
             else if constexpr (CK == CK_ANY) c[it] = load_cat_raw(a.cat, b + j);
             else c[it] = 0;
@@ -820,8 +821,12 @@ __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p)
     const size_t sink = p.sink + (size_t)g * 64 + lane;
     using R = ScatterRound<TV, HAS_VAL>;
     auto load = [&](R &x, int32_t r) {  // one vector address, immediate offsets
-        const uint32_t *q = cellp + ((size_t)r * kWv + lane);
-        const TV *qv = valp + (HAS_VAL ? (size_t)r * kWv + lane : 0);
+        // a round wholly past my sub-segment (the prefetch after the last round, an
+        // empty round) re-reads the first kWv entries of the list, which stay cached
+        const bool live = r * kWv < (int32_t)len;
+        const uint32_t *q = live ? cellp + ((size_t)r * kWv + lane) : p.cell + lane;
+        const TV *qv = live ? valp + (HAS_VAL ? (size_t)r * kWv + lane : 0)
+                            : reinterpret_cast<const TV *>(p.val) + (HAS_VAL ? lane : 0);
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             x.c[k] = q[k * 64];
@@ -1610,12 +1615,12 @@ __global__ __launch_bounds__(256) void rowindex_kernel(RiArgs r) {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
         const int64_t i = r.mode == DSX_RI_LAST ? a.n - 1 - k : k;
         int32_t px, py, pc;
-        if (!locate<TC>(a, i, reinterpret_cast<const TC *>(a.x)[i],
-                        reinterpret_cast<const TC *>(a.y)[i], px, py, pc))
+        if (!locate<TC>(a, i, stream_load(reinterpret_cast<const TC *>(a.x) + i),
+                        stream_load(reinterpret_cast<const TC *>(a.y) + i), px, py, pc))
             continue;
         const int64_t cell = ((int64_t)py * a.W + px) * a.cat.ncat + pc;
         if (r.nan_check) {
-            const TV v = reinterpret_cast<const TV *>(a.val)[i];
+            const TV v = stream_load(reinterpret_cast<const TV *>(a.val) + i);
             if (v != v) continue;
             if (r.mode == DSX_RI_ARGMAX || r.mode == DSX_RI_ARGMIN) {
                 if constexpr (sizeof(TV) == 4) {
