@@ -1608,36 +1608,70 @@ struct RiArgs {
     unsigned long long *index;
 };
 
-template <typename TC, typename TV>
+// Each thread takes kRiU rows per iteration (a grid stride apart) and issues the loads
+// of all of them before it waits for any: the argmax / argmin pass is bound by the
+// latency of its dependent reads (column -> cell -> key plane -> index plane), so rows in
+// flight is what counts (where(max) 1.36 -> 1.06 ms on 100M rows).  first / last keep
+// one row per iteration: their skip test works best with the fewest rows in flight
+// (the rows in flight before a cell is first set all take the atomic; 4 rows per
+// iteration measured 1.15 -> 1.60 ms).  The result does not depend on the order rows
+// are visited in (atomic min / max).
+template <typename TC, typename TV, int kRiU>
 __global__ __launch_bounds__(256) void rowindex_kernel(RiArgs r) {
     const PointArgs &a = r.a;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.n; k += stride) {
-        const int64_t i = r.mode == DSX_RI_LAST ? a.n - 1 - k : k;
-        int32_t px, py, pc;
-        if (!locate<TC>(a, i, stream_load(reinterpret_cast<const TC *>(a.x) + i),
-                        stream_load(reinterpret_cast<const TC *>(a.y) + i), px, py, pc))
-            continue;
-        const int64_t cell = ((int64_t)py * a.W + px) * a.cat.ncat + pc;
-        if (r.nan_check) {
-            const TV v = stream_load(reinterpret_cast<const TV *>(a.val) + i);
-            if (v != v) continue;
-            if (r.mode == DSX_RI_ARGMAX || r.mode == DSX_RI_ARGMIN) {
+    const bool last = r.mode == DSX_RI_LAST;
+    const bool arg = r.mode == DSX_RI_ARGMAX || r.mode == DSX_RI_ARGMIN;
+    for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k0 < a.n;
+         k0 += kRiU * stride) {
+        int64_t row[kRiU];
+        bool ok[kRiU];
+        TC xs[kRiU], ys[kRiU];
+        TV vs[kRiU];
+#pragma unroll
+        for (int u = 0; u < kRiU; ++u) {  // row k0 is in range, so rows past n re-read it
+            const int64_t k = k0 + u * stride;
+            ok[u] = k < a.n;
+            const int64_t kk = ok[u] ? k : k0;
+            row[u] = last ? a.n - 1 - kk : kk;
+            xs[u] = stream_load(reinterpret_cast<const TC *>(a.x) + row[u]);
+            ys[u] = stream_load(reinterpret_cast<const TC *>(a.y) + row[u]);
+            vs[u] = r.nan_check ? stream_load(reinterpret_cast<const TV *>(a.val) + row[u]) : TV(0);
+        }
+        int64_t cell[kRiU];
+#pragma unroll
+        for (int u = 0; u < kRiU; ++u) {
+            int32_t px = 0, py = 0, pc = 0;
+            ok[u] = ok[u] && locate<TC>(a, row[u], xs[u], ys[u], px, py, pc);
+            cell[u] = ok[u] ? ((int64_t)py * a.W + px) * a.cat.ncat + pc : 0;
+            if (r.nan_check) ok[u] = ok[u] && !(vs[u] != vs[u]);
+        }
+        if (arg) {  // keep rows whose value is the cell's extreme
+#pragma unroll
+            for (int u = 0; u < kRiU; ++u) {
+                if (!ok[u]) continue;
                 if constexpr (sizeof(TV) == 4) {
-                    if (fkey32((float)v) != reinterpret_cast<const uint32_t *>(r.key)[cell]) continue;
+                    ok[u] = fkey32((float)vs[u]) == reinterpret_cast<const uint32_t *>(r.key)[cell[u]];
                 } else {
-                    if (fkey64((double)v) != reinterpret_cast<const uint64_t *>(r.key)[cell]) continue;
+                    ok[u] = fkey64((double)vs[u]) == reinterpret_cast<const uint64_t *>(r.key)[cell[u]];
                 }
             }
         }
-        unsigned long long *slot = r.index + cell;
-        if (r.mode == DSX_RI_LAST) {
-            const long long cur = *reinterpret_cast<volatile long long *>(slot);
-            if ((long long)i > cur) atomicMax(reinterpret_cast<long long *>(slot), (long long)i);
-        } else {
-            // -1 (empty) is the largest unsigned value, so min keeps the smallest row
-            const unsigned long long cur = *reinterpret_cast<volatile unsigned long long *>(slot);
-            if ((unsigned long long)i < cur) atomicMin(slot, (unsigned long long)i);
+        long long cur[kRiU];
+#pragma unroll
+        for (int u = 0; u < kRiU; ++u)  // a plain read first: most rows cannot improve the cell
+            cur[u] = ok[u] ? *reinterpret_cast<volatile long long *>(r.index + cell[u]) : 0;
+#pragma unroll
+        for (int u = 0; u < kRiU; ++u) {
+            if (!ok[u]) continue;
+            unsigned long long *slot = r.index + cell[u];
+            if (last) {
+                if ((long long)row[u] > cur[u]) atomicMax(reinterpret_cast<long long *>(slot), (long long)row[u]);
+            } else {
+                // -1 (empty) is the largest unsigned value, so min keeps the smallest row
+                if ((unsigned long long)row[u] < (unsigned long long)cur[u])
+                    atomicMin(slot, (unsigned long long)row[u]);
+            }
         }
     }
 }
@@ -1645,10 +1679,14 @@ __global__ __launch_bounds__(256) void rowindex_kernel(RiArgs r) {
 template <typename TC>
 static int rowindex_launch(const RiArgs &r, int val_dtype, hipStream_t st) {
     const int grid = grid_for(r.a.n < 1 ? 1 : r.a.n, 256, 8);
-    if (val_dtype == DSX_F32)
-        hipLaunchKernelGGL((rowindex_kernel<TC, float>), dim3(grid), dim3(256), 0, st, r);
-    else
-        hipLaunchKernelGGL((rowindex_kernel<TC, double>), dim3(grid), dim3(256), 0, st, r);
+    const bool arg = r.mode == DSX_RI_ARGMAX || r.mode == DSX_RI_ARGMIN;
+    if (val_dtype == DSX_F32) {
+        if (arg) hipLaunchKernelGGL((rowindex_kernel<TC, float, 4>), dim3(grid), dim3(256), 0, st, r);
+        else hipLaunchKernelGGL((rowindex_kernel<TC, float, 1>), dim3(grid), dim3(256), 0, st, r);
+    } else {
+        if (arg) hipLaunchKernelGGL((rowindex_kernel<TC, double, 4>), dim3(grid), dim3(256), 0, st, r);
+        else hipLaunchKernelGGL((rowindex_kernel<TC, double, 1>), dim3(grid), dim3(256), 0, st, r);
+    }
     return launch_status();
 }
 
