@@ -42,18 +42,22 @@ __device__ __forceinline__ uint64_t value_key(TV v, int descending) {
     return k;
 }
 
-// Row-ordered modes and 32-bit value keys: one composite key (cell << 32 | order).
+// Row-ordered modes and 32-bit value keys: one composite key (cell << 32 | order).  A
+// dropped row takes cell `ncells`, so the keys span 32 + bit_width(ncells) bits and the
+// sort runs only the passes those bits need.
 template <typename TC, typename TV>
 __global__ __launch_bounds__(256) void composite_kernel(PointArgs a, int mode, int nan_check,
-                                                        uint64_t *keys, uint32_t *rows) {
+                                                        uint32_t ncells, uint64_t *keys,
+                                                        uint32_t *rows) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
-        const uint32_t c = cell_of<TC, TV>(a, i, nan_check);
+        uint32_t c = cell_of<TC, TV>(a, i, nan_check);
         uint64_t order;
         if (mode == DSX_NS_FIRST) order = (uint64_t)i;
         else if (mode == DSX_NS_LAST) order = 0xFFFFFFFFull - (uint64_t)i;
         else order = c == kDropped ? 0ull : value_key<TV>(reinterpret_cast<const TV *>(a.val)[i],
                                                           mode == DSX_NS_MAX);
+        if (c == kDropped) c = ncells;
         keys[i] = ((uint64_t)c << 32) | order;
         rows[i] = (uint32_t)i;
     }
@@ -71,27 +75,29 @@ __global__ __launch_bounds__(256) void value_key_kernel(PointArgs a, int descend
     }
 }
 
-// Second sort's keys: the cell of each row in value order.
+// Second sort's keys: the cell of each row in value order (`ncells` when dropped).
 template <typename TC, typename TV>
 __global__ __launch_bounds__(256) void cell_key_kernel(PointArgs a, const uint32_t *rows,
-                                                       uint32_t *cells) {
+                                                       uint32_t ncells, uint32_t *cells) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < a.n; j += stride)
-        cells[j] = cell_of<TC, TV>(a, rows[j], 1);
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < a.n; j += stride) {
+        const uint32_t c = cell_of<TC, TV>(a, rows[j], 1);
+        cells[j] = c == kDropped ? ncells : c;
+    }
 }
 
 // The first n rows of every cell's run of the sorted order.
 __global__ __launch_bounds__(256) void take_runs_kernel(int64_t n, const uint64_t *keys64,
                                                         const uint32_t *keys32, int shift,
                                                         const uint32_t *rows, int nsel,
-                                                        int64_t *index) {
+                                                        uint32_t ncells, int64_t *index) {
     auto cell = [&](int64_t j) -> uint32_t {
         return keys64 ? (uint32_t)(keys64[j] >> shift) : keys32[j];
     };
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
         const uint32_t c = cell(j);
-        if (c == kDropped) continue;
+        if (c >= ncells) continue;  // dropped rows
         if (j > 0 && cell(j - 1) == c) continue;  // not the start of the cell's run
         for (int k = 0; k < nsel && j + k < n; ++k) {
             if (cell(j + k) != c) break;
@@ -102,15 +108,21 @@ __global__ __launch_bounds__(256) void take_runs_kernel(int64_t n, const uint64_
 
 struct NsPlan {
     bool two_sorts;
+    uint32_t ncells;
+    int cell_bits;  // bit width of ncells: cells and the dropped marker fit
     size_t o_k0, o_k1, o_r0, o_r1, o_c0, o_c1, o_tmp, tmp_bytes, total;
 };
 
 size_t al(size_t x) { return (x + 255) / 256 * 256; }
 
-int ns_plan(const dsx_points_desc *d, const dsx_nselect_desc *r, NsPlan &P) {
+int ns_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_nselect_desc *r,
+            NsPlan &P) {
     const size_t n = (size_t)d->n;
     const bool value = r->mode == DSX_NS_MAX || r->mode == DSX_NS_MIN;
     P.two_sorts = value && d->val_dtype == DSX_F64;
+    P.ncells = (uint32_t)((int64_t)v->width * v->height * cat_of(d->cat).ncat);  // < kDropped
+    P.cell_bits = 1;
+    while (P.cell_bits < 32 && (1ull << P.cell_bits) <= P.ncells) ++P.cell_bits;
     size_t o = 0;
     P.o_k0 = o; o = al(o + 8 * n);
     P.o_k1 = o; o = al(o + 8 * n);
@@ -121,12 +133,13 @@ int ns_plan(const dsx_points_desc *d, const dsx_nselect_desc *r, NsPlan &P) {
     size_t t1 = 0, t2 = 0;
     hipError_t e = rocprim::radix_sort_pairs(nullptr, t1, (const uint64_t *)nullptr,
                                              (uint64_t *)nullptr, (const uint32_t *)nullptr,
-                                             (uint32_t *)nullptr, n, 0, 64, (hipStream_t)0);
+                                             (uint32_t *)nullptr, n, 0,
+                                             P.two_sorts ? 64 : 32 + P.cell_bits, (hipStream_t)0);
     if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
     if (P.two_sorts) {
         e = rocprim::radix_sort_pairs(nullptr, t2, (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                                      (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 32,
-                                      (hipStream_t)0);
+                                      (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 0,
+                                      P.cell_bits, (hipStream_t)0);
         if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
     }
     P.tmp_bytes = t1 > t2 ? t1 : t2;
@@ -166,13 +179,13 @@ int ns_run(const dsx_points_desc *d, const dsx_view *v, const dsx_nselect_desc *
     hipError_t e;
     if (!P.two_sorts) {
         hipLaunchKernelGGL((composite_kernel<TC, TV>), dim3(grid), dim3(256), 0, st, a, r->mode,
-                           nan_check, k0, r0);
+                           nan_check, P.ncells, k0, r0);
         if ((s = launch_status())) return s;
-        e = rocprim::radix_sort_pairs(tmp, tb, k0, k1, r0, r1, (size_t)n, 0, 64, st);
+        e = rocprim::radix_sort_pairs(tmp, tb, k0, k1, r0, r1, (size_t)n, 0, 32 + P.cell_bits, st);
         if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
         hipLaunchKernelGGL(take_runs_kernel, dim3(grid), dim3(256), 0, st, n,
                            (const uint64_t *)k1, (const uint32_t *)nullptr, 32,
-                           (const uint32_t *)r1, r->n, r->index);
+                           (const uint32_t *)r1, r->n, P.ncells, r->index);
         return launch_status();
     }
     uint32_t *c0 = reinterpret_cast<uint32_t *>(ws + P.o_c0);
@@ -183,14 +196,14 @@ int ns_run(const dsx_points_desc *d, const dsx_view *v, const dsx_nselect_desc *
     e = rocprim::radix_sort_pairs(tmp, tb, k0, k1, r0, r1, (size_t)n, 0, 64, st);
     if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
     hipLaunchKernelGGL((cell_key_kernel<TC, TV>), dim3(grid), dim3(256), 0, st, a,
-                       (const uint32_t *)r1, c0);
+                       (const uint32_t *)r1, P.ncells, c0);
     if ((s = launch_status())) return s;
     tb = P.tmp_bytes;
-    e = rocprim::radix_sort_pairs(tmp, tb, c0, c1, r1, r0, (size_t)n, 0, 32, st);
+    e = rocprim::radix_sort_pairs(tmp, tb, c0, c1, r1, r0, (size_t)n, 0, P.cell_bits, st);
     if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
     hipLaunchKernelGGL(take_runs_kernel, dim3(grid), dim3(256), 0, st, n,
                        (const uint64_t *)nullptr, (const uint32_t *)c1, 0, (const uint32_t *)r0,
-                       r->n, r->index);
+                       r->n, P.ncells, r->index);
     return launch_status();
 }
 
@@ -203,7 +216,7 @@ extern "C" size_t dsx_points_nselect_workspace_size(const dsx_points_desc *d, co
                                                     const dsx_nselect_desc *r) {
     if (ns_check(d, v, r)) return 0;
     NsPlan P;
-    if (ns_plan(d, r, P)) return 0;
+    if (ns_plan(d, v, r, P)) return 0;
     return P.total;
 }
 
@@ -214,7 +227,7 @@ extern "C" int dsx_points_nselect(const dsx_points_desc *d, const dsx_view *v,
     if (s) return s;
     if (!r->index) return DSX_ERR_NULL;
     NsPlan P;
-    if ((s = ns_plan(d, r, P))) return s;
+    if ((s = ns_plan(d, v, r, P))) return s;
     if (!workspace || workspace_bytes < P.total) return DSX_ERR_WORKSPACE;
     hipStream_t st = as_stream(stream);
     const int64_t cells = (int64_t)v->width * v->height * cat_of(d->cat).ncat * r->n;
