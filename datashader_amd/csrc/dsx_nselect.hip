@@ -22,8 +22,8 @@ constexpr uint32_t kDropped = 0xFFFFFFFFu;  // cell of a row that is not kept
 template <typename TC, typename TV>
 __device__ __forceinline__ uint32_t cell_of(const PointArgs &a, int64_t i, int nan_check) {
     int32_t px, py, pc;
-    if (!locate<TC>(a, i, reinterpret_cast<const TC *>(a.x)[i],
-                    reinterpret_cast<const TC *>(a.y)[i], px, py, pc))
+    if (!locate<TC>(a, i, stream_load(reinterpret_cast<const TC *>(a.x) + i),
+                    stream_load(reinterpret_cast<const TC *>(a.y) + i), px, py, pc))
         return kDropped;
     if (nan_check) {
         const TV v = reinterpret_cast<const TV *>(a.val)[i];
