@@ -699,6 +699,37 @@ __device__ __forceinline__ unsigned long long block_exscan(unsigned long long v,
     return r;
 }
 
+// Block-wide exclusive scan of one 32-bit value per thread (1024 threads, totals below
+// 2^32): a wave scan, then wave 0 scans the 16 wave totals (no serial loop).
+__device__ __forceinline__ uint32_t block_exscan32(uint32_t v, uint32_t *sh, uint32_t &total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    if (wv == 0) {
+        constexpr int kW = kPartThreads / 64;
+        const uint32_t w = lane < kW ? sh[lane] : 0u;
+        uint32_t z = w;
+#pragma unroll
+        for (int off = 1; off < kW; off <<= 1) {
+            const uint32_t y = __shfl_up(z, off, 64);
+            if (lane >= off) z += y;
+        }
+        if (lane < kW) sh[lane] = z - w;            // exclusive prefix of the wave totals
+        if (lane == kW - 1) sh[kW] = z;             // block total
+    }
+    __syncthreads();
+    total = sh[kPartThreads / 64];
+    const uint32_t r = sh[wv] + x - v;
+    __syncthreads();
+    return r;
+}
+
 // Per tile (one workgroup each): exclusive prefix of the partition groups' counts, in
 // place, and the tile total.
 __global__ __launch_bounds__(kPartThreads) void spill_colscan_kernel(PartArgs p,
@@ -790,7 +821,7 @@ __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p)
     __shared__ uint32_t wb[kMaxTiles];   // sorted position of round slot 0, per tile
     __shared__ uint32_t s_cell[kSub];
     __shared__ TV s_val[HAS_VAL ? kSub : 1];
-    __shared__ unsigned long long sh[kPartThreads / 64 + 1];
+    __shared__ uint32_t sh[kPartThreads / 64 + 1];
     __shared__ uint32_t s_rounds;
     const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar bases
@@ -851,8 +882,8 @@ __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p)
             v[i] = cnt[tid * kTpt + i];  // 0 past the last tile
             sum += v[i];
         }
-        unsigned long long total;
-        uint32_t run = (uint32_t)block_exscan(sum, sh, total);
+        uint32_t total;
+        uint32_t run = block_exscan32(sum, sh, total);
 #pragma unroll
         for (int i = 0; i < kTpt; ++i) {
             const int t = tid * kTpt + i;
@@ -861,7 +892,7 @@ __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p)
             cur[i] += v[i];
             run += v[i];
         }
-        const uint32_t nsub = (uint32_t)total;
+        const uint32_t nsub = total;
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {  // counting sort into LDS
@@ -876,16 +907,35 @@ __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p)
         // indices out of the round loop and spills registers to hold them
         int32_t me;
         asm volatile("v_mov_b32 %0, %1" : "=v"(me) : "v"(tid));
+        // write out: kPer stores per thread, always; kOut entries at a time, their LDS
+        // reads issued together (one entry at a time left each store waiting on two
+        // dependent LDS round trips)
+        constexpr int kOut = 4;
+        static_assert(kPer % kOut == 0, "write-out batches");
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) {  // write out: kPer stores per thread, always
-            const uint32_t j = (uint32_t)(k * kPartThreads + me);
-            const bool ok = me < (int32_t)nsub - k * kPartThreads;
-            const uint32_t c = s_cell[j];
-            const uint32_t t = ok ? c >> p.tile_shift : 0u;
-            const size_t dst = ok ? (size_t)(wb[t] + j) : sink;
-            p.sorted_cell[dst] = c;
-            if constexpr (HAS_VAL) sval[dst] = s_val[j];
-            __builtin_amdgcn_sched_barrier(0);  // one entry at a time: bounded registers
+        for (int k0 = 0; k0 < kPer; k0 += kOut) {
+            uint32_t c[kOut], w[kOut];
+            TV vv[kOut];
+#pragma unroll
+            for (int b = 0; b < kOut; ++b) {
+                const uint32_t j = (uint32_t)((k0 + b) * kPartThreads + me);
+                c[b] = s_cell[j];
+                if constexpr (HAS_VAL) vv[b] = s_val[j];
+            }
+#pragma unroll
+            for (int b = 0; b < kOut; ++b) {
+                const bool ok = me < (int32_t)nsub - (k0 + b) * kPartThreads;
+                w[b] = wb[ok ? c[b] >> p.tile_shift : 0u];
+            }
+#pragma unroll
+            for (int b = 0; b < kOut; ++b) {
+                const uint32_t j = (uint32_t)((k0 + b) * kPartThreads + me);
+                const bool ok = me < (int32_t)nsub - (k0 + b) * kPartThreads;
+                const size_t dst = ok ? (size_t)(w[b] + j) : sink;
+                p.sorted_cell[dst] = c[b];
+                if constexpr (HAS_VAL) sval[dst] = vv[b];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // bounded registers
         }
 #pragma unroll
         for (int i = 0; i < kTpt; ++i) cnt[tid * kTpt + i] = 0u;

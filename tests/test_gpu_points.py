@@ -265,6 +265,24 @@ def test_autorange_on_device_matches_reference_bounds():
     assert agg.values.sum() == 0
 
 
+@pytest.mark.parametrize("n", [4_194_305, 12_000_007])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_bounds_of_long_columns(n, dtype):
+    """dsx_bounds over columns long enough for the eight-loads-per-thread loop and its
+    tail: NaN-skipping min / max equal numpy's nanmin / nanmax, with the extremes planted
+    at the start, the middle and the last row."""
+    from datashader_amd import _device as dev
+    g = torch.Generator(device="cuda").manual_seed(n)
+    col = torch.randn(n, generator=g, device="cuda", dtype=dtype)
+    col[torch.randint(0, n, (1000,), generator=g, device="cuda")] = float("nan")
+    for lo_at, hi_at in ((0, n - 1), (n // 2, 3), (n - 1, n // 2 + 1)):
+        c = col.clone()
+        c[lo_at] = -50.0
+        c[hi_at] = 60.0
+        host = c.cpu().numpy()
+        assert dev.bounds(c) == (float(np.nanmin(host)), float(np.nanmax(host)))
+
+
 def test_empty_and_tiny_inputs():
     df = pd.DataFrame({"x": np.zeros(0), "y": np.zeros(0), "v": np.zeros(0)})
     ds = dsa.Canvas(3, 2, x_range=(0, 1), y_range=(0, 1)).points(
