@@ -44,6 +44,7 @@ constexpr int kTileThreads = 512;
 // Larger tiles mean fewer tiles, so the scatter writes longer runs per tile.
 constexpr int kTileLds = 78 * 1024;
 constexpr int kMaxTiles = 4096;
+constexpr int kMinTileShift = 10;               // smallest tile: 1024 cells
 constexpr int kMinPart = 4096;                  // fewest spill entries a tile part is given
 constexpr int kScatterPad = 8192;               // slack entries after the spill sub-segments
 constexpr int kSelBins = 8192;                  // coarse density histogram bins
@@ -974,6 +975,28 @@ struct TileRows {
     }
 };
 
+// Write tile t's cells from its LDS region: every cell outside the window (the window
+// merge wrote the others; their only spilled rows are rows with a NaN value when count
+// and count(v) are both asked for, and those are added).
+template <int KEYB>
+__device__ void tile_store(const uint8_t *smem, const Lay &L, const PartArgs &p, int t,
+                           const int32_t *window, int32_t W, int32_t H, int64_t wcap_px,
+                           int32_t ncat, int64_t ncells, const Planes &out) {
+    const int64_t c0 = (int64_t)t * p.tile_cells;
+    const Window w = load_window(window, W, H, wcap_px);
+    int64_t c1 = c0 + p.tile_cells;
+    if (c1 > ncells) c1 = ncells;
+    for (int64_t gcell = c0 + threadIdx.x; gcell < c1; gcell += kTileThreads) {
+        const int64_t pix = gcell / ncat;
+        const int32_t py = (int32_t)(pix / W), px = (int32_t)(pix - (int64_t)py * W);
+        if (in_window(w, px, py)) {
+            if (L.flags & PL_NAN) region_add_to<KEYB>(smem, L, (uint32_t)(gcell - c0), out, gcell);
+            continue;
+        }
+        region_store<KEYB>(smem, L, (uint32_t)(gcell - c0), out, gcell);
+    }
+}
+
 template <typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
 __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay L,
                                                                const int32_t *window, int32_t W,
@@ -981,7 +1004,6 @@ __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay 
                                                                int32_t ncat, int64_t ncells,
                                                                const double *mean, Planes out) {
     extern __shared__ __align__(16) uint8_t smem[];
-    __shared__ int32_t s_last;
     using R = TileRows<TV, HAS_VAL>;
     constexpr unsigned long long kStep = (unsigned long long)R::kU * kTileThreads;
     // this workgroup's part: the last tile whose first part is at or before it
@@ -1032,41 +1054,80 @@ __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay 
             step(rc, rb);
         }
     }
+    __syncthreads();
     if (nparts > 1) {
-        // a split tile: every part leaves its planes in its scratch region; the part that
-        // finishes last adds the others' regions to its own and writes the tile
-        __syncthreads();
+        // a split tile: every part leaves its planes in its scratch region and
+        // tile_merge_kernel, launched next, folds them and writes the tile (a kernel
+        // boundary publishes the regions: a device-scope fence per part, as a last-part
+        // merge needs, writes back L2 each time and cost more than the tile work)
         uint4 *mine = reinterpret_cast<uint4 *>(p.scratch + (size_t)part * L.bytes);
         const uint4 *src = reinterpret_cast<const uint4 *>(smem);
         for (uint32_t j = threadIdx.x; j < L.bytes / 16u; j += kTileThreads) mine[j] = src[j];
-        __threadfence();  // release: the region is visible before the count below
-        __syncthreads();
-        if (threadIdx.x == 0) s_last = atomicAdd(p.done + t, 1u) == nparts - 1;
-        __syncthreads();
-        if (!s_last) return;
-        __threadfence();  // acquire: the other parts' regions
+        return;
+    }
+    tile_store<KEYB>(smem, L, p, t, window, W, H, wcap_px, ncat, ncells, out);
+}
+
+// One plane of a split tile: cell c of the LDS region = op over the parts' scratch
+// regions (part q at parts + q * part_bytes).  Each thread folds kFold cells at a time,
+// so each part's loads for them are in flight together.  cap is a multiple of
+// kTileThreads (tiles are powers of two of at least 1024 cells).
+template <typename T, typename Op>
+__device__ __forceinline__ void fold_plane(uint8_t *lds, const uint8_t *parts, size_t part_bytes,
+                                           uint32_t off, uint32_t nparts, uint32_t cap, T ident,
+                                           Op op) {
+    constexpr int kFold = 8;
+    T *dst = reinterpret_cast<T *>(lds + off);
+    for (uint32_t c0 = threadIdx.x; c0 < cap; c0 += kFold * kTileThreads) {
+        T acc[kFold];
+#pragma unroll
+        for (int b = 0; b < kFold; ++b) acc[b] = ident;
         for (uint32_t q = 0; q < nparts; ++q) {
-            if (first + q == part) continue;
-            const uint8_t *other = p.scratch + (size_t)(first + q) * L.bytes;
-            for (uint32_t c = threadIdx.x; c < L.cap; c += kTileThreads)
-                region_merge<KEYB>(smem, L, c, other);
+            const T *src = reinterpret_cast<const T *>(parts + (size_t)q * part_bytes + off);
+            T v[kFold];
+#pragma unroll
+            for (int b = 0; b < kFold; ++b) {
+                const uint32_t c = c0 + (uint32_t)(b * kTileThreads);
+                v[b] = c < cap ? src[c] : ident;  // uniform: cap is a multiple of the block
+            }
+#pragma unroll
+            for (int b = 0; b < kFold; ++b) acc[b] = op(acc[b], v[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < kFold; ++b) {
+            const uint32_t c = c0 + (uint32_t)(b * kTileThreads);
+            if (c < cap) dst[c] = acc[b];
         }
     }
+}
+
+// Split tiles: one workgroup per tile folds its parts' scratch regions in LDS and writes
+// the tile.  Tiles of one part were written by tile_agg_kernel.
+template <int KEYB>
+__global__ __launch_bounds__(kTileThreads) void tile_merge_kernel(PartArgs p, Lay L,
+                                                                 const int32_t *window, int32_t W,
+                                                                 int32_t H, int64_t wcap_px,
+                                                                 int32_t ncat, int64_t ncells,
+                                                                 Planes out) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int t = blockIdx.x;
+    if (t >= p.ntiles) return;
+    const uint32_t first = p.part_start[t], nparts = p.part_start[t + 1] - first;
+    if (nparts <= 1) return;  // uniform
+    using T = typename KeyOf<KEYB>::T;
+    const uint8_t *parts = p.scratch + (size_t)first * L.bytes;
+    auto add64 = [](double a, double b) { return a + b; };
+    auto add32 = [](uint32_t a, uint32_t b) { return a + b; };
+    auto kmin = [](T a, T b) { return b < a ? b : a; };
+    auto kmax = [](T a, T b) { return b > a ? b : a; };
+    if (L.flags & PL_SUM) fold_plane<double>(smem, parts, L.bytes, L.o_sum, nparts, L.cap, 0.0, add64);
+    if (L.flags & PL_MIN) fold_plane<T>(smem, parts, L.bytes, L.o_min, nparts, L.cap, KeyOf<KEYB>::kMinEmpty, kmin);
+    if (L.flags & PL_MAX) fold_plane<T>(smem, parts, L.bytes, L.o_max, nparts, L.cap, T(0), kmax);
+    if (L.flags & PL_CNT) fold_plane<uint32_t>(smem, parts, L.bytes, L.o_cnt, nparts, L.cap, 0u, add32);
+    if (L.flags & PL_CNTV) fold_plane<uint32_t>(smem, parts, L.bytes, L.o_cntv, nparts, L.cap, 0u, add32);
+    if (L.flags & PL_NAN) fold_plane<uint32_t>(smem, parts, L.bytes, L.o_nan, nparts, L.cap, 0u, add32);
     __syncthreads();
-    const Window w = load_window(window, W, H, wcap_px);
-    int64_t c1 = c0 + p.tile_cells;
-    if (c1 > ncells) c1 = ncells;
-    for (int64_t gcell = c0 + threadIdx.x; gcell < c1; gcell += kTileThreads) {
-        const int64_t pix = gcell / ncat;
-        const int32_t py = (int32_t)(pix / W), px = (int32_t)(pix - (int64_t)py * W);
-        if (in_window(w, px, py)) {
-            // written by the window merge; the only spilled rows of a window cell are
-            // rows with a NaN value (count with countv): add them
-            if (L.flags & PL_NAN) region_add_to<KEYB>(smem, L, (uint32_t)(gcell - c0), out, gcell);
-            continue;
-        }
-        region_store<KEYB>(smem, L, (uint32_t)(gcell - c0), out, gcell);
-    }
+    tile_store<KEYB>(smem, L, p, t, window, W, H, wcap_px, ncat, ncells, out);
 }
 
 // ---- host side --------------------------------------------------------------------------------
@@ -1106,11 +1167,22 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     // tiles: runs of 2^tshift cells (a cell's tile is one shift) whose planes fit kTileLds
     // (several tile workgroups share a CU), or up to the whole LDS when that would make
     // more than kMaxTiles tiles
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0)
+            cus = c;
+    }
     P.tshift = 0;
     while ((int64_t)(2ll << P.tshift) * cb <= kTileLds - 16) ++P.tshift;
+    // a small canvas still gets one tile per CU (down to 1024-cell tiles): 800x600 counts
+    // made 30 tiles of 16384 cells, so 30 workgroups did all of the tile work
+    while (P.tshift > kMinTileShift && ((P.ncells + (1ll << P.tshift) - 1) >> P.tshift) < cus)
+        --P.tshift;
     while ((P.ncells + (1ll << P.tshift) - 1) >> P.tshift > kMaxTiles) ++P.tshift;
     const int64_t tcap = 1ll << P.tshift;
     if (tcap * (int64_t)cb > kLdsBytes - 16) return DSX_ERR_SHAPE;
+    if (tcap % kTileThreads) return DSX_ERR_SHAPE;  // never: tiles hold >= 1024 cells
     P.ntiles = (int32_t)((P.ncells + tcap - 1) >> P.tshift);
     if (P.ntiles < 1) P.ntiles = 1;
     P.tl = make_lay(P.flags, P.keyb, (uint32_t)tcap);
@@ -1140,12 +1212,6 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     P.cw = (v->width + (1 << P.bshift) - 1) >> P.bshift;
     P.ch = (v->height + (1 << P.bshift) - 1) >> P.bshift;
     // scan workgroups: one per CU at most, fewer for small inputs
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        int c = 0;
-        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0)
-            cus = c;
-    }
     const int64_t chunks = (d->n + kChunk - 1) / kChunk;
     P.scan_groups = (int)(chunks < cus ? (chunks < 1 ? 1 : chunks) : cus);
     // each wave's spill sub-segment holds every row that wave could spill
@@ -1289,6 +1355,13 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
     hipLaunchKernelGGL(tile, dim3(P.max_parts), dim3(kTileThreads), P.tile_lds, st, pa, P.tl,
                        (const int32_t *)window, a.W, a.H, wcap_px, ncat, P.ncells, mean, out);
+    if ((s = launch_status())) return s;
+    auto merge = tile_merge_kernel<KEYB>;
+    e = hipFuncSetAttribute((const void *)merge, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)P.tile_lds);
+    if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
+    hipLaunchKernelGGL(merge, dim3(P.ntiles), dim3(kTileThreads), P.tile_lds, st, pa, P.tl,
+                       (const int32_t *)window, a.W, a.H, wcap_px, ncat, P.ncells, out);
     return launch_status();
 }
 
