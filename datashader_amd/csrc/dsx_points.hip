@@ -1069,16 +1069,16 @@ __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay 
 }
 
 // One plane of a split tile: cell c of the LDS region = op over the parts' scratch
-// regions (part q at parts + q * part_bytes).  Each thread folds kFold cells at a time,
-// so each part's loads for them are in flight together.  cap is a multiple of
-// kTileThreads (tiles are powers of two of at least 1024 cells).
-template <typename T, typename Op>
+// regions (part q at parts + q * part_bytes).  Each of the NT threads folds kFold cells
+// at a time, so each part's loads for them are in flight together (an index past cap
+// re-reads the last cell and is not stored).
+template <int NT, typename T, typename Op>
 __device__ __forceinline__ void fold_plane(uint8_t *lds, const uint8_t *parts, size_t part_bytes,
                                            uint32_t off, uint32_t nparts, uint32_t cap, T ident,
                                            Op op) {
     constexpr int kFold = 8;
     T *dst = reinterpret_cast<T *>(lds + off);
-    for (uint32_t c0 = threadIdx.x; c0 < cap; c0 += kFold * kTileThreads) {
+    for (uint32_t c0 = threadIdx.x; c0 < cap; c0 += kFold * NT) {
         T acc[kFold];
 #pragma unroll
         for (int b = 0; b < kFold; ++b) acc[b] = ident;
@@ -1087,18 +1087,35 @@ __device__ __forceinline__ void fold_plane(uint8_t *lds, const uint8_t *parts, s
             T v[kFold];
 #pragma unroll
             for (int b = 0; b < kFold; ++b) {
-                const uint32_t c = c0 + (uint32_t)(b * kTileThreads);
-                v[b] = c < cap ? src[c] : ident;  // uniform: cap is a multiple of the block
+                const uint32_t c = c0 + (uint32_t)(b * NT);
+                v[b] = src[c < cap ? c : cap - 1];
             }
 #pragma unroll
             for (int b = 0; b < kFold; ++b) acc[b] = op(acc[b], v[b]);
         }
 #pragma unroll
         for (int b = 0; b < kFold; ++b) {
-            const uint32_t c = c0 + (uint32_t)(b * kTileThreads);
+            const uint32_t c = c0 + (uint32_t)(b * NT);
             if (c < cap) dst[c] = acc[b];
         }
     }
+}
+
+// Every plane of a split tile's region folded into LDS.
+template <int NT, int KEYB>
+__device__ __forceinline__ void fold_parts(uint8_t *smem, const Lay &L, const uint8_t *parts,
+                                           uint32_t nparts) {
+    using T = typename KeyOf<KEYB>::T;
+    auto add64 = [](double a, double b) { return a + b; };
+    auto add32 = [](uint32_t a, uint32_t b) { return a + b; };
+    auto kmin = [](T a, T b) { return b < a ? b : a; };
+    auto kmax = [](T a, T b) { return b > a ? b : a; };
+    if (L.flags & PL_SUM) fold_plane<NT, double>(smem, parts, L.bytes, L.o_sum, nparts, L.cap, 0.0, add64);
+    if (L.flags & PL_MIN) fold_plane<NT, T>(smem, parts, L.bytes, L.o_min, nparts, L.cap, KeyOf<KEYB>::kMinEmpty, kmin);
+    if (L.flags & PL_MAX) fold_plane<NT, T>(smem, parts, L.bytes, L.o_max, nparts, L.cap, T(0), kmax);
+    if (L.flags & PL_CNT) fold_plane<NT, uint32_t>(smem, parts, L.bytes, L.o_cnt, nparts, L.cap, 0u, add32);
+    if (L.flags & PL_CNTV) fold_plane<NT, uint32_t>(smem, parts, L.bytes, L.o_cntv, nparts, L.cap, 0u, add32);
+    if (L.flags & PL_NAN) fold_plane<NT, uint32_t>(smem, parts, L.bytes, L.o_nan, nparts, L.cap, 0u, add32);
 }
 
 // Split tiles: one workgroup per tile folds its parts' scratch regions in LDS and writes
@@ -1114,18 +1131,7 @@ __global__ __launch_bounds__(kTileThreads) void tile_merge_kernel(PartArgs p, La
     if (t >= p.ntiles) return;
     const uint32_t first = p.part_start[t], nparts = p.part_start[t + 1] - first;
     if (nparts <= 1) return;  // uniform
-    using T = typename KeyOf<KEYB>::T;
-    const uint8_t *parts = p.scratch + (size_t)first * L.bytes;
-    auto add64 = [](double a, double b) { return a + b; };
-    auto add32 = [](uint32_t a, uint32_t b) { return a + b; };
-    auto kmin = [](T a, T b) { return b < a ? b : a; };
-    auto kmax = [](T a, T b) { return b > a ? b : a; };
-    if (L.flags & PL_SUM) fold_plane<double>(smem, parts, L.bytes, L.o_sum, nparts, L.cap, 0.0, add64);
-    if (L.flags & PL_MIN) fold_plane<T>(smem, parts, L.bytes, L.o_min, nparts, L.cap, KeyOf<KEYB>::kMinEmpty, kmin);
-    if (L.flags & PL_MAX) fold_plane<T>(smem, parts, L.bytes, L.o_max, nparts, L.cap, T(0), kmax);
-    if (L.flags & PL_CNT) fold_plane<uint32_t>(smem, parts, L.bytes, L.o_cnt, nparts, L.cap, 0u, add32);
-    if (L.flags & PL_CNTV) fold_plane<uint32_t>(smem, parts, L.bytes, L.o_cntv, nparts, L.cap, 0u, add32);
-    if (L.flags & PL_NAN) fold_plane<uint32_t>(smem, parts, L.bytes, L.o_nan, nparts, L.cap, 0u, add32);
+    fold_parts<kTileThreads, KEYB>(smem, L, p.scratch + (size_t)first * L.bytes, nparts);
     __syncthreads();
     tile_store<KEYB>(smem, L, p, t, window, W, H, wcap_px, ncat, ncells, out);
 }
@@ -1499,11 +1505,22 @@ __global__ __launch_bounds__(kPartThreads) void idx_plan_kernel(IdxArgs A) {
     if (threadIdx.x == 0) A.part_start[A.ntiles] = (uint32_t)pcarry;
 }
 
+// Write an indexed tile's pixels [px0, px1) x [py0, py1) from its LDS region.
+template <int KEYB>
+__device__ void idx_store(const uint8_t *smem, const IdxArgs &A, int px0, int px1, int py0,
+                          int py1) {
+    const int w = px1 - px0, h = py1 - py0;
+    for (int c = threadIdx.x; c < w * h; c += kIdxThreads) {
+        const int ly = c / w, lx = c - ly * w;
+        region_store<KEYB>(smem, A.L, (uint32_t)(ly * A.tw + lx), A.out,
+                           (int64_t)(py0 + ly) * A.a.W + (px0 + lx));
+    }
+}
+
 template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
 __global__ __launch_bounds__(kIdxThreads) void idx_tile_kernel(IdxArgs A) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ unsigned long long sh[kIdxThreads / 64 + 1];
-    __shared__ int32_t s_last;
     const PointArgs &a = A.a;
     const Lay &L = A.L;
     const uint32_t part = blockIdx.x;
@@ -1571,31 +1588,31 @@ __global__ __launch_bounds__(kIdxThreads) void idx_tile_kernel(IdxArgs A) {
             region_add<KEYB>(smem, L, lc, vnan, v, add);
         }
     }
+    __syncthreads();
     if (nparts > 1) {
-        __syncthreads();
+        // a split tile: each part leaves its region in scratch; idx_merge_kernel,
+        // launched next, folds them and writes the tile (no device-scope fence per part)
         uint4 *mine = reinterpret_cast<uint4 *>(A.scratch + (size_t)part * L.bytes);
         const uint4 *src = reinterpret_cast<const uint4 *>(smem);
         for (uint32_t j = threadIdx.x; j < L.bytes / 16u; j += kIdxThreads) mine[j] = src[j];
-        __threadfence();  // release: the region is visible before the count below
-        __syncthreads();
-        if (threadIdx.x == 0) s_last = atomicAdd(A.done + t, 1u) == nparts - 1;
-        __syncthreads();
-        if (!s_last) return;
-        __threadfence();  // acquire: the other parts' regions
-        for (uint32_t q = 0; q < nparts; ++q) {
-            if (first + q == part) continue;
-            const uint8_t *other = A.scratch + (size_t)(first + q) * L.bytes;
-            for (uint32_t c = threadIdx.x; c < L.cap; c += kIdxThreads)
-                region_merge<KEYB>(smem, L, c, other);
-        }
+        return;
     }
+    idx_store<KEYB>(smem, A, px0, px1, py0, py1);
+}
+
+// Split indexed tiles: fold the parts' regions, write the tile.
+template <int KEYB>
+__global__ __launch_bounds__(kIdxThreads) void idx_merge_kernel(IdxArgs A) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int t = blockIdx.x;
+    if (t >= A.ntiles) return;
+    const uint32_t first = A.part_start[t], nparts = A.part_start[t + 1] - first;
+    if (nparts <= 1) return;  // uniform
+    int px0, px1, py0, py1, gx0, gx1, gy0, gy1;
+    idx_rect(A, t, px0, px1, py0, py1, gx0, gx1, gy0, gy1);
+    fold_parts<kIdxThreads, KEYB>(smem, A.L, A.scratch + (size_t)first * A.L.bytes, nparts);
     __syncthreads();
-    const int w = px1 - px0, h = py1 - py0;
-    for (int c = threadIdx.x; c < w * h; c += kIdxThreads) {
-        const int ly = c / w, lx = c - ly * w;
-        region_store<KEYB>(smem, L, (uint32_t)(ly * A.tw + lx), A.out,
-                           (int64_t)(py0 + ly) * a.W + (px0 + lx));
-    }
+    idx_store<KEYB>(smem, A, px0, px1, py0, py1);
 }
 
 struct IdxPlan {
@@ -1671,6 +1688,13 @@ static int idx_run(IdxArgs &A, const IdxPlan &P, hipStream_t st) {
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds);
     if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
     hipLaunchKernelGGL(kern, dim3(P.max_parts), dim3(kIdxThreads), P.lds, st, A);
+    if ((s = launch_status())) return s;
+    auto merge = idx_merge_kernel<KEYB>;
+    const hipError_t e2 = hipFuncSetAttribute((const void *)merge,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)A.L.bytes);
+    if (e2 != hipSuccess) { record_hip_error(e2); return DSX_ERR_HIP; }
+    hipLaunchKernelGGL(merge, dim3(P.ntiles), dim3(kIdxThreads), A.L.bytes, st, A);
     return launch_status();
 }
 
