@@ -662,13 +662,11 @@ struct PartArgs {
     void *sorted_val;                 // 64-slot sink line per scatter workgroup
     size_t sink;
     // tile work split: tile t is processed by parts [part_start[t], part_start[t + 1]),
-    // over which tile t's entries are split evenly (*part_len: the largest part size);
-    // done[t] counts finished parts of a split tile
+    // over which tile t's entries are split evenly (*part_len: the largest part size)
     uint32_t *part_start;             // [ntiles + 1]
     uint32_t *part_tile;              // [max_parts] tile of each part
     int32_t max_parts;
     uint32_t *part_len;               // [1]
-    uint32_t *done;                   // [ntiles]
     uint8_t *scratch;                 // one tile-layout region per part of a split tile
 };
 
@@ -773,7 +771,6 @@ __global__ __launch_bounds__(kPartThreads) void spill_tilescan_kernel(PartArgs p
         if (t < p.ntiles) {
             np = (totals[t] + E - 1) / E;
             np = np < 1 ? 1 : np;
-            p.done[t] = 0u;
         }
         unsigned long long tot;
         const unsigned long long ex = block_exscan(np, sh, tot);
@@ -1150,7 +1147,7 @@ struct Plan {
     uint32_t o_thist, o_stage_cell, o_stage_val;  // scan LDS offsets
     // workspace offsets
     size_t o_window, o_hist_sel, o_count, o_partials, o_spill_cell, o_spill_val, o_hist,
-        o_totals, o_tile_start, o_sorted_cell, o_sorted_val, o_part_start, o_part_len, o_part_tile, o_done,
+        o_totals, o_tile_start, o_sorted_cell, o_sorted_val, o_part_start, o_part_len, o_part_tile,
         o_scratch, total;
     int32_t max_parts;   // upper bound of the tile parts (grid of the tile kernel)
     bool whole;          // the window is the whole canvas
@@ -1243,7 +1240,6 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     P.o_part_start = o; o = align_up(o + 4 * (size_t)(P.ntiles + 1), 256);
     P.o_part_len = o; o = align_up(o + 4, 256);
     P.o_part_tile = o; o = align_up(o + 4 * (size_t)P.max_parts, 256);
-    P.o_done = o; o = align_up(o + 4 * (size_t)P.ntiles, 256);
     P.o_scratch = o; o = align_up(o + (size_t)P.max_parts * P.tl.bytes, 256);
     P.total = o;
     return DSX_OK;
@@ -1346,7 +1342,6 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     pa.part_len = reinterpret_cast<uint32_t *>(ws + P.o_part_len);
     pa.part_tile = reinterpret_cast<uint32_t *>(ws + P.o_part_tile);
     pa.max_parts = P.max_parts;
-    pa.done = reinterpret_cast<uint32_t *>(ws + P.o_done);
     pa.scratch = ws + P.o_scratch;
     unsigned long long *totals = reinterpret_cast<unsigned long long *>(ws + P.o_totals);
     hipLaunchKernelGGL(spill_colscan_kernel, dim3(P.ntiles), dim3(kPartThreads), 0, st, pa, totals);
@@ -1388,8 +1383,8 @@ static int dispatch_val(const dsx_points_desc *d, const PointArgs &a, const Plan
 // the lattice rows under the tile.  Tile t reads the rows of the lattice cells its pixels
 // can receive (one extra cell on each side), keeps the rows whose pixel is in the tile
 // and writes every cell of the tile: each row lands in exactly one tile, and no row is
-// spilled.  A tile with many rows is split into parts of at most part_len rows; the part
-// that finishes last merges the others' regions, as in the spill tiles.
+// spilled.  A tile with many rows is split into parts of at most part_len rows, whose
+// regions a merge kernel folds, as for the spill tiles.
 constexpr int kIdxThreads = 1024;
 constexpr int kIdxMaxRows = 1024;       // lattice rows a tile may span (grid <= 1024)
 constexpr int kIdxTable = kIdxMaxRows * 16 + 16;
@@ -1405,7 +1400,6 @@ struct IdxArgs {
     unsigned long long *tile_rows;   // [ntiles]
     uint32_t *part_start;            // [ntiles + 1]
     uint32_t *part_tile;             // [max_parts]
-    uint32_t *done;                  // [ntiles]
     unsigned long long *part_len;    // [1]
     int32_t max_parts, cus;
     uint8_t *scratch;
@@ -1490,7 +1484,6 @@ __global__ __launch_bounds__(kPartThreads) void idx_plan_kernel(IdxArgs A) {
         if (t < A.ntiles) {
             np = (A.tile_rows[t] + E - 1) / E;
             np = np < 1 ? 1 : np;
-            A.done[t] = 0u;
         }
         unsigned long long tot;
         const unsigned long long ex = block_exscan(np, sh, tot);
@@ -1618,7 +1611,7 @@ __global__ __launch_bounds__(kIdxThreads) void idx_merge_kernel(IdxArgs A) {
 struct IdxPlan {
     Lay L;
     int32_t tw, th, ntx, nty, ntiles, max_parts, cus;
-    size_t lds, o_rows, o_pstart, o_ptile, o_done, o_plen, o_scratch, total;
+    size_t lds, o_rows, o_pstart, o_ptile, o_plen, o_scratch, total;
 };
 
 static int idx_make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_planes *pl,
@@ -1653,7 +1646,6 @@ static int idx_make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_
     P.o_rows = o; o = align_up(o + 8 * (size_t)P.ntiles, 256);
     P.o_pstart = o; o = align_up(o + 4 * (size_t)(P.ntiles + 1), 256);
     P.o_ptile = o; o = align_up(o + 4 * (size_t)P.max_parts, 256);
-    P.o_done = o; o = align_up(o + 4 * (size_t)P.ntiles, 256);
     P.o_plen = o; o = align_up(o + 8, 256);
     P.o_scratch = o; o = align_up(o + (size_t)P.max_parts * P.L.bytes, 256);
     P.total = o;
@@ -1893,7 +1885,6 @@ extern "C" int dsx_points_agg_indexed(const dsx_points_desc *d, const dsx_view *
     A.tile_rows = reinterpret_cast<unsigned long long *>(ws + P.o_rows);
     A.part_start = reinterpret_cast<uint32_t *>(ws + P.o_pstart);
     A.part_tile = reinterpret_cast<uint32_t *>(ws + P.o_ptile);
-    A.done = reinterpret_cast<uint32_t *>(ws + P.o_done);
     A.part_len = reinterpret_cast<unsigned long long *>(ws + P.o_plen);
     A.max_parts = P.max_parts;
     A.cus = P.cus;
