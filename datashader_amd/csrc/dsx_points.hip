@@ -1151,6 +1151,7 @@ struct Plan {
         o_scratch, total;
     int32_t max_parts;   // upper bound of the tile parts (grid of the tile kernel)
     bool whole;          // the window is the whole canvas
+    bool global;         // the canvas is too large for the tiles: global-atomic pass
     int bshift, cw, ch;
     int tshift;          // tile_cells = 1 << tshift
 };
@@ -1184,7 +1185,14 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
         --P.tshift;
     while ((P.ncells + (1ll << P.tshift) - 1) >> P.tshift > kMaxTiles) ++P.tshift;
     const int64_t tcap = 1ll << P.tshift;
-    if (tcap * (int64_t)cb > kLdsBytes - 16) return DSX_ERR_SHAPE;
+    P.global = false;
+    if (tcap * (int64_t)cb > kLdsBytes - 16) {
+        // more cells than kMaxTiles tiles of LDS hold (e.g. 8192 x 8192 count + mean):
+        // one pass of global atomics instead, slower but exact
+        P.global = true;
+        P.total = 256;
+        return DSX_OK;
+    }
     if (tcap % kTileThreads) return DSX_ERR_SHAPE;  // never: tiles hold >= 1024 cells
     P.ntiles = (int32_t)((P.ncells + tcap - 1) >> P.tshift);
     if (P.ntiles < 1) P.ntiles = 1;
@@ -1363,6 +1371,60 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
     hipLaunchKernelGGL(merge, dim3(P.ntiles), dim3(kTileThreads), P.tile_lds, st, pa, P.tl,
                        (const int32_t *)window, a.W, a.H, wcap_px, ncat, P.ncells, out);
+    return launch_status();
+}
+
+// ---- canvases beyond the tiles: one pass of global atomics ------------------------------
+// Every kept row updates its cell of the output planes directly (the planes start at the
+// reference's initial values), as the reference's append does (reductions.py).
+template <typename TC, typename TV, bool HAS_VAL, bool SQDEV>
+__global__ __launch_bounds__(256) void points_global_kernel(PointArgs a, Planes out,
+                                                            const double *mean) {
+    const TC *X = reinterpret_cast<const TC *>(a.x);
+    const TC *Y = reinterpret_cast<const TC *>(a.y);
+    const TV *V = reinterpret_cast<const TV *>(a.val);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+        int32_t px, py, pc;
+        if (!locate<TC>(a, i, stream_load(X + i), stream_load(Y + i), px, py, pc)) continue;
+        const int64_t cell = ((int64_t)py * a.W + px) * a.cat.ncat + pc;
+        TV v = TV(0);
+        if constexpr (HAS_VAL) v = stream_load(V + i);
+        if constexpr (SQDEV) {
+            if (v != v) continue;
+            const double dd = (double)v - mean[cell];
+            unsafeAtomicAdd(out.sum + cell, dd * dd);
+        } else {
+            append<TV, HAS_VAL>(out, cell, v);
+        }
+    }
+}
+
+template <typename TC>
+static int run_global(const dsx_points_desc *d, const PointArgs &a, const dsx_planes *pl,
+                      const Planes &out, const double *mean, hipStream_t s) {
+    const int64_t ncells = (int64_t)a.W * a.H * a.cat.ncat;
+    const bool f32 = has_value(d) && d->val_dtype == DSX_F32;
+    int st;
+    if (pl->count && (st = fill_async(pl->count, DSX_FILL_ZERO32, ncells, s))) return st;
+    if (pl->countv && (st = fill_async(pl->countv, DSX_FILL_ZERO32, ncells, s))) return st;
+    if (pl->sum && (st = fill_async(pl->sum, DSX_FILL_ZERO64, ncells, s))) return st;
+    if (pl->min && (st = fill_async(pl->min, f32 ? DSX_FILL_ONES32 : DSX_FILL_ONES64, ncells, s)))
+        return st;
+    if (pl->max && (st = fill_async(pl->max, f32 ? DSX_FILL_ZERO32 : DSX_FILL_ZERO64, ncells, s)))
+        return st;
+    if (d->n == 0) return DSX_OK;
+    const int grid = grid_for(d->n, 256, 8);
+    if (!has_value(d))
+        hipLaunchKernelGGL((points_global_kernel<TC, float, false, false>), dim3(grid), dim3(256), 0, s, a, out, mean);
+    else if (f32 && mean)
+        hipLaunchKernelGGL((points_global_kernel<TC, float, true, true>), dim3(grid), dim3(256), 0, s, a, out, mean);
+    else if (f32)
+        hipLaunchKernelGGL((points_global_kernel<TC, float, true, false>), dim3(grid), dim3(256), 0, s, a, out, mean);
+    else if (mean)
+        hipLaunchKernelGGL((points_global_kernel<TC, double, true, true>), dim3(grid), dim3(256), 0, s, a, out, mean);
+    else
+        hipLaunchKernelGGL((points_global_kernel<TC, double, true, false>), dim3(grid), dim3(256), 0, s, a, out, mean);
     return launch_status();
 }
 
@@ -1731,6 +1793,10 @@ extern "C" int dsx_points_agg(const dsx_points_desc *d, const dsx_view *v, const
     const Planes out = planes_of(p);
     hipStream_t s = as_stream(stream);
     uint8_t *ws = reinterpret_cast<uint8_t *>(workspace);
+    if (P.global) {
+        if (d->xy_dtype == DSX_F32) return run_global<float>(d, a, p, out, mean, s);
+        return run_global<double>(d, a, p, out, mean, s);
+    }
     if (d->xy_dtype == DSX_F32) return dispatch_val<float>(d, a, P, out, mean, ws, s);
     return dispatch_val<double>(d, a, P, out, mean, ws, s);
 }

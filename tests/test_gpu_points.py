@@ -197,6 +197,37 @@ def test_random_points_vs_oracle(dtype, shape):
     assert se <= vt, se
 
 
+@pytest.mark.parametrize("dtype", ["f4", "f8"])
+def test_canvas_beyond_the_tiles_vs_oracle(dtype):
+    """5000 x 4000 = 20M cells of count / countv / sum / min / max / m2 planes: more than
+    4096 LDS tiles hold, so the call takes the global-atomic pass; same results."""
+    W, H = 5000, 4000
+    rng = np.random.default_rng(21)
+    n = 2_000_003
+    x = rng.normal(0.0, 2.0, n).astype(dtype)
+    y = rng.normal(0.0, 2.0, n).astype(dtype)
+    v = rng.lognormal(0.0, 1.0, n).astype(dtype)
+    v[rng.choice(n, n // 100, replace=False)] = np.nan
+    df = pd.DataFrame({"x": x, "y": y, "v": v})
+    xr_, yr_ = (-5.0, 5.0), (-4.0, 4.0)
+    agg = dsa.summary(cnt=dsa.count(), cntv=dsa.count("v"), mean=dsa.mean("v"),
+                      mn=dsa.min("v"), mx=dsa.max("v"), var=dsa.var("v"))
+    ds = dsa.Canvas(W, H, x_range=xr_, y_range=yr_).points(df, "x", "y", agg)
+    view = G.view_for(W, H, xr_, yr_)
+    r = oracle.points(x, y, view, W, H, ("count", "countv", "sumz", "min", "max", "m2"), val=v)
+    np.testing.assert_array_equal(ds["cnt"].values, r["count"])
+    np.testing.assert_array_equal(ds["cntv"].values, r["countv"])
+    G.assert_same(ds["mn"].values, np.where(r["countv"] > 0, r["min"], np.nan))
+    G.assert_same(ds["mx"].values, np.where(r["countv"] > 0, r["max"], np.nan))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        m_exp = np.where(r["countv"] > 0, r["sumz"] / r["countv"], np.nan)
+        v_exp = np.where(r["countv"] > 0, r["m2"] / r["countv"], np.nan)
+    assert _rel_err(ds["mean"].values, m_exp) <= (1e-6 if dtype == "f4" else 1e-12)
+    big = r["countv"] >= 2
+    ve = _rel_err(np.where(big, ds["var"].values, np.nan), np.where(big, v_exp, np.nan))
+    assert ve <= (1e-5 if dtype == "f4" else 1e-9), ve
+
+
 def test_skewed_hot_bins_compensated_sum():
     """90% of the rows in a 0.01-sigma blob: thousands of rows per hot bin.  The float32
     compensated sum stays within 1e-6 of the float64 row-ordered sum."""
