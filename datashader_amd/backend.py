@@ -188,10 +188,8 @@ def _points_call(lib, d, vw, planes_struct, mean, device, stream, grid=None):
     """dsx_points_agg (or, over an indexed frame, dsx_points_agg_indexed) with a
     workspace from torch's caching allocator."""
     t = dev.torch()
-    if grid is not None:
-        ws = int(lib.dsx_points_indexed_workspace_size(d, vw, planes_struct, grid))
-        if ws == 0:
-            raise _abi.DsxError("dsx_points_indexed_workspace_size rejected the call")
+    ws = int(lib.dsx_points_indexed_workspace_size(d, vw, planes_struct, grid)) if grid else 0
+    if grid is not None and ws > 0:  # a view the lattice tiles cannot take: plain call
         work = t.empty(ws, dtype=t.uint8, device=device)
         _abi.check(_timed(lambda: lib.dsx_points_agg_indexed(
             d, vw, planes_struct, dev.ptr(mean), grid, dev.ptr(work), ws, stream)),
@@ -199,8 +197,8 @@ def _points_call(lib, d, vw, planes_struct, mean, device, stream, grid=None):
         return
     ws = int(lib.dsx_points_workspace_size(d, vw, planes_struct))
     if ws == 0:
-        raise _abi.DsxError("dsx_points_workspace_size rejected the call (canvas too large "
-                            "for the tile partition, or invalid descriptor)")
+        raise _abi.DsxError("dsx_points_workspace_size rejected the call (invalid "
+                            "descriptor or shape)")
     work = t.empty(ws, dtype=t.uint8, device=device)
     _abi.check(_timed(lambda: lib.dsx_points_agg(d, vw, planes_struct, dev.ptr(mean),
                                                  dev.ptr(work), ws, stream)),
