@@ -795,8 +795,9 @@ __global__ __launch_bounds__(kPartThreads) void spill_tilescan_kernel(PartArgs p
 // compiler counts the loads and stores in flight exactly, and the next round's loads (two
 // register sets alternate) stay in flight behind this round's stores.
 template <typename TV, bool HAS_VAL>
-struct ScatterCfg {  // entries per round: cell + value staging fits LDS next to the tile arrays
-    static constexpr int kSub = !HAS_VAL ? 16384 : (sizeof(TV) == 8 ? 8192 : 12288);
+struct ScatterCfg {  // entries per round: cell + value staging fits LDS next to the tile arrays,
+                     // and two register sets of them fit 128 VGPRs (no spills)
+    static constexpr int kSub = !HAS_VAL ? 20480 : (sizeof(TV) == 8 ? 8192 : 12288);
     static constexpr int kPer = kSub / kPartThreads;  // per thread
     static constexpr int kWv = kSub / kWaves;         // per wave
 };
