@@ -122,10 +122,38 @@ __global__ __launch_bounds__(256) void fill_kernel(T *p, T value, int64_t n) {
 }
 
 // ---- finalize -----------------------------------------------------------------------------
+// sum-based modes: four cells per thread per iteration, their loads issued together
+__device__ __forceinline__ double fin_value(int mode, uint32_t c, double s) {
+    switch (mode) {
+    case DSX_FIN_SUM: return c > 0u ? s : (double)NAN;
+    case DSX_FIN_SUMZERO: return s;
+    case DSX_FIN_MEAN: return c > 0u ? s / (double)c : (double)NAN;
+    case DSX_FIN_VAR: return c > 0u ? s / (double)c : (double)NAN;
+    default: return c > 0u ? sqrt(s / (double)c) : (double)NAN;
+    }
+}
+
 __global__ __launch_bounds__(256) void finalize_kernel(dsx_finalize_desc f) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const bool f32 = f.key_bytes == 4;
     double *out = reinterpret_cast<double *>(f.out);
+    if (f.mode != DSX_FIN_ANY && f.mode != DSX_FIN_MIN && f.mode != DSX_FIN_MAX) {
+        constexpr int kU = 4;
+        int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        for (; i + (kU - 1) * stride < f.ncells; i += kU * stride) {
+            uint32_t c[kU];
+            double s[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                c[u] = f.countv[i + u * stride];
+                s[u] = f.sum[i + u * stride];
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) out[i + u * stride] = fin_value(f.mode, c[u], s[u]);
+        }
+        for (; i < f.ncells; i += stride) out[i] = fin_value(f.mode, f.countv[i], f.sum[i]);
+        return;
+    }
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.ncells; i += stride) {
         switch (f.mode) {
         case DSX_FIN_ANY:
@@ -146,19 +174,8 @@ __global__ __launch_bounds__(256) void finalize_kernel(dsx_finalize_desc f) {
             out[i] = r;
             break;
         }
-        default: {
-            const uint32_t c = f.countv[i];
-            const double s = f.sum[i];
-            double r;
-            switch (f.mode) {
-            case DSX_FIN_SUM: r = c > 0u ? s : (double)NAN; break;
-            case DSX_FIN_SUMZERO: r = s; break;
-            case DSX_FIN_MEAN: r = c > 0u ? s / (double)c : (double)NAN; break;
-            case DSX_FIN_VAR: r = c > 0u ? s / (double)c : (double)NAN; break;
-            default: r = c > 0u ? sqrt(s / (double)c) : (double)NAN; break;
-            }
-            out[i] = r;
-        }
+        default:
+            break;  // the sum-based modes took the loop above
         }
     }
 }
