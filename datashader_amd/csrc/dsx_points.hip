@@ -1885,9 +1885,9 @@ __global__ __launch_bounds__(256) void rowindex_kernel(RiArgs r) {
 template <typename TC>
 static int rowindex_launch(const RiArgs &r, int val_dtype, hipStream_t st) {
     const bool arg = r.mode == DSX_RI_ARGMAX || r.mode == DSX_RI_ARGMIN;
-    // first / last: four workgroups per CU (fewer rows in flight for the skip test:
-    // 0.97 -> 0.89 ms on 100M rows; two per CU is slower again)
-    const int grid = grid_for(r.a.n < 1 ? 1 : r.a.n, 256, arg ? 8 : 4);
+    // four workgroups per CU: fewer rows in flight for the skip test (on 100M rows
+    // first 0.97 -> 0.90 ms, where(max) 1.05 -> 1.01 ms against eight; two is slower)
+    const int grid = grid_for(r.a.n < 1 ? 1 : r.a.n, 256, 4);
     if (val_dtype == DSX_F32) {
         if (arg) hipLaunchKernelGGL((rowindex_kernel<TC, float, 4>), dim3(grid), dim3(256), 0, st, r);
         else hipLaunchKernelGGL((rowindex_kernel<TC, float, 1>), dim3(grid), dim3(256), 0, st, r);
