@@ -23,6 +23,8 @@ NS_FIRST, NS_LAST, NS_MAX, NS_MIN = 1, 2, 3, 4
 # shade kinds and transforms
 SHADE_CMAP, SHADE_COLOR, SHADE_CATEGORICAL, SHADE_COLOR_KEY = 0, 1, 2, 3
 HOW_CODES = {"linear": 0, "log": 1, "cbrt": 2, "eq_hist": 3}
+# lines: plot_start of a chunk whose first vertex is context only (axis 0)
+PLOT_CONTEXT = -1
 # fill modes
 FILL_ZERO32, FILL_ONES32, FILL_ZERO64, FILL_ONES64 = 1, 2, 3, 4
 # finalize modes

@@ -155,17 +155,24 @@ _WINDOW_CACHE_MAX = 64
 def _window_slot(frame, key, device):
     """(device int32[4] window, reuse flag) of the hot-window cache kept on a
     DeviceFrame.  The window only steers performance (which canvas cells are kept in
-    LDS during the scan); any stored window gives the same result."""
+    LDS during the scan); any stored window gives the same result.  A call copies the
+    stored window into its own workspace before any of its kernels reads it, so calls
+    on several streams may share a slot; the slot is marked as used by the current
+    stream, so that evicting it cannot hand its memory to another allocation while a
+    call on that stream still reads or writes it."""
     t = dev.torch()
     cache = frame.__dict__.setdefault("_window_cache", OrderedDict())
     slot = cache.get(key)
+    reuse = 1
     if slot is None:
         while len(cache) >= _WINDOW_CACHE_MAX:  # oldest first: a server sees many views
             cache.popitem(last=False)
         slot = cache[key] = t.zeros(4, dtype=t.int32, device=device)
-        return slot, 0
-    cache.move_to_end(key)
-    return slot, 1
+        reuse = 0
+    else:
+        cache.move_to_end(key)
+    slot.record_stream(t.cuda.current_stream(slot.device))
+    return slot, reuse
 
 
 def _grid_ref(frame, glyph, p, view):
@@ -231,7 +238,8 @@ def _pass_desc(frame, schema, glyph, view, p, device, catc):
     d.x_line_stride, d.x_vert_stride = lay["xls"], lay["xvs"]
     d.y_line_stride, d.y_vert_stride = lay["yls"], lay["yvs"]
     d.axis = lay["axis"]
-    d.plot_start = 1
+    # axis 0: a streamed chunk of a longer line carries its own plot_start
+    d.plot_start = int(getattr(frame, "plot_start", 1))
     d.val, d.val_dtype = dev.ptr(val), vdt
     d.cat = catc
     return d, (lay, val)

@@ -127,9 +127,16 @@ __device__ __forceinline__ void draw_segment(const LineArgs &a, const S &put, bo
     double t0 = 0.0, t1 = 1.0;
     const double dx1 = (double)(TX)(x1c - x0c);  // column precision (numba typing)
     const double dy1 = (double)(TY)(y1c - y0c);
-    // both ends inside the view: every _clipt call leaves (t0, t1) = (0, 1), so skip them
+    // Both ends inside the view and each delta equal to the float64 difference of its
+    // ends: then every _clipt call leaves (t0, t1) = (0, 1) and they are skipped.  (With
+    // x1 <= xmax, the exact xmax - x0 is >= the exact x1 - x0, so the rounded q = xmax - x0
+    // is >= p = fl64(x1 - x0); _clipt's q < p branch cannot fire, and likewise for the
+    // other three bounds.)  A float32 delta that rounds away from the float64 one can
+    // exceed q when a vertex lies on a bound; _clipt then clips (t1 < 1), so such a
+    // segment takes the four calls.
     const bool inside = a.xmin <= x0 && x0 <= a.xmax && a.xmin <= x1 && x1 <= a.xmax &&
-                        a.ymin <= y0 && y0 <= a.ymax && a.ymin <= y1 && y1 <= a.ymax;
+                        a.ymin <= y0 && y0 <= a.ymax && a.ymin <= y1 && y1 <= a.ymax &&
+                        dx1 == __dsub_rn(x1, x0) && dy1 == __dsub_rn(y1, y0);
     if (!inside) {
         if (!clipt(-dx1, __dsub_rn(x0, a.xmin), t0, t1)) skip = true;
         if (!clipt(dx1, __dsub_rn(a.xmax, x0), t0, t1)) skip = true;
@@ -171,6 +178,7 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a, Planes p) {
     const TY *Y = reinterpret_cast<const TY *>(a.y);
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += stride) {
         const int64_t l = s / nseg, k = s - l * nseg;
+        if (k == 0 && a.axis == 0 && a.plot_start == DSX_PLOT_CONTEXT) continue;  // context vertex
         const TX x0 = X[l * a.xls + k * a.xvs], x1 = X[l * a.xls + (k + 1) * a.xvs];
         const TY y0 = Y[l * a.yls + k * a.yvs], y1 = Y[l * a.yls + (k + 1) * a.yvs];
         bool start;

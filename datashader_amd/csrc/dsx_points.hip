@@ -217,6 +217,12 @@ __global__ void set_window_kernel(int32_t *w, int32_t x0, int32_t y0, int32_t ww
     if (threadIdx.x == 0) { w[0] = x0; w[1] = y0; w[2] = ww; w[3] = wh; }
 }
 
+// Copy a window [x0, y0, w, h] (a call's snapshot of a stored window, or the window a
+// call picked back into the caller's store).
+__global__ void copy_window_kernel(const int32_t *src, int32_t *dst) {
+    if (threadIdx.x < 4) dst[threadIdx.x] = src[threadIdx.x];
+}
+
 // Coarse density histogram of a sample of rows: kSampleRuns runs of kSampleRun
 // consecutive rows spread evenly over the input; bins of (1 << bshift)^2 pixels.
 template <typename TC>
@@ -1257,7 +1263,10 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
 template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
 static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &P,
                       const Planes &out, const double *mean, uint8_t *ws, hipStream_t st) {
-    int32_t *window = d->window ? d->window : reinterpret_cast<int32_t *>(ws + P.o_window);
+    // Every kernel of this call reads the window from the call's own workspace: a stored
+    // window (d->window) is copied in first, so that calls on other streams that share
+    // the store and rewrite it cannot hand one call's kernels two different windows.
+    int32_t *window = reinterpret_cast<int32_t *>(ws + P.o_window);
     uint32_t *hist_sel = reinterpret_cast<uint32_t *>(ws + P.o_hist_sel);
     uint32_t *seg_count = reinterpret_cast<uint32_t *>(ws + P.o_count);
     const int ncat = a.cat.ncat;
@@ -1267,6 +1276,8 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
         hipLaunchKernelGGL(set_window_kernel, dim3(1), dim3(64), 0, st, window, 0, 0, a.W, a.H);
     } else if (d->window && d->reuse_window) {
         // the caller's stored window (sanitised by every kernel that reads it)
+        hipLaunchKernelGGL(copy_window_kernel, dim3(1), dim3(64), 0, st,
+                           (const int32_t *)d->window, window);
     } else {
         if ((s = fill_async(hist_sel, DSX_FILL_ZERO32, kSelBins, st))) return s;
         hipLaunchKernelGGL((select_hist_kernel<TC>), dim3(64), dim3(1024), 0, st, a, P.bshift,
@@ -1274,6 +1285,9 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
         hipLaunchKernelGGL(select_pick_kernel, dim3(1), dim3(1024), 0, st, (const uint32_t *)hist_sel,
                            P.bshift, P.cw, P.ch, a.W, a.H, (int64_t)(P.wl.cap / (uint32_t)ncat),
                            window);
+        if (d->window)
+            hipLaunchKernelGGL(copy_window_kernel, dim3(1), dim3(64), 0, st,
+                               (const int32_t *)window, d->window);
     }
     if ((s = launch_status())) return s;
     ScanArgs sa;

@@ -17,8 +17,16 @@ partitions first).  var / std stream the table twice: the second pass accumulate
 squared deviations from the final per-cell means (the deviations of every chunk from the
 same mean add, so the merge is exact in the arithmetic sense).
 
-Glyphs whose rows are independent stream: points and the axis-1 line layouts (one line
-per row).  Axis-0 lines join consecutive rows and are not taken (NotImplementedError).
+Points and the axis-1 line layouts (one line per row) have independent rows.  Axis-0
+lines join consecutive rows, so each chunk after the first is uploaded with the two rows
+before it in front: the first is context only (``DSX_PLOT_CONTEXT``: not drawn, its NaN
+test gives the next segment its segment_start), so the segment that joins the chunks is
+drawn once, exactly as in the unchunked line (a chunk boundary is invisible).  Between
+partitions the reference's dask stitching is followed (``data_libraries/dask.py:222-266``):
+a ``LineAxis0`` partition is drawn as the previous partition's last row followed by its
+own rows, with ``plot_start=False``; ``LinesAxis0Multi`` partitions are drawn one by one
+(dask's default path, ``dask.py:86-219``).  Row-index reductions (first, last, where,
+the n-value reductions) are not streamed (NotImplementedError).
 """
 from __future__ import annotations
 
@@ -33,7 +41,8 @@ from . import _device as dev
 from . import backend
 from .core import _cols_to_keep, bypixel
 from .frame import DeviceFrame, Schema
-from .glyphs import LinesAxis1, LinesAxis1XConstant, LinesAxis1YConstant, Point
+from .glyphs import (LineAxis0, LineAxis0Multi, LinesAxis1, LinesAxis1XConstant,
+                     LinesAxis1YConstant, Point)
 
 
 class ChunkedFrame:
@@ -82,32 +91,59 @@ class ChunkedFrame:
                 out[n] = np.ascontiguousarray(s.to_numpy())
         return out
 
-    def chunks(self, names):
-        """{name: numpy view} per chunk, partition by partition."""
-        for part in self.partitions:
+    def chunks(self, names, stitch=None):
+        """(cols, head, plot_start) per chunk, partition by partition: ``cols`` is
+        {name: numpy view} of the chunk's rows; ``head`` is None or {name: array of 1 or
+        2 rows} to upload in front of them; ``plot_start`` is the line kernel's
+        plot_start for the uploaded rows (axis-0 lines).
+
+        ``stitch``: None (rows independent), "axis0" (``LineAxis0``: partitions joined as
+        dask joins them) or "multi" (``LinesAxis0Multi``: partitions drawn one by one).
+        Within a partition a chunk gets the two rows before it in front, the first as
+        context (``DSX_PLOT_CONTEXT``), or the one row before it with the partition's own
+        plot_start when that row opens the partition's line."""
+        prev_last = None  # last row of the previous partition (dask's df.iloc[-1:])
+        for ip, part in enumerate(self.partitions):
             cols = self._host_columns(part, names)
             n = len(part)
+            # LineAxis0: partition i > 0 is drawn as concat(previous partition's last
+            # row, its rows) with plot_start False (dask.py:242-248), also when the
+            # previous partition is empty and hands on no row
+            tail = [prev_last] if stitch == "axis0" and prev_last is not None else []
+            part_start = not (stitch == "axis0" and ip > 0)
             for a in range(0, n, self.chunk_rows):
                 b = min(n, a + self.chunk_rows)
-                yield {k: v[a:b] for k, v in cols.items()}
+                chunk = {k: v[a:b] for k, v in cols.items()}
+                if stitch is None or not tail:
+                    yield chunk, None, part_start
+                else:
+                    head = {k: np.concatenate([r[k] for r in tail]) for k in cols}
+                    yield chunk, head, (_abi.PLOT_CONTEXT if len(tail) == 2 else part_start)
+                if stitch is not None:
+                    rows = [{k: v[j:j + 1].copy() for k, v in cols.items()}
+                            for j in range(max(a, b - 2), b)]
+                    tail = (tail + rows)[-2:]
+            prev_last = {k: v[n - 1:n].copy() for k, v in cols.items()} if n else None
 
 
 class _Uploader:
     """Two pinned staging slots + two device slots; yields one DeviceFrame per chunk
     with its upload overlapped with the previous chunk's aggregation."""
 
-    def __init__(self, source, names, workers=8):
+    def __init__(self, source, names, workers=8, stitch=None):
         t = dev.torch()
         self.t = t
         self.source = source
         self.names = list(names)
+        self.stitch = stitch
         part0 = source.partitions[0]
         probe = source._host_columns(part0.iloc[:1], self.names)
         self.dtypes = {n: probe[n].dtype for n in self.names}
         self.cats = {n: source.schema.categories(n) for n in self.names
                      if source.schema[n] == "category"}
         longest = max(len(p) for p in source.partitions)
-        self.cap = max(1, min(source.chunk_rows, longest))
+        # (+2: an axis-0 chunk is uploaded behind the two rows before it)
+        self.cap = max(1, min(source.chunk_rows, longest)) + (2 if stitch else 0)
         groups = OrderedDict()
         for n in self.names:
             groups.setdefault(self.dtypes[n], []).append(n)
@@ -129,19 +165,23 @@ class _Uploader:
         self.pool = ThreadPoolExecutor(max_workers=workers)
         self.workers = workers
 
-    def _stage(self, slot, chunk):
+    def _stage(self, slot, chunk, head=None):
         t = self.t
-        m = len(next(iter(chunk.values())))
+        h = 0 if head is None else len(next(iter(head.values())))
+        m = len(next(iter(chunk.values()))) + h
         if self.h2d[slot] is not None:
             self.h2d[slot].synchronize()  # the pinned slot's previous DMA has finished
         jobs = []
         for dt, ns in self.groups.items():
             host = self.pinned[slot][dt].numpy()
             for j, n in enumerate(ns):
-                src, dst = chunk[n], host[j]
-                step = max(1, -(-m // self.workers))
-                for a in range(0, m, step):
-                    b = min(m, a + step)
+                if h:
+                    host[j][:h] = head[n]
+                src, dst = chunk[n], host[j][h:]
+                m0 = m - h
+                step = max(1, -(-m0 // self.workers))
+                for a in range(0, m0, step):
+                    b = min(m0, a + step)
                     jobs.append(self.pool.submit(np.copyto, dst[a:b], src[a:b]))
         for f in jobs:
             f.result()
@@ -159,11 +199,12 @@ class _Uploader:
 
     def frames(self, window_cache=None):
         t = self.t
-        it = self.source.chunks(self.names)
+        it = self.source.chunks(self.names, self.stitch)
         first = next(it, None)
         if first is None:
             return
-        m = self._stage(0, first)
+        m = self._stage(0, first[0], first[1])
+        plot_start = first[2]
         slot = 0
         while True:
             t.cuda.current_stream().wait_event(self.h2d[slot])
@@ -172,6 +213,7 @@ class _Uploader:
                 for j, n in enumerate(ns):
                     cols[n] = self.device[slot][dt][j, :m]
             frame = DeviceFrame({n: cols[n] for n in self.names}, self.cats)
+            frame.plot_start = plot_start
             if window_cache is not None:
                 frame.__dict__["_window_cache"] = window_cache  # one tuning state per stream
             yield frame
@@ -182,7 +224,8 @@ class _Uploader:
             if nxt is None:
                 break
             slot = 1 - slot
-            m = self._stage(slot, nxt)  # overlaps the aggregation just enqueued
+            m = self._stage(slot, nxt[0], nxt[1])  # overlaps the aggregation just enqueued
+            plot_start = nxt[2]
 
     def close(self):
         self.pool.shutdown(wait=False)
@@ -244,10 +287,19 @@ def pipeline(source, schema, canvas, glyph, agg, *, antialias=False, to_host=Tru
     """``bypixel.pipeline`` for a ``ChunkedFrame``: stream, aggregate, combine, finalize."""
     if antialias:
         raise NotImplementedError("antialiased lines are not supported by the MI355X backend")
-    if not isinstance(glyph, (Point, LinesAxis1)):
+    if not isinstance(glyph, (Point, LinesAxis1, LineAxis0, LineAxis0Multi)):
         raise NotImplementedError(
-            f"{type(glyph).__name__} joins consecutive rows and is not streamed in chunks; "
+            f"{type(glyph).__name__} is not streamed in chunks; upload the table as a "
+            "DeviceFrame")
+    passes, leaves = backend.plan(agg)
+    rowindex = [type(red.reduction if hasattr(red, "reduction") else red).__name__
+                for (_, red, key) in leaves if key[0] == "rowindex"]
+    if rowindex:
+        raise NotImplementedError(
+            f"row-index reductions ({', '.join(rowindex)}) are not streamed in chunks; "
             "upload the table as a DeviceFrame")
+    stitch = ("axis0" if isinstance(glyph, LineAxis0) else
+              "multi" if isinstance(glyph, LineAxis0Multi) else None)
     dev.require_gpu()
     lib = _abi.load()
     x_range, y_range = canvas.x_range, canvas.y_range
@@ -261,11 +313,10 @@ def pipeline(source, schema, canvas, glyph, agg, *, antialias=False, to_host=Tru
     canvas.validate_ranges(x_range, y_range)
     view = backend.make_view(canvas, x_range, y_range)
     H, W = view["H"], view["W"]
-    passes, leaves = backend.plan(agg)
     names = [c for c in _cols_to_keep(source.columns, glyph, agg)]
     cache = OrderedDict()
     totals = {}
-    up = _Uploader(source, names)
+    up = _Uploader(source, names, stitch=stitch)
     try:
         for frame in up.frames(cache):
             for key, p in passes.items():

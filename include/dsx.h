@@ -120,7 +120,9 @@ typedef struct {
     /* Hot window (a performance hint; results do not depend on it): a device int32[4]
      * {x0, y0, w, h} in pixels, or NULL.  reuse_window = 0: the call picks the window
      * from a sample of the rows and, when window != NULL, stores it there; 1: the call
-     * uses the stored window (an invalid one is treated as empty).                     */
+     * uses the stored window (an invalid one is treated as empty).  Either way every
+     * kernel of the call reads the call's own copy in its workspace (copied in, or picked
+     * there and copied out), so calls on several streams may share one store.          */
     int32_t *window;
     int32_t reuse_window;
 } dsx_points_desc;
@@ -151,13 +153,19 @@ int dsx_points_agg(const dsx_points_desc *d, const dsx_view *v, const dsx_planes
  * Segment (l, k) -> (l, k+1) starts a new run when k == 0 (then segment_start =
  * plot_start for axis 0 and 1 for axis 1) or when vertex k-1 has a NaN coordinate.
  * The value / category index of a segment is k for axis 0 and l for axis 1.          */
+#define DSX_PLOT_CONTEXT (-1)
+
 typedef struct {
     int64_t nlines, nverts;
     const void *x, *y;
     int32_t x_dtype, y_dtype;     /* DSX_F32 | DSX_F64 each                               */
     int64_t x_line_stride, x_vert_stride, y_line_stride, y_vert_stride;
     int32_t axis;                 /* 0 or 1                                               */
-    int32_t plot_start;           /* axis 0: draw the first vertex                        */
+    int32_t plot_start;           /* axis 0: 1 draw the first vertex, 0 do not;            */
+                                  /* DSX_PLOT_CONTEXT: vertex 0 only precedes the line     */
+                                  /* (segment 0 is not drawn, and vertex 0's NaN test      */
+                                  /* gives segment 1 its segment_start) -- a chunk of a    */
+                                  /* longer line uploaded behind its two previous rows     */
     const void *val; int32_t val_dtype;
     dsx_categorizer cat;          /* indexed like val                                     */
     const double *mean;           /* NULL, or per-cell means: then the only plane given,   */

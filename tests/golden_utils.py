@@ -400,3 +400,45 @@ def channels_close(a, b, tol):
     x = np.asarray(a, dtype=np.uint32).view(np.uint8).astype(int)
     y = np.asarray(b, dtype=np.uint32).view(np.uint8).astype(int)
     return x.shape == y.shape and bool(np.all(np.abs(x - y) <= tol))
+
+
+# ---- float32 segments ending exactly on a view bound --------------------------------
+
+def f32_bound_cases():
+    """Two-segment float32 polylines whose last vertex lies exactly on one of the four
+    view bounds, with a float32 segment delta that rounds above the float64 distance q
+    from the segment's start to that bound.
+
+    Under the reference's typing (``dx1 = x1 - x0`` in the column's float32,
+    ``q = x0 - xmin`` / ``xmax - x0`` in float64, ``line.py:752-759``), ``_clipt``
+    takes its ``q < p`` branch (``line.py:792-798``), sets ``t1 = q / p < 1`` and the
+    second segment is clipped; both its ends snap to one pixel, so it draws nothing
+    (``line.py:1003-1008``: a clipped segment without ``segment_start`` appends only
+    while stepping).  With float64 deltas the same segment is unclipped and zero-length
+    and appends its pixel once.  The first segment lies inside the view and, by
+    Bresenham, touches max(|dx|, |dy|) + 1 = 4 pixels either way.  So every case draws
+    4 pixels in float32 mode and 5 in float64 mode.
+
+    The base case puts the last vertex on ymax on a 34 x 7 canvas; the others mirror it
+    (y -> -y) and swap the axes.  The premise p > q is checked with numpy's float32 and
+    float64 arithmetic for each case."""
+    f = np.float32
+    bx = np.array([0.30, 0.3397695, 0.33822325], f)
+    by = np.array([-0.5, -0.027073061, 0.043172628], f)
+    top = float(by[2])
+    base = [
+        ("ymax", bx, by, 34, 7, (0.0, 1.0), (-1.0, top), "y", +1),
+        ("ymin", bx, -by, 34, 7, (0.0, 1.0), (-top, 1.0), "y", -1),
+        ("xmax", by, bx, 7, 34, (-1.0, top), (0.0, 1.0), "x", +1),
+        ("xmin", -by, bx, 7, 34, (-top, 1.0), (0.0, 1.0), "x", -1),
+    ]
+    out = []
+    for name, xs, ys, W, H, xr, yr, axis, side in base:
+        c = ys if axis == "y" else xs
+        lo, hi = (yr if axis == "y" else xr)
+        p = float(f(c[2]) - f(c[1])) * side          # float32 delta toward the bound
+        q = (hi - float(c[1])) if side > 0 else (float(c[1]) - lo)  # float64 distance
+        assert c[2] == (hi if side > 0 else lo) and p > q > 0, name
+        out.append(dict(name=name, xs=xs, ys=ys, W=W, H=H, x_range=xr, y_range=yr,
+                        total_f32=4, total_f64=5))
+    return out

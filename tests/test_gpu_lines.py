@@ -24,16 +24,17 @@ GC = G.cases()
 
 
 def abi_lines(xs, ys, view, W, H, axis=0, plot_start=True, x_ls=0, x_vs=1, y_ls=0, y_vs=1,
-              nlines=1, nverts=None):
+              nlines=1, nverts=None, xdtype=np.float64, ydtype=np.float64):
     """One dsx_lines_agg call (count plane) on explicit device buffers."""
     lib = _abi.load()
-    xt = torch.as_tensor(np.ascontiguousarray(xs, np.float64), device="cuda")
-    yt = torch.as_tensor(np.ascontiguousarray(ys, np.float64), device="cuda")
+    xt = torch.as_tensor(np.ascontiguousarray(xs, xdtype), device="cuda")
+    yt = torch.as_tensor(np.ascontiguousarray(ys, ydtype), device="cuda")
     cnt = torch.zeros(H * W, dtype=torch.int32, device="cuda")
     d = _abi.LinesDesc()
     d.nlines, d.nverts = nlines, (len(xs) if nverts is None else nverts)
     d.x, d.y = _device.ptr(xt), _device.ptr(yt)
-    d.x_dtype = d.y_dtype = _abi.F64
+    d.x_dtype = _abi.F32 if xdtype == np.float32 else _abi.F64
+    d.y_dtype = _abi.F32 if ydtype == np.float32 else _abi.F64
     d.x_line_stride, d.x_vert_stride, d.y_line_stride, d.y_vert_stride = x_ls, x_vs, y_ls, y_vs
     d.axis, d.plot_start = axis, int(plot_start)
     v = _abi.View()
@@ -293,3 +294,88 @@ def _oracle_frame_case(case, df):
         return G.line_case_oracle(case)
     finally:
         G.line_frame = orig
+
+
+# ---- float32 vertices exactly on a view bound -----------------------------------------
+# The reference computes a segment's delta in the column's float32 and its distance to
+# each bound in float64 (line.py:752-759); a vertex on a bound whose float32 delta rounds
+# above that distance makes _clipt clip the segment (line.py:792-798), which changes the
+# pixels of a zero-length segment (line.py:1003-1008).
+
+@pytest.mark.parametrize("case", G.f32_bound_cases(), ids=lambda c: c["name"])
+def test_f32_vertex_on_bound_axis0(case):
+    W, H = case["W"], case["H"]
+    view = G.view_for(W, H, case["x_range"], case["y_range"], xf32=True, yf32=True)
+    exp = oracle.line_axis0(case["xs"].astype(float), case["ys"].astype(float), view, W, H)["count"]
+    assert exp.sum() == case["total_f32"]
+    got = abi_lines(case["xs"], case["ys"], view, W, H, xdtype=np.float32, ydtype=np.float32)
+    np.testing.assert_array_equal(got, exp)
+    df = pd.DataFrame({"x": case["xs"], "y": case["ys"]})
+    agg = dsa.Canvas(W, H, x_range=case["x_range"], y_range=case["y_range"]).line(
+        df, "x", "y", agg=dsa.count())
+    np.testing.assert_array_equal(agg.values, exp)
+
+
+@pytest.mark.parametrize("xdtype", [np.float32, np.float64])
+@pytest.mark.parametrize("case", G.f32_bound_cases(), ids=lambda c: c["name"])
+def test_f32_vertex_on_bound_xconst_strips(case, xdtype):
+    """The same polylines as x-constant axis-1 lines (the LDS-strip kernel): many lines
+    sharing x, every one with the case's float32 y."""
+    W, H = case["W"], case["H"]
+    nlines = 3000
+    xs = case["xs"].astype(xdtype)
+    ys = np.tile(case["ys"], (nlines, 1))
+    view = G.view_for(W, H, case["x_range"], case["y_range"], xf32=xdtype == np.float32,
+                      yf32=True)
+    exp = oracle.lines_xconst(xs.astype(float), ys.astype(float), view, W, H)["count"]
+    one = oracle.line_axis0(xs.astype(float), case["ys"].astype(float), view, W, H)["count"]
+    np.testing.assert_array_equal(exp, one * nlines)
+    got = abi_lines(xs, ys.T, view, W, H, axis=1, x_ls=0, x_vs=1, y_ls=1, y_vs=nlines,
+                    nlines=nlines, nverts=3, xdtype=xdtype, ydtype=np.float32)
+    np.testing.assert_array_equal(got, exp)
+    names = ["y0", "y1", "y2"]
+    agg = dsa.Canvas(W, H, x_range=case["x_range"], y_range=case["y_range"]).line(
+        dsa.DeviceFrame.from_pandas(pd.DataFrame(ys, columns=names)), x=xs, y=names,
+        agg=dsa.count(), axis=1)
+    np.testing.assert_array_equal(agg.values, exp)
+
+
+@pytest.mark.parametrize("side", ["ymin", "ymax"])
+def test_f32_random_vertices_on_bounds_vs_oracle(side):
+    """Random float32 lines whose vertices sit on a bound next to vertices of the other
+    sign inside the bound's pixel (rounded float32 deltas): x-constant strips, per-line x
+    (the per-segment kernel) and one axis-0 polyline, all against the oracle's float32
+    typing."""
+    rng = np.random.default_rng(31 if side == "ymin" else 32)
+    f = np.float32
+    W, H = 9, 7
+    b = float(f(0.043172628))
+    yr = (-b, 0.5) if side == "ymin" else (-0.5, b)
+    bound = f(yr[0] if side == "ymin" else yr[1])
+    xr = (0.0, 1.0)
+    nlines, nverts = 4000, 6
+    xs = np.sort(rng.uniform(0.23, 0.32, nverts)).astype(f)  # one pixel column
+    ys = rng.uniform(yr[0], yr[1], (nlines, nverts)).astype(f)
+    pick = rng.random((nlines, nverts)) < 0.4
+    ys[pick] = bound
+    near = (rng.random((nlines, nverts)) < 0.4) & ~pick
+    ys[near] = (-bound * rng.uniform(0.05, 0.75, near.sum())).astype(f)
+    view = G.view_for(W, H, xr, yr, xf32=True, yf32=True)
+    exp = oracle.lines_xconst(xs.astype(float), ys.astype(float), view, W, H)["count"]
+    strip = abi_lines(xs, ys.T, view, W, H, axis=1, x_ls=0, x_vs=1, y_ls=1, y_vs=nlines,
+                      nlines=nlines, nverts=nverts, xdtype=np.float32, ydtype=np.float32)
+    np.testing.assert_array_equal(strip, exp)
+    general = abi_lines(np.tile(xs, (nlines, 1)), ys, view, W, H, axis=1, x_ls=nverts,
+                        x_vs=1, y_ls=nverts, y_vs=1, nlines=nlines, nverts=nverts,
+                        xdtype=np.float32, ydtype=np.float32)
+    np.testing.assert_array_equal(general, exp)
+    # the same vertices as one long axis-0 polyline
+    px = np.tile(xs, nlines)
+    py = ys.reshape(-1)
+    exp0 = oracle.line_axis0(px.astype(float), py.astype(float), view, W, H)["count"]
+    got0 = abi_lines(px, py, view, W, H, xdtype=np.float32, ydtype=np.float32)
+    np.testing.assert_array_equal(got0, exp0)
+    # the float64 typing of the same vertices differs (the case is not vacuous)
+    v64 = G.view_for(W, H, xr, yr)
+    assert not np.array_equal(exp, oracle.lines_xconst(xs.astype(float), ys.astype(float),
+                                                        v64, W, H)["count"])

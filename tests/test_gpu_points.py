@@ -356,3 +356,63 @@ def test_window_cache_reuse_and_stale_hint():
         np.testing.assert_allclose(third["m"].values, first["m"].values, rtol=1e-12)
     for k in ("n", "m", "hi"):
         np.testing.assert_array_equal(second[k].values, first[k].values)
+
+
+def test_by_category_1024x1024x16_vs_oracle():
+    """by('cat', count()) at the categorical workload's own shape: 1024 x 1024 x 16
+    cells (64 MB of planes, about a thousand LDS tiles) from 4M ridge rows with Zipf
+    categories, ranges = the data extent; bit-exact vs the oracle."""
+    from datashader_amd import synthetic
+    n = 4_000_000
+    x, y, cat = synthetic.points_ridges_np(n, seed=3)
+    cats = [f"c{k}" for k in range(16)]
+    frame = dsa.DeviceFrame({"x": torch.as_tensor(x, device="cuda"),
+                             "y": torch.as_tensor(y, device="cuda"),
+                             "cat": torch.as_tensor(cat, device="cuda")},
+                            categories={"cat": cats})
+    W = H = 1024
+    agg = dsa.Canvas(W, H).points(frame, "x", "y", dsa.by("cat", dsa.count()))
+    xr_, yr_ = agg.attrs["x_range"], agg.attrs["y_range"]
+    assert xr_ == G.maybe_expand(oracle.bounds(x)) and yr_ == G.maybe_expand(oracle.bounds(y))
+    view = G.view_for(W, H, xr_, yr_)
+    exp = oracle.points(x, y, view, W, H, ("count",), cat=cat.astype(np.int64), ncat=16,
+                        squeeze=False)["count"]
+    got = agg.values
+    assert got.shape == (H, W, 16)
+    np.testing.assert_array_equal(got, exp)
+    assert int(exp.sum()) == n and (exp > 0).sum() > 1_000_000
+
+
+def test_skewed_min_max_std_vs_oracle():
+    """The float-reduction workload's shape (1920 x 1080, 90% of the rows in a 0.01-sigma
+    blob, lognormal float32 values) at 4M rows: min and max bit-exact, var / std within
+    1e-5 relative of the two-pass float64 closed form, sum / mean within 1e-6."""
+    from datashader_amd import synthetic
+    n = 4_000_000
+    x, y, v = synthetic.points_skewed_np(n, seed=2)
+    W, H = 1920, 1080
+    xr_ = (float(np.nanmin(x)), float(np.nanmax(x)))
+    yr_ = (float(np.nanmin(y)), float(np.nanmax(y)))
+    frame = dsa.DeviceFrame({k: torch.as_tensor(a, device="cuda")
+                             for k, a in (("x", x), ("y", y), ("v", v))})
+    ds = dsa.Canvas(W, H, x_range=xr_, y_range=yr_).points(
+        frame, "x", "y", dsa.summary(c=dsa.count(), s=dsa.sum("v"), m=dsa.mean("v"),
+                                     lo=dsa.min("v"), hi=dsa.max("v"), sd=dsa.std("v"),
+                                     var=dsa.var("v")))
+    view = G.view_for(W, H, xr_, yr_)
+    bins = cf.point_bins(x, y, view, W, H)
+    np.testing.assert_array_equal(ds["c"].values, cf.count(bins, W, H))
+    mn, mx = cf.minmax(bins, v, W, H)
+    G.assert_same(ds["lo"].values, mn)
+    G.assert_same(ds["hi"].values, mx)
+    assert _rel_err(ds["s"].values, cf.sum_nan(bins, v, W, H)) <= 1e-6
+    assert _rel_err(ds["m"].values, cf.mean(bins, v, W, H)) <= 1e-6
+    var = cf.var2pass(bins, v, W, H)
+    assert _rel_err(ds["var"].values, var) <= 1e-5
+    assert _rel_err(ds["sd"].values, np.sqrt(var)) <= 1e-5
+    # and the reference's own sequential m2 (reductions.py:1128-1137) via the oracle
+    r = oracle.points(x, y, view, W, H, ("countv", "m2"), val=v)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        seq = np.where(r["countv"] > 0, np.sqrt(r["m2"] / r["countv"]), np.nan)
+    assert _rel_err(ds["sd"].values, seq) <= 1e-5
+    assert ds["c"].values.max() > 1000

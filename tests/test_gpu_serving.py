@@ -120,3 +120,42 @@ def test_indexed_all_reductions(resident):
             np.testing.assert_array_equal(np.isnan(va), np.isnan(vb))
             ok = ~np.isnan(va)
             np.testing.assert_allclose(va[ok], vb[ok], rtol=tol, atol=1e-12)
+
+
+def test_indexed_vs_oracle_at_serving_canvases(resident):
+    """dsx_points_agg_indexed (Canvas.points over the indexed frame) against the oracle on
+    the unsorted rows, at the serving workload's canvases {256^2, 512^2, 1024x768,
+    1920x1080} and views drawn by make_requests: counts bit-exact, means within 1e-12
+    (float32 values summed in float64, another order)."""
+    import oracle
+    import golden_utils as G
+    from datashader_amd import backend
+    frame, index = resident
+    x = frame["x"].cpu().numpy()
+    y = frame["y"].cpu().numpy()
+    v = frame["v"].cpu().numpy()
+    rows = (index.frame["x"][:index.indexed].cpu().numpy(),
+            index.frame["y"][:index.indexed].cpu().numpy())
+    sizes = ((256, 256), (512, 512), (1024, 768), (1920, 1080))
+    reqs = serving.make_requests(12, rows, ((index.x0, index.x1), (index.y0, index.y1)),
+                                 seed=11, sizes=sizes, zoom_max_log2=6)
+    reqs += [serving.Request((index.x0, index.x1), (index.y0, index.y1), w, h, reqs[0].agg)
+             for w, h in sizes]
+    seen = set()
+    for req in reqs:
+        W, H = req.width, req.height
+        seen.add((W, H))
+        view = backend.make_view(req.canvas(), req.x_range, req.y_range)
+        p = backend.Pass("v", None)
+        assert backend._grid_ref(index.frame, dsa.glyphs.Point("x", "y"), p, view) is not None
+        got = serving.run_indexed(index, req)
+        ov = G.view_for(W, H, req.x_range, req.y_range)
+        r = oracle.points(x, y, ov, W, H, ("count", "countv", "sumz"), val=v)
+        np.testing.assert_array_equal(got["count"].values, r["count"])
+        with np.errstate(divide="ignore", invalid="ignore"):
+            mean = np.where(r["countv"] > 0, r["sumz"] / r["countv"], np.nan)
+        gm = got["mean"].values
+        np.testing.assert_array_equal(np.isnan(gm), np.isnan(mean))
+        ok = ~np.isnan(mean)
+        np.testing.assert_allclose(gm[ok], mean[ok], rtol=1e-12, atol=0)
+    assert seen == set(sizes)

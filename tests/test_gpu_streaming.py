@@ -82,7 +82,60 @@ def test_lines_axis1_stream_equals_resident():
         _close(streamed.values, resident.values, 1e-12)
 
 
-def test_axis0_lines_are_not_streamed():
-    df = pd.DataFrame({"x": np.arange(10.0), "y": np.arange(10.0)})
-    with pytest.raises(NotImplementedError):
-        dsa.Canvas(10, 10).line(dsa.ChunkedFrame(df, chunk_rows=3), "x", "y")
+def _walk(n, seed, nan_every=0):
+    rng = np.random.default_rng(seed)
+    x = np.cumsum(rng.normal(0, 0.05, n))
+    y = np.cumsum(rng.normal(0, 0.05, n))
+    if nan_every:
+        x[rng.random(n) < 1.0 / nan_every] = np.nan
+    return pd.DataFrame({"x": x, "y": y, "v": rng.uniform(0, 1, n),
+                         "x2": x[::-1].copy(), "y2": y + 0.3})
+
+
+@pytest.mark.parametrize("chunk_rows", [1, 2, 3, 997, 40_000])
+def test_axis0_line_stream_equals_resident(chunk_rows):
+    """LineAxis0 over one partition: a chunk boundary is invisible (the joining segment is
+    drawn once, with the segment_start the unchunked line gives it), NaN gaps included."""
+    df = _walk(3000 if chunk_rows < 4 else 30_000, 8, nan_every=50)
+    cvs = dsa.Canvas(157, 113)
+    for agg in (dsa.count(), dsa.max("v"), dsa.sum("v")):
+        resident = cvs.line(df, "x", "y", agg=agg)
+        streamed = cvs.line(dsa.ChunkedFrame(df, chunk_rows=chunk_rows), "x", "y", agg=agg)
+        assert streamed.attrs == resident.attrs
+        _close(streamed.values, resident.values, 1e-12)
+
+
+def test_axis0_line_partitions_follow_dask_stitching():
+    """LineAxis0 over partitions: partition i > 0 is drawn as the previous partition's
+    last row followed by its own rows with plot_start False (dask.py:242-248); the
+    expected canvas is the oracle's sum of those drawings."""
+    df = _walk(20_000, 9, nan_every=40)
+    cuts = [0, 5_000, 5_001, 12_345, 20_000]
+    parts = [df.iloc[a:b] for a, b in zip(cuts[:-1], cuts[1:])]
+    W, H = 140, 90
+    cvs = dsa.Canvas(W, H)
+    got = cvs.line(dsa.ChunkedFrame(parts, chunk_rows=1_000), "x", "y", agg=dsa.count())
+    xr, yr = got.attrs["x_range"], got.attrs["y_range"]
+    sx, sy = W / (xr[1] - xr[0]), H / (yr[1] - yr[0])
+    view = oracle.View(sx, -xr[0] * sx, sy, -yr[0] * sy, xr[0], xr[1], yr[0], yr[1])
+    want = np.zeros((H, W), np.int64)
+    for i, p in enumerate(parts):
+        rows = p if i == 0 else pd.concat([parts[i - 1].iloc[-1:], p])
+        want += oracle.line_axis0(rows["x"].to_numpy(), rows["y"].to_numpy(), view, W, H,
+                                  plot_start=(i == 0))["count"]
+    np.testing.assert_array_equal(got.values, want)
+
+
+def test_axis0_multi_stream():
+    """LinesAxis0Multi: chunks of one partition equal the resident lines; partitions are
+    drawn one by one (dask's default path)."""
+    df = _walk(12_000, 10, nan_every=30)
+    cvs = dsa.Canvas(120, 80, x_range=(-3, 3), y_range=(-3, 3))
+    kw = dict(x=["x", "x2"], y=["y", "y2"], axis=0)
+    resident = cvs.line(df, agg=dsa.count(), **kw)
+    streamed = cvs.line(dsa.ChunkedFrame(df, chunk_rows=777), agg=dsa.count(), **kw)
+    np.testing.assert_array_equal(streamed.values, resident.values)
+    parts = [df.iloc[:4_000], df.iloc[4_000:]]
+    got = cvs.line(dsa.ChunkedFrame(parts, chunk_rows=999), agg=dsa.count(), **kw)
+    want = sum(cvs.line(p, agg=dsa.count(), **kw).values.astype(np.int64) for p in parts)
+    np.testing.assert_array_equal(got.values, want)

@@ -234,6 +234,114 @@ def test_line_f32_delta_flag():
     np.testing.assert_array_equal(a, b)
 
 
+def test_line_f32_delta_known_answer():
+    """Known answer, derived by hand from line.py's arithmetic for the float32 polyline
+    (0.30, -0.5) -> (0.3397695, -0.027073061) -> (0.33822325, 0.043172628) on a 34 x 7
+    canvas, x_range (0, 1), y_range (-1, 0.043172628f):
+    sx = 34, sy = 7 / 1.043172628 = 6.7103, so the vertices snap to (10, 3), (11, 6) and
+    (11, 7 -> 6).  Segment 1 is inside the view: Bresenham from (10, 3) to (11, 6) (y major,
+    error -2, +2 -> step x, ...) gives (10, 3), (10, 4), (11, 5), (11, 6).  Segment 2 ends
+    on ymax; its float32 dy = fl32(0.043172628 + 0.027073061) = 0.0702456906 exceeds
+    q = ymax - y0 = 0.0702456888 (float64), so _clipt sets t1 = q / p < 1: the segment is
+    clipped, zero-length in pixels, and draws nothing.  With float64 deltas it is not
+    clipped and appends (11, 6) once more."""
+    f = np.float32
+    assert float(f(0.043172628) - f(-0.027073061)) > float(f(0.043172628)) - float(f(-0.027073061))
+    xs = np.array([0.30, 0.3397695, 0.33822325], f).astype(float)
+    ys = np.array([-0.5, -0.027073061, 0.043172628], f).astype(float)
+    yr = (-1.0, float(f(0.043172628)))
+    want = np.zeros((7, 34), np.uint32)
+    for r, c in ((3, 10), (4, 10), (5, 11), (6, 11)):
+        want[r, c] = 1
+    v32 = G.view_for(34, 7, (0.0, 1.0), yr, xf32=True, yf32=True)
+    np.testing.assert_array_equal(oracle.line_axis0(xs, ys, v32, 34, 7)["count"], want)
+    v64 = G.view_for(34, 7, (0.0, 1.0), yr)
+    want[6, 11] = 2
+    np.testing.assert_array_equal(oracle.line_axis0(xs, ys, v64, 34, 7)["count"], want)
+
+
+@pytest.mark.parametrize("case", G.f32_bound_cases(), ids=lambda c: c["name"])
+def test_line_f32_delta_on_each_bound(case):
+    """A float32 vertex exactly on each of the four bounds with a rounded-up float32
+    delta: float32 mode clips the second segment (4 pixels), float64 mode draws its
+    unclipped zero-length pixel once more (5 pixels, one cell differing by one)."""
+    xs, ys = case["xs"].astype(float), case["ys"].astype(float)
+    W, H = case["W"], case["H"]
+    v32 = G.view_for(W, H, case["x_range"], case["y_range"], xf32=True, yf32=True)
+    v64 = G.view_for(W, H, case["x_range"], case["y_range"])
+    a = oracle.line_axis0(xs, ys, v32, W, H)["count"].astype(np.int64)
+    b = oracle.line_axis0(xs, ys, v64, W, H)["count"].astype(np.int64)
+    assert a.sum() == case["total_f32"] and b.sum() == case["total_f64"]
+    d = b - a
+    assert d.min() == 0 and d.sum() == 1 and a[d == 1].item() == 1
+
+
+def test_line_f32_delta_matches_f64_when_deltas_agree():
+    """When both ends are inside the view and each float32 delta equals the float64
+    difference of its ends, every _clipt call leaves (t0, t1) = (0, 1) (q >= p by the
+    monotonicity of rounding), so float32 and float64 modes draw the same pixels."""
+    rng = np.random.default_rng(23)
+    W, H = 29, 11
+    f = np.float32
+    xr = (float(f(-0.61)), float(f(0.47)))
+    yr = (float(f(-0.043)), float(f(0.39)))
+    v32 = G.view_for(W, H, xr, yr, xf32=True, yf32=True)
+    v64 = G.view_for(W, H, xr, yr)
+    same = 0
+    for _ in range(3000):
+        xs = rng.uniform(xr[0], xr[1], 2).astype(f)
+        ys = rng.uniform(yr[0], yr[1], 2).astype(f)
+        k = rng.integers(0, 4)  # one end on a bound
+        (xs if k < 2 else ys)[1] = f((xr if k < 2 else yr)[k % 2])
+        exact = (float(xs[1] - xs[0]) == float(xs[1]) - float(xs[0]) and
+                 float(ys[1] - ys[0]) == float(ys[1]) - float(ys[0]))
+        if not exact:
+            continue
+        a = oracle.line_axis0(xs.astype(float), ys.astype(float), v32, W, H)["count"]
+        b = oracle.line_axis0(xs.astype(float), ys.astype(float), v64, W, H)["count"]
+        np.testing.assert_array_equal(a, b)
+        same += 1
+    assert same > 500
+
+
+@pytest.mark.parametrize("side", ["xmin", "xmax", "ymin", "ymax"])
+def test_line_f32_zero_length_clip_rule(side):
+    """A non-start segment inside one pixel that ends exactly on a bound, from a vertex of
+    the opposite sign: with float64 deltas it is unclipped and zero-length and appends
+    its pixel once (line.py:1003-1005); with float32 deltas it is clipped exactly when
+    p = fl32(delta toward the bound) exceeds q = the float64 distance to the bound
+    (_clipt's q < p branch, line.py:792-798), and then draws nothing.  The rule is
+    evaluated with numpy's float32 / float64 arithmetic on random vertices."""
+    rng = np.random.default_rng({"xmin": 1, "xmax": 2, "ymin": 3, "ymax": 4}[side])
+    f = np.float32
+    W = H = 7
+    b = float(f(0.043172628))
+    lo_hi = (-b, 0.5) if side.endswith("min") else (-0.5, b)
+    rest = (0.0, 1.0)
+    xr, yr = (lo_hi, rest) if side[0] == "x" else (rest, lo_hi)
+    v32 = G.view_for(W, H, xr, yr, xf32=True, yf32=True)
+    v64 = G.view_for(W, H, xr, yr)
+    clipped = kept = 0
+    for _ in range(2000):
+        bound = f(lo_hi[0] if side.endswith("min") else lo_hi[1])
+        # the other vertex: opposite sign, within the bound's pixel (pixel size 0.0776)
+        other = f(-bound * rng.uniform(0.05, 0.75))
+        o2 = f(rng.uniform(0.31, 0.32))   # the other coordinate, same pixel for both ends
+        p = abs(float(f(bound - other)))
+        q = abs(float(bound) - float(other))
+        if side[0] == "x":
+            x0, x1, y0, y1 = float(other), float(bound), float(o2), float(o2)
+        else:
+            x0, x1, y0, y1 = float(o2), float(o2), float(other), float(bound)
+        a = oracle.draw_segment(v32, W, H, False, x0, y0, x1, y1)["count"].sum()
+        c = oracle.draw_segment(v64, W, H, False, x0, y0, x1, y1)["count"].sum()
+        assert c == 1
+        assert a == (0 if p > q else 1), (x0, y0, x1, y1)
+        clipped += p > q
+        kept += p <= q
+    assert clipped > 100 and kept > 100
+
+
 @pytest.mark.parametrize("mode", ["first", "last", "max", "min"])
 def test_row_index_loop_matches_closed_form(mode):
     """The sequential row-index restatement against its vectorised statement, on random
