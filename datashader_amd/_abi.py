@@ -31,7 +31,7 @@ FILL_ZERO32, FILL_ONES32, FILL_ZERO64, FILL_ONES64 = 1, 2, 3, 4
 FIN_ANY, FIN_SUM, FIN_MEAN, FIN_VAR, FIN_STD, FIN_MIN, FIN_MAX, FIN_SUMZERO = range(1, 9)
 
 EXPORTS = ("dsx_points_workspace_size", "dsx_points_agg", "dsx_lines_agg",
-           "dsx_bounds_workspace_size", "dsx_bounds", "dsx_fill", "dsx_finalize",
+           "dsx_bounds_workspace_size", "dsx_bounds", "dsx_bounds_noalloc", "dsx_fill", "dsx_finalize",
            "dsx_grid_index_workspace_size", "dsx_grid_index_build", "dsx_gather_rows",
            "dsx_points_indexed_workspace_size", "dsx_points_agg_indexed",
            "dsx_shade_workspace_size", "dsx_shade", "dsx_combine", "dsx_points_rowindex",
@@ -165,6 +165,7 @@ def load():
         lib.dsx_bounds_workspace_size.argtypes = [i64]
         lib.dsx_bounds_workspace_size.restype = sz
         lib.dsx_bounds.argtypes = [vp, i32, i64, vp, vp, sz, vp]
+        lib.dsx_bounds_noalloc.argtypes = [vp, i32, i64, vp, vp]
         lib.dsx_fill.argtypes = [vp, i32, i64, vp]
         lib.dsx_finalize.argtypes = [P(FinalizeDesc), vp]
         lib.dsx_grid_index_workspace_size.argtypes = [P(GridIndexDesc)]
@@ -190,7 +191,7 @@ def load():
         lib.dsx_strerror.restype = ctypes.c_char_p
         lib.dsx_last_hip_error.argtypes = []
         lib.dsx_last_hip_error.restype = ctypes.c_char_p
-        for f in ("dsx_points_agg", "dsx_lines_agg", "dsx_bounds", "dsx_fill", "dsx_finalize",
+        for f in ("dsx_points_agg", "dsx_lines_agg", "dsx_bounds", "dsx_bounds_noalloc", "dsx_fill", "dsx_finalize",
                   "dsx_grid_index_build", "dsx_gather_rows", "dsx_points_agg_indexed",
                   "dsx_shade", "dsx_combine", "dsx_points_rowindex", "dsx_take",
                   "dsx_points_nselect", "dsx_abi_version"):

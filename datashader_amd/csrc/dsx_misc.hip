@@ -2,6 +2,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "dsx_common.h"
 
 namespace dsx {
@@ -202,6 +204,68 @@ extern "C" int dsx_bounds(const void *col, int32_t dtype, int64_t n, double *out
     else
         bounds_partial<double><<<grid, kBoundsBlock, 0, s>>>((const double *)col, n, part);
     int st = launch_status();
+    if (st) return st;
+    bounds_final<<<1, kBoundsBlock, 0, s>>>(part, grid, out2);
+    return launch_status();
+}
+
+// ---- bounds without allocation ------------------------------------------------------------
+// The per-workgroup partials live in a file-scope device array instead of a caller
+// workspace.  Each GPU holds its own copy of the array (at its own address) for as long as
+// the process runs; host code reaches it only through the address hipGetSymbolAddress
+// returns on that device, fetched once per device on the first call there.  Every call on
+// a GPU shares that GPU's array: calls that overlap on one GPU (other streams, other
+// threads) race.  bounds_partial writes the partials of every workgroup it launches before
+// bounds_final reads them, so nothing needs clearing between calls.
+namespace dsx {
+constexpr int kScratchBlocks = 4096;        // partials held: 4096 x {min, max} = 64 KiB
+constexpr int kScratchDevices = 64;
+}  // namespace dsx
+
+static __device__ __align__(256) double scratch[2 * dsx::kScratchBlocks];
+
+namespace dsx {
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kScratchDevices];  // per device, set once under the mutex
+
+static int scratch_address(double **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
+    if (dev < 0 || dev >= kScratchDevices) return DSX_ERR_SHAPE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            record_hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol);
+            return DSX_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = reinterpret_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return DSX_OK;
+}
+}  // namespace dsx
+
+extern "C" int dsx_bounds_noalloc(const void *col, int32_t dtype, int64_t n, double *out2,
+                                  void *stream) {
+    if (!out2 || (n > 0 && !col)) return DSX_ERR_NULL;
+    if (n < 0) return DSX_ERR_SHAPE;
+    if (dtype != DSX_F32 && dtype != DSX_F64) return DSX_ERR_DTYPE;
+    // dsx_bounds's launch shape, capped at the partials the array holds (the loops stride
+    // over the grid, so any grid covers every row): no input needs more than the array
+    int grid = grid_for(n < 1 ? 1 : n, kBoundsBlock, 8);
+    grid = grid < kScratchBlocks ? grid : kScratchBlocks;
+    double *part = nullptr;
+    int st = scratch_address(&part);
+    if (st) return st;
+    hipStream_t s = as_stream(stream);
+    if (dtype == DSX_F32)
+        bounds_partial<float><<<grid, kBoundsBlock, 0, s>>>((const float *)col, n, part);
+    else
+        bounds_partial<double><<<grid, kBoundsBlock, 0, s>>>((const double *)col, n, part);
+    st = launch_status();
     if (st) return st;
     bounds_final<<<1, kBoundsBlock, 0, s>>>(part, grid, out2);
     return launch_status();

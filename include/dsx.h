@@ -6,8 +6,9 @@
  * memory.  Every call is asynchronous on the given HIP stream (a hipStream_t passed as
  * void*; NULL = the default stream), returns DSX_OK (0) or a negative dsx_status, and
  * never throws.  Descriptors are validated on the host before anything is launched.
- * The library keeps no mutable global state: calls on disjoint buffers are reentrant;
- * calls sharing planes or a workspace must be ordered (one stream).
+ * The library keeps no mutable global state other than the per-device address of
+ * dsx_bounds_noalloc's array: calls on disjoint buffers are reentrant; calls sharing
+ * planes or a workspace must be ordered (one stream).
  *
  * Correspondence with the reference (datashader 0.19.1) -- this ABI replaces the numba
  * kernels its backends call:
@@ -183,6 +184,16 @@ int dsx_lines_agg(const dsx_lines_desc *d, const dsx_view *v, const dsx_planes *
 size_t dsx_bounds_workspace_size(int64_t n);
 int dsx_bounds(const void *col, int32_t dtype, int64_t n, double *out2, void *workspace,
                size_t workspace_bytes, void *stream);
+
+/* dsx_bounds without a workspace, for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The per-workgroup partials live in a 64 KiB array
+ * inside the library (4096 partials; each GPU the library runs on holds its own copy
+ * until the process ends); the call allocates nothing and n is unlimited, since the
+ * launch never uses more workgroups than the array has partials.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (several streams or threads)
+ * race and must be ordered by the caller.  Same results and status codes as dsx_bounds. */
+int dsx_bounds_noalloc(const void *col, int32_t dtype, int64_t n, double *out2, void *stream);
 
 /* ---- fill / finalize -------------------------------------------------------------- */
 typedef enum {
