@@ -55,6 +55,46 @@ def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_threads():
+    """Host threads for the multi-threaded CPU baseline: the process's CPU share
+    (OMP_NUM_THREADS when the host sets it), else the CPUs it may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_forms(run, rows_1t, rows_pt, what, reps=5):
+    """Time the CPU baseline (oracle/cpu_baseline.c) in its two forms: one thread (the
+    reference's pandas path) on the first rows_1t rows, and P threads (its dask path,
+    npartitions = P, private canvases, binary combine tree) on the first rows_pt rows.
+    run(rows, threads) performs one whole call.  One untimed warm-up call each, then the
+    median of `reps`.  Returns (rows/s at P threads, P, rows/s at 1 thread, sample text,
+    host info)."""
+    from oracle import baseline
+
+    def med(rows, threads):
+        run(rows, threads)
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            run(rows, threads)
+            times.append(time.perf_counter() - t0)
+        return statistics.median(times)
+
+    P = cpu_threads()
+    t1 = med(rows_1t, 1)
+    tp = med(rows_pt, P)
+    sample = (f"{what}: the CPU baseline (specialised C, -O3) on the first {rows_pt} rows "
+              f"with {P} threads (dask form: {P} partitions, private canvases, binary "
+              f"combine tree) and on the first {rows_1t} rows with 1 thread (pandas form); "
+              f"median of {reps} after one warm-up")
+    return rows_pt / tp, P, rows_1t / t1, sample, baseline.host_info()
+
+
 def build_workload(name, torch, scale=1.0):
     """Returns (callable step, rows per step, algorithmic bytes per aggregation launch,
     config dict, cpu_baseline callable or None)."""
@@ -82,21 +122,31 @@ def build_workload(name, torch, scale=1.0):
                            "0.01-sigma Gaussian, 10% uniform), value lognormal(0,2)",
                "rows": n, "canvas": [W, H], "aggs": ["count()", "mean('v')"]}
 
-        def cpu(sample_rows):
+        def cpu(sample_rows, res=None):
             import oracle
-            m = min(sample_rows, n)
-            xs = x[:m].double().cpu().numpy()
-            ys = y[:m].double().cpu().numpy()
-            vs = v[:m].double().cpu().numpy()
+            from oracle import baseline
+            m1, mp = min(sample_rows, n), n
+            xs, ys, vs = (t_[:mp].cpu().numpy() for t_ in (x, y, v))
             sx, tx = canvas.x_axis.compute_scale_and_translate(xr, W)
             sy, ty = canvas.y_axis.compute_scale_and_translate(yr, H)
             view = oracle.View(sx, tx, sy, ty, xr[0], xr[1], yr[0], yr[1])
-            times = []
-            for _ in range(3):
-                t0 = time.perf_counter()
-                oracle.points(xs, ys, view, W, H, ("count", "countv", "sumz"), val=vs)
-                times.append(time.perf_counter() - t0)
-            return m, statistics.median(times)
+            last = {}
+
+            def run(m, th):
+                last[m] = baseline.points_count_mean_f32(xs[:m], ys[:m], vs[:m], view, W, H, th)
+
+            out = cpu_forms(run, m1, mp, "count + mean")
+            if res is not None:  # the GPU call's planes against the baseline's, all rows
+                ref = last[mp]
+                cnt = res["count"].values
+                gm, rm = res["mean"].values, ref["mean"]
+                ok = ~np.isnan(rm)
+                rel = float(np.max(np.abs(gm[ok] - rm[ok]) / np.abs(rm[ok]))) if ok.any() else 0.0
+                out[4]["agreement"] = {
+                    "count_mismatched_cells": int(np.count_nonzero(cnt != ref["count"])),
+                    "mean_nan_pattern_equal": bool(np.array_equal(np.isnan(gm), np.isnan(rm))),
+                    "mean_max_rel_err": rel}
+            return out
 
         def check(res):
             c = res["count"].data
@@ -130,19 +180,16 @@ def build_workload(name, torch, scale=1.0):
                            "lognormal(0,2)", "rows": n, "chunk_rows": chunk,
                "canvas": [W, H], "aggs": ["count()", "mean('v')"]}
 
-        def cpu(sample_rows):
+        def cpu(sample_rows, res=None):
             import oracle
-            m = min(sample_rows, n)
-            xs, ys, vs = (host[k][:m].astype(np.float64) for k in ("x", "y", "v"))
+            from oracle import baseline
+            m1, mp = min(sample_rows, n), min(n, 100_000_000)
+            xs, ys, vs = (host[k] for k in ("x", "y", "v"))
             sx, tx = canvas.x_axis.compute_scale_and_translate(xr, W)
             sy, ty = canvas.y_axis.compute_scale_and_translate(yr, H)
             view = oracle.View(sx, tx, sy, ty, xr[0], xr[1], yr[0], yr[1])
-            times = []
-            for _ in range(3):
-                t0 = time.perf_counter()
-                oracle.points(xs, ys, view, W, H, ("count", "countv", "sumz"), val=vs)
-                times.append(time.perf_counter() - t0)
-            return m, statistics.median(times)
+            return cpu_forms(lambda m, th: baseline.points_count_mean_f32(
+                xs[:m], ys[:m], vs[:m], view, W, H, th), m1, mp, "count + mean")
 
         def check(res):
             return int(res["count"].values.sum(dtype=np.int64))
@@ -164,17 +211,14 @@ def build_workload(name, torch, scale=1.0):
         cfg = {"workload": "points 800x600 count, 10M float64 rows (5-Gaussian mixture, "
                            "0.1% NaN rows)", "rows": n, "canvas": [W, H], "aggs": ["count()"]}
 
-        def cpu(sample_rows):
+        def cpu(sample_rows, res=None):
             import oracle
-            m = min(sample_rows, n)
-            xs, ys = x[:m].cpu().numpy(), y[:m].cpu().numpy()
+            from oracle import baseline
+            m1, mp = min(sample_rows, n), n
+            xs, ys = x.cpu().numpy(), y.cpu().numpy()
             view = oracle.View(40.0, 400.0, 30.0, 300.0, -10, 10, -10, 10)
-            times = []
-            for _ in range(3):
-                t0 = time.perf_counter()
-                oracle.points(xs, ys, view, W, H, ("count",))
-                times.append(time.perf_counter() - t0)
-            return m, statistics.median(times)
+            return cpu_forms(lambda m, th: baseline.points_count_f64(
+                xs[:m], ys[:m], view, W, H, th), m1, mp, "count")
 
         def check(res):
             return int(res.data.sum().item())
@@ -200,20 +244,17 @@ def build_workload(name, torch, scale=1.0):
                            "(3 sinusoidal ridges), 16 Zipf(1.2) categories",
                "rows": n, "canvas": [W, H], "aggs": ["by('cat', count())"]}
 
-        def cpu(sample_rows):
+        def cpu(sample_rows, res=None):
             import oracle
-            m = min(sample_rows, n)
-            xs, ys = x[:m].double().cpu().numpy(), y[:m].double().cpu().numpy()
-            cs = cat[:m].long().cpu().numpy()
+            from oracle import baseline
+            m1, mp = min(sample_rows, n), min(n, 100_000_000)
+            xs, ys = x[:mp].cpu().numpy(), y[:mp].cpu().numpy()
+            cs = cat[:mp].cpu().numpy().astype(np.uint8)
             sx, tx = canvas.x_axis.compute_scale_and_translate(xr, W)
             sy, ty = canvas.y_axis.compute_scale_and_translate(yr, H)
             view = oracle.View(sx, tx, sy, ty, xr[0], xr[1], yr[0], yr[1])
-            times = []
-            for _ in range(3):
-                t0 = time.perf_counter()
-                oracle.points(xs, ys, view, W, H, ("count",), cat=cs, ncat=16)
-                times.append(time.perf_counter() - t0)
-            return m, statistics.median(times)
+            return cpu_forms(lambda m, th: baseline.points_count_cat_f32(
+                xs[:m], ys[:m], cs[:m], view, W, H, 16, th), m1, mp, "by(cat, count())")
 
         def check(res):
             return int(res.data.sum().item())
@@ -241,19 +282,21 @@ def build_workload(name, torch, scale=1.0):
                            "shared x = linspace(0,1,256)", "lines": nlines, "canvas": [W, H],
                "aggs": ["count()"]}
 
-        def cpu(sample_rows):
+        def cpu(sample_rows, res=None):
             import oracle
-            m = min(sample_rows, nlines)
-            ys = block[:, :m].t().double().cpu().numpy()
+            from oracle import baseline
+            # series, row-major float32 as the reference's to_numpy() gives them
+            m1 = max(1, min(sample_rows // 256, nlines))
+            mp = max(1, min(nlines, 8 * m1))
+            ys = block[:, :mp].t().contiguous().cpu().numpy()
             sx, tx = canvas.x_axis.compute_scale_and_translate(xr, W)
             sy, ty = canvas.y_axis.compute_scale_and_translate(yr, H)
             view = oracle.View(sx, tx, sy, ty, xr[0], xr[1], yr[0], yr[1], yf32=True)
-            times = []
-            for _ in range(3):
-                t0 = time.perf_counter()
-                oracle.lines_xconst(xs, ys, view, W, H, ("count",))
-                times.append(time.perf_counter() - t0)
-            return m * 256, statistics.median(times)
+            xs64 = np.asarray(xs, np.float64)
+            rate_p, P, rate_1, sample, info = cpu_forms(
+                lambda m, th: baseline.lines_xconst_f32(xs64, ys[:m], view, W, H, th),
+                m1, mp, "lines count (series of 256 samples)", reps=3)
+            return rate_p * 256, P, rate_1 * 256, sample.replace("rows", "series"), info
 
         def check(res):
             return int(res.data.sum().item())
@@ -276,12 +319,17 @@ def _traffic(workload):
 
 
 def run_serving(args, torch):
-    """Batched / served viewports (count + mean) over a resident point set against the
-    plain per-request form; prints one JSON line."""
+    """Viewport serving (count + mean) over a 1B-row resident set, three forms on the same
+    request sample: plain (one Canvas.points call over the whole set per request, one
+    at a time), indexed (the same call over the lattice-sorted set, per request, on four
+    streams) and batched (datashader_amd.batching: continuous batching, each batch one
+    graph replay of the work-queue kernels, knobs from the autotuner).  Each form runs
+    saturated (all requests queued at the start) and under the Poisson arrivals; prints
+    one JSON line."""
     import numpy as np
 
     import datashader_amd as dsa
-    from datashader_amd import serving, synthetic
+    from datashader_amd import batching, serving, synthetic
 
     n = int(1_000_000_000 * args.scale)
     _log(f"[bench] serving: generating {n} resident rows")
@@ -290,56 +338,103 @@ def run_serving(args, torch):
     frame = dsa.DeviceFrame({"x": x, "y": y, "v": v})
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    index = serving.PointIndex(x, y, v, grid=1024)
+    index = serving.PointIndex(x, y, v, grid=args.grid)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
-    _log(f"[bench] index built in {build_s:.2f} s")
+    _log(f"[bench] index (grid {args.grid}) built in {build_s:.2f} s")
     g = torch.Generator(device="cuda")
     g.manual_seed(6)
     pick = torch.randint(0, n, (100_000,), generator=g, device="cuda")
     centres = (x[pick].cpu().numpy(), y[pick].cpu().numpy())
     extent = ((index.x0, index.x1), (index.y0, index.y1))
     reqs = serving.make_requests(args.requests, centres, extent, seed=5, rate=200.0)
-    # warm both forms (library load, first-call selection kernels)
-    for r in reqs[:3]:
-        serving.run_plain(frame, r)
-        serving.run_indexed(index, r)
-    torch.cuda.synchronize()
-    # plain: one Canvas.points call over the whole set per request, back to back
-    k = min(args.plain_requests, len(reqs))
-    plat = []
-    for r in reqs[:k]:
-        t1 = time.perf_counter()
-        serving.run_plain(frame, r)
-        torch.cuda.synchronize()
-        plat.append(time.perf_counter() - t1)
-    plat = np.array(plat)
-    plain_rps = k / plat.sum()
-    _log(f"[bench] plain: {plain_rps:.1f} req/s over {k} requests")
-    server = serving.ViewportServer(index, slots=4)
-    open_ = server.serve(reqs, realtime=True)
-    _log(f"[bench] served (Poisson 200 req/s): {len(reqs) / open_['wall']:.1f} req/s")
-    sat = server.serve(reqs, realtime=False)
-    sat_rps = len(reqs) / sat["wall"]
-    _log(f"[bench] served (saturated): {sat_rps:.1f} req/s")
+    est = batching.RowEstimator(index)
+    scanned = sum(12 * est.rows(r.x_range, r.y_range) + 12 * r.width * r.height for r in reqs)
 
     def pct(a):
-        return {"p50_ms": float(np.percentile(a, 50) * 1e3), "p99_ms": float(np.percentile(a, 99) * 1e3)}
+        return {"p50_ms": float(np.percentile(a, 50) * 1e3),
+                "p99_ms": float(np.percentile(a, 99) * 1e3)}
+
+    # plain: one call per request over the unsorted set, FIFO on one stream
+    for r in reqs[:3]:
+        serving.run_plain(frame, r)
+    torch.cuda.synchronize()
+
+    def plain(realtime):
+        lat = np.zeros(len(reqs))
+        s0 = time.perf_counter()
+        for i, r in enumerate(reqs):
+            if realtime:
+                wait = r.arrival - (time.perf_counter() - s0)
+                if wait > 0:
+                    time.sleep(wait)
+            serving.run_plain(frame, r)
+            torch.cuda.synchronize()
+            lat[i] = time.perf_counter() - s0 - (r.arrival if realtime else 0.0)
+        return lat, time.perf_counter() - s0
+
+    plat_sat, pwall = plain(False)
+    plain_rps = len(reqs) / pwall
+    _log(f"[bench] plain saturated: {plain_rps:.1f} req/s")
+    plat_rt, _ = plain(True)
+    _log(f"[bench] plain Poisson 200/s: p99 {np.percentile(plat_rt, 99) * 1e3:.1f} ms")
+
+    # indexed, per request (no batching) on four streams: dsx_points_agg_indexed reads
+    # lattices of up to 1024 cells a side
+    index1k = index if index.grid <= 1024 else serving.PointIndex(x, y, v, grid=1024)
+    vs = serving.ViewportServer(index1k, slots=4)
+    for r in reqs[:3]:
+        serving.run_indexed(index1k, r)
+    torch.cuda.synchronize()
+    isat = vs.serve(reqs, realtime=False)
+    irt = vs.serve(reqs, realtime=True)
+    idx_rps = len(reqs) / isat["wall"]
+    _log(f"[bench] indexed per request: {idx_rps:.1f} req/s saturated")
+    if index1k is not index:
+        del vs, index1k
+        torch.cuda.empty_cache()
+
+    # batched: tune on a separate sample, then serve
+    tsample = serving.make_requests(args.tune_requests, centres, extent, seed=55, rate=200.0)
+    t1 = time.perf_counter()
+    tuning = batching.BatchTuner(index).tune(tsample, max_batch=args.max_batch)
+    tune_s = time.perf_counter() - t1
+    _log(f"[bench] tuned in {tune_s:.1f} s: threads {tuning.threads}, workgroups "
+         f"{tuning.workgroups}, {len(tuning.per_class)} classes")
+    server = batching.BatchServer(index, engines=2, max_batch=args.max_batch, tuning=tuning)
+    server.serve(reqs[:64], realtime=False)  # warm (allocator pools, graph upload)
+    bsat = server.serve(reqs, realtime=False)
+    brt = server.serve(reqs, realtime=True)
+    b_rps = len(reqs) / bsat["wall"]
+    _log(f"[bench] batched: {b_rps:.1f} req/s saturated ({bsat['batches']} batches, mean "
+         f"{bsat['mean_batch']:.1f}); Poisson p99 {np.percentile(brt['latency'], 99) * 1e3:.1f} ms")
+    server.close()
 
     out = {
-        "metric": "viewport req/s (count + mean), served vs plain per-request form",
-        "value": sat_rps, "unit": "req/s", "n_gpus": 1, "steps": len(reqs), "warmup": 3,
-        "ms_per_step": 1e3 / sat_rps, "higher_is_better": True, "scaling": "weak",
+        "metric": "viewport req/s (count + mean), batched served form vs plain per-request form",
+        "value": b_rps, "unit": "req/s", "n_gpus": 1, "steps": len(reqs), "warmup": 3,
+        "ms_per_step": 1e3 / b_rps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded, generated on the device)",
         "config": {"workload": "serving: 1B resident float32 rows (5-Gaussian mixture), value "
                                "U(0,1); Poisson 200 req/s, zoom 2^U(0,12), centres at random "
                                "rows, canvases 256^2/512^2/1024x768/1920x1080",
-                   "rows": n, "requests": len(reqs), "grid": index.grid, "slots": 4,
-                   "index_build_s": build_s},
-        "plain": {"req_per_s": plain_rps, "requests": k, **pct(plat)},
-        "served_poisson_200": {"req_per_s": len(reqs) / open_["wall"], **pct(open_["latency"])},
-        "served_saturated": {"req_per_s": sat_rps, **pct(sat["latency"])},
-        "speedup_vs_plain": sat_rps / plain_rps,
+                   "rows": n, "requests": len(reqs), "grid": index.grid,
+                   "index_build_s": build_s, "max_batch": args.max_batch, "engines": 2,
+                   "tuning": tuning.to_dict(), "tune_s": tune_s,
+                   "tune_requests": len(tsample)},
+        "plain": {"saturated_req_per_s": plain_rps, **pct(plat_sat),
+                  "poisson_200": pct(plat_rt)},
+        "indexed_per_request": {"saturated_req_per_s": idx_rps, **pct(isat["latency"]),
+                                "poisson_200": pct(irt["latency"])},
+        "batched": {"saturated_req_per_s": b_rps, **pct(bsat["latency"]),
+                    "batches": bsat["batches"], "mean_batch": bsat["mean_batch"],
+                    "poisson_200": {**pct(brt["latency"]), "batches": brt["batches"],
+                                    "mean_batch": brt["mean_batch"]},
+                    "algorithmic_bytes": scanned,
+                    "achieved_gbs": scanned / bsat["wall"] / 1e9},
+        "speedup_vs_plain": b_rps / plain_rps,
+        "p99_poisson_batched_vs_plain": float(np.percentile(brt["latency"], 99) /
+                                              np.percentile(plat_rt, 99)),
     }
     print(json.dumps(out), flush=True)
 
@@ -514,8 +609,12 @@ def main():
                     help="also run one plain 1 GiB device copy (the reference dispatch that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     ap.add_argument("--requests", type=int, default=2000, help="serving: viewport requests")
-    ap.add_argument("--plain-requests", type=int, default=200,
-                    help="serving: requests timed in the plain per-request form")
+    ap.add_argument("--grid", type=int, default=4096,
+                    help="serving: cells per side of the index's lattice")
+    ap.add_argument("--tune-requests", type=int, default=96,
+                    help="serving: sample requests the autotuner times")
+    ap.add_argument("--max-batch", type=int, default=32,
+                    help="serving: requests per batch")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -573,11 +672,13 @@ def main():
 
     cpu_baseline = None
     if cpu is not None and not args.no_cpu:
-        _log(f"[bench] CPU oracle baseline on {args.cpu_sample} rows")
-        m, sec = cpu(args.cpu_sample)
-        cpu_baseline = {"value": m / sec / 1e9, "unit": unit_rows, "cores": 1, "kind": "port",
-                        "sample": f"first {m} rows of the same workload, single-thread C "
-                                  f"oracle, median of 3"}
+        _log("[bench] CPU baseline (1 thread and the host's CPU share)")
+        rate_p, P, rate_1, sample, info = cpu(args.cpu_sample, res)
+        cpu_baseline = {"value": rate_p / 1e9, "unit": unit_rows, "cores": P, "kind": "port",
+                        "sample": sample, "threads": P, "single_thread": rate_1 / 1e9,
+                        **info}
+        _log(f"[bench] CPU baseline: {rate_1 / 1e9:.3f} {unit_rows} on 1 thread, "
+             f"{rate_p / 1e9:.3f} on {P} threads ({info['cpu_model']}, nproc {info['nproc']})")
 
     out = {
         "metric": "Gpoints/s aggregated + % HBM roofline" if args.workload != "lines"

@@ -89,7 +89,8 @@ _lib = None
 
 
 def build():
-    """Compile ``libdsx_oracle.so`` with the committed Makefile."""
+    """Compile ``libdsx_oracle.so`` (and the timed CPU baseline ``libdsx_cpubase.so``)
+    with the committed Makefile."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 

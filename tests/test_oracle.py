@@ -355,3 +355,74 @@ def test_row_index_loop_matches_closed_form(mode):
     for v in (val, None) if mode in ("first", "last") else (val,):
         np.testing.assert_array_equal(ri.select_rows(cell, ncells, mode, v),
                                       ri.selected_rows_closed_form(cell, ncells, mode, v))
+
+
+# ---- the timed CPU baseline (oracle/cpu_baseline.c) equals the sequential oracle -------
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_cpu_baseline_count_mean_equals_oracle(threads):
+    from oracle import baseline
+    rng = np.random.default_rng(31)
+    n, W, H = 200_003, 97, 61
+    x = rng.normal(0.5, 0.3, n).astype(np.float32)
+    y = rng.normal(0.5, 0.3, n).astype(np.float32)
+    v = rng.lognormal(0, 2, n).astype(np.float32)
+    x[::101] = np.nan
+    v[::7] = np.nan
+    x[5], y[5] = 1.0, 1.0  # on the upper bounds: the W-1 / H-1 clamp
+    sx, sy = W / 1.0, H / 1.0
+    view = oracle.View(sx, 0.0, sy, 0.0, 0.0, 1.0, 0.0, 1.0)
+    got = baseline.points_count_mean_f32(x, y, v, view, W, H, threads)
+    ref = oracle.points(x, y, view, W, H, ("count", "countv", "sumz"), val=v)
+    np.testing.assert_array_equal(got["count"], ref["count"])
+    np.testing.assert_array_equal(got["countv"], ref["countv"])
+    if threads == 1:  # one partition: the oracle's row order exactly
+        np.testing.assert_array_equal(got["sumz"], ref["sumz"])
+    else:             # partitions merged by the combine tree: another summation order
+        np.testing.assert_allclose(got["sumz"], ref["sumz"], rtol=1e-12)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        mean = np.where(ref["countv"] > 0, ref["sumz"] / ref["countv"], np.nan)
+    np.testing.assert_allclose(got["mean"], mean, rtol=1e-12)
+    assert np.array_equal(np.isnan(got["mean"]), np.isnan(mean))
+
+
+@pytest.mark.parametrize("threads", [1, 5])
+def test_cpu_baseline_count_f64_and_categories_equal_oracle(threads):
+    from oracle import baseline
+    rng = np.random.default_rng(32)
+    n, W, H = 150_000, 80, 60
+    x = rng.normal(0, 5, n)
+    y = rng.normal(0, 5, n)
+    x[::997] = np.nan
+    view = oracle.View(4.0, 40.0, 3.0, 30.0, -10, 10, -10, 10)
+    got = baseline.points_count_f64(x, y, view, W, H, threads)
+    np.testing.assert_array_equal(got, oracle.points(x, y, view, W, H, ("count",))["count"])
+    cat = rng.integers(0, 16, n).astype(np.uint8)
+    got = baseline.points_count_cat_f32(x.astype(np.float32), y.astype(np.float32), cat, view,
+                                        W, H, 16, threads)
+    ref = oracle.points(x.astype(np.float32), y.astype(np.float32), view, W, H, ("count",),
+                        cat=cat.astype(np.int64), ncat=16, squeeze=False)["count"]
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_cpu_baseline_lines_equal_oracle(threads):
+    from oracle import baseline
+    rng = np.random.default_rng(33)
+    nl, nv, W, H = 600, 64, 120, 90
+    xs = np.linspace(0, 1, nv)
+    ys = np.cumsum(rng.normal(0, 0.05, (nl, nv)), axis=1).astype(np.float32)
+    ys[3, 10] = np.nan
+    ys[7, :] = 0.25            # a constant line: horizontal segments only
+    lo, hi = float(np.nanmin(ys)), float(np.nanmax(ys))
+    sx, sy = W / 1.0, H / (hi - lo)
+    view = oracle.View(sx, 0.0, sy, -lo * sy, 0.0, 1.0, lo, hi, yf32=True)
+    got = baseline.lines_xconst_f32(xs, ys, view, W, H, threads)
+    ref = oracle.lines_xconst(xs, ys.astype(np.float64), view, W, H)["count"]
+    np.testing.assert_array_equal(got, ref)
+    # a narrower view clips many segments (float32 deltas on the clipped ones)
+    view = oracle.View(W / 0.5, -0.25 * W / 0.5, H / 0.4, 0.2 * H / 0.4, 0.25, 0.75, -0.2, 0.2,
+                       yf32=True)
+    got = baseline.lines_xconst_f32(xs, ys, view, W, H, threads)
+    ref = oracle.lines_xconst(xs, ys.astype(np.float64), view, W, H)["count"]
+    np.testing.assert_array_equal(got, ref)
