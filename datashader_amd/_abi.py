@@ -36,6 +36,8 @@ EXPORTS = ("dsx_points_workspace_size", "dsx_points_agg", "dsx_lines_agg",
            "dsx_points_indexed_workspace_size", "dsx_points_agg_indexed",
            "dsx_shade_workspace_size", "dsx_shade", "dsx_combine", "dsx_points_rowindex",
            "dsx_take", "dsx_points_nselect_workspace_size", "dsx_points_nselect",
+           "dsx_batch_workspace_size", "dsx_batch_table_bytes", "dsx_batch_tile_cells",
+           "dsx_batch_create", "dsx_batch_table", "dsx_batch_submit", "dsx_batch_destroy",
            "dsx_abi_version", "dsx_strerror", "dsx_last_hip_error")
 
 vp = ctypes.c_void_p
@@ -127,6 +129,31 @@ class CombineDesc(ctypes.Structure):
                 ("reserved", ctypes.c_int32), ("into", Planes), ("part", Planes)]
 
 
+BATCH_COUNT, BATCH_COUNTV, BATCH_SUM, BATCH_MEAN = 1, 2, 4, 8
+
+
+class BatchReq(ctypes.Structure):
+    _fields_ = [("sx", ctypes.c_double), ("tx", ctypes.c_double),
+                ("sy", ctypes.c_double), ("ty", ctypes.c_double),
+                ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+                ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32),
+                ("part_rows", ctypes.c_int64), ("tile_base", ctypes.c_int32),
+                ("reserved", ctypes.c_int32),
+                ("count", vp), ("countv", vp), ("sum", vp), ("mean", vp)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("x", vp), ("y", vp), ("val", vp),
+                ("grid", ctypes.c_int32), ("planes", ctypes.c_uint32),
+                ("x0", ctypes.c_double), ("x1", ctypes.c_double),
+                ("y0", ctypes.c_double), ("y1", ctypes.c_double), ("cell_start", vp),
+                ("max_requests", ctypes.c_int32), ("max_tiles", ctypes.c_int32),
+                ("max_regions", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("workgroups", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class ShadeDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("dtype", ctypes.c_int32),
                 ("H", ctypes.c_int64), ("W", ctypes.c_int64), ("ncat", ctypes.c_int64),
@@ -186,6 +213,17 @@ def load():
         lib.dsx_shade_workspace_size.argtypes = [P(ShadeDesc)]
         lib.dsx_shade_workspace_size.restype = sz
         lib.dsx_shade.argtypes = [P(ShadeDesc), vp, sz, vp]
+        lib.dsx_batch_workspace_size.argtypes = [P(BatchConfig)]
+        lib.dsx_batch_workspace_size.restype = sz
+        lib.dsx_batch_table_bytes.argtypes = [i32]
+        lib.dsx_batch_table_bytes.restype = sz
+        lib.dsx_batch_tile_cells.argtypes = [P(BatchConfig)]
+        lib.dsx_batch_tile_cells.restype = i32
+        lib.dsx_batch_create.argtypes = [P(BatchConfig), vp, sz, vp, i32, P(vp)]
+        lib.dsx_batch_table.argtypes = [vp]
+        lib.dsx_batch_table.restype = P(BatchReq)
+        lib.dsx_batch_submit.argtypes = [vp, i32, vp]
+        lib.dsx_batch_destroy.argtypes = [vp]
         lib.dsx_abi_version.argtypes = []
         lib.dsx_strerror.argtypes = [ctypes.c_int]
         lib.dsx_strerror.restype = ctypes.c_char_p
@@ -194,7 +232,8 @@ def load():
         for f in ("dsx_points_agg", "dsx_lines_agg", "dsx_bounds", "dsx_bounds_noalloc", "dsx_fill", "dsx_finalize",
                   "dsx_grid_index_build", "dsx_gather_rows", "dsx_points_agg_indexed",
                   "dsx_shade", "dsx_combine", "dsx_points_rowindex", "dsx_take",
-                  "dsx_points_nselect", "dsx_abi_version"):
+                  "dsx_points_nselect", "dsx_batch_create", "dsx_batch_submit",
+                  "dsx_batch_destroy", "dsx_abi_version"):
             getattr(lib, f).restype = ctypes.c_int
         if lib.dsx_abi_version() != ABI_VERSION:
             raise RuntimeError("libdsx.so ABI version mismatch; rebuild the library")

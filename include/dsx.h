@@ -362,6 +362,71 @@ typedef struct {
 
 int dsx_gather_rows(const dsx_gather_desc *g, void *stream);
 
+/* ---- batched viewports (the served form) ------------------------------------------- *
+ * No counterpart in the reference, which scans every row on every call
+ * (core.py:1334-1359 -> data_libraries/pandas.py:23-66).  An engine holds one resident
+ * set sorted by dsx_grid_index_build (float32 x, y, value) and answers batches of
+ * viewport requests -- each its own ranges (linear axes), canvas size and output planes
+ * -- with one replay of a HIP graph captured at dsx_batch_create: a fixed kernel
+ * sequence driven by a work queue of (request, canvas tile, row part) items that
+ * persistent workgroups drain.  Each request's planes are those of dsx_points_agg over
+ * the whole set for count() / count(v) / sum(v) / mean(v) (reductions.py 582-584,
+ * 554-558, 1056-1063 with 1092-1096, 1294-1297), finalized: count and count(v) u32, sum
+ * f64 (NaN where no value), mean f64 (NaN where no value).  Every output cell is
+ * written, so outputs need no initialisation.                                          */
+#define DSX_BATCH_COUNT 1
+#define DSX_BATCH_COUNTV 2
+#define DSX_BATCH_SUM 4
+#define DSX_BATCH_MEAN 8
+
+typedef struct {
+    double sx, tx, sy, ty;        /* the request's view (dsx_view, linear axes, sx, sy > 0) */
+    double xmin, xmax, ymin, ymax;
+    int32_t width, height;
+    int32_t tile_w, tile_h;       /* in: tile shape in pixels (0: default); dsx_batch_submit */
+                                  /* narrows it to the LDS tile                             */
+    int64_t part_rows;            /* rows per work item of a heavy tile (0: 65536)          */
+    int32_t tile_base;            /* set by dsx_batch_submit                                */
+    int32_t reserved;
+    uint32_t *count, *countv;     /* device outputs, H*W each (those the engine produces)   */
+    double *sum, *mean;
+} dsx_batch_req;
+
+typedef struct {
+    int64_t n;                    /* resident rows, < 2^32                                  */
+    const float *x, *y, *val;     /* the index's sorted columns (val NULL: count only)      */
+    int32_t grid;                 /* the index's lattice, 1 .. 4096                         */
+    uint32_t planes;              /* DSX_BATCH_* bits                                       */
+    double x0, x1, y0, y1;        /* the lattice's extent                                   */
+    const int64_t *cell_start;    /* device [grid*grid + 1]                                 */
+    int32_t max_requests;         /* requests per batch                                     */
+    int32_t max_tiles;            /* canvas tiles per batch (all requests)                  */
+    int32_t max_regions;          /* scratch regions for the parts of split tiles           */
+    int32_t threads;              /* aggregation workgroup: 256, 512 or 1024 threads        */
+    int32_t workgroups;           /* persistent aggregation workgroups                      */
+    int32_t reserved;
+} dsx_batch_config;
+
+typedef struct dsx_batch dsx_batch;
+
+/* Device workspace of an engine, the pinned host table it reads requests from
+ * (dsx_batch_table_bytes, allocated device-mapped by the caller, e.g. hipHostMalloc), and
+ * the cells of one LDS tile. */
+size_t dsx_batch_workspace_size(const dsx_batch_config *c);
+size_t dsx_batch_table_bytes(int32_t max_requests);
+int32_t dsx_batch_tile_cells(const dsx_batch_config *c);
+/* Create an engine over caller-owned workspace and pinned table; with use_graph the
+ * kernel sequence is captured once into a HIP graph (replayed by every submit), else it
+ * is launched kernel by kernel. */
+int dsx_batch_create(const dsx_batch_config *c, void *workspace, size_t workspace_bytes,
+                     void *host_table, int32_t use_graph, dsx_batch **out);
+/* The request slots of the pinned table (max_requests of them). */
+dsx_batch_req *dsx_batch_table(dsx_batch *b);
+/* Validate the first nreq requests written into the table and run the batch on stream.
+ * The table must not be rewritten until the batch has completed. */
+int dsx_batch_submit(dsx_batch *b, int32_t nreq, void *stream);
+int dsx_batch_destroy(dsx_batch *b);
+
 /* ---- shade: aggregate planes -> RGBA images ---------------------------------------- *
  * Replaces transfer_functions/__init__.py shade (616-747) for device-resident planes:
  *   DSX_SHADE_CMAP       _interpolate (251-359) with a list colormap: colour channels

@@ -43,7 +43,8 @@ STRUCTS = {
     "dsx_gather_desc": _abi.GatherDesc, "dsx_grid_ref": _abi.GridRef,
     "dsx_shade_desc": _abi.ShadeDesc, "dsx_combine_desc": _abi.CombineDesc,
     "dsx_rowindex_desc": _abi.RowIndexDesc, "dsx_take_desc": _abi.TakeDesc,
-    "dsx_nselect_desc": _abi.NSelectDesc,
+    "dsx_nselect_desc": _abi.NSelectDesc, "dsx_batch_req": _abi.BatchReq,
+    "dsx_batch_config": _abi.BatchConfig,
 }
 
 
