@@ -12,7 +12,7 @@ import os
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libdsx.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # dtype codes
 F32, F64, I8, U8, I16, I32, I64 = 1, 2, 3, 4, 5, 6, 7
@@ -35,6 +35,7 @@ EXPORTS = ("dsx_points_workspace_size", "dsx_points_agg", "dsx_lines_agg",
            "dsx_grid_index_workspace_size", "dsx_grid_index_build", "dsx_gather_rows",
            "dsx_points_indexed_workspace_size", "dsx_points_agg_indexed",
            "dsx_shade_workspace_size", "dsx_shade", "dsx_combine", "dsx_points_rowindex",
+           "dsx_points_rowindex_workspace_size",
            "dsx_take", "dsx_points_nselect_workspace_size", "dsx_points_nselect",
            "dsx_batch_workspace_size", "dsx_batch_table_bytes", "dsx_batch_tile_cells",
            "dsx_batch_create", "dsx_batch_table", "dsx_batch_submit", "dsx_batch_destroy",
@@ -205,7 +206,9 @@ def load():
         lib.dsx_points_agg_indexed.argtypes = [P(PointsDesc), P(View), P(Planes), vp,
                                                P(GridRef), vp, sz, vp]
         lib.dsx_combine.argtypes = [P(CombineDesc), vp]
-        lib.dsx_points_rowindex.argtypes = [P(PointsDesc), P(View), P(RowIndexDesc), vp]
+        lib.dsx_points_rowindex.argtypes = [P(PointsDesc), P(View), P(RowIndexDesc), vp, sz, vp]
+        lib.dsx_points_rowindex_workspace_size.argtypes = [P(PointsDesc), P(View), P(RowIndexDesc)]
+        lib.dsx_points_rowindex_workspace_size.restype = sz
         lib.dsx_take.argtypes = [P(TakeDesc), vp]
         lib.dsx_points_nselect_workspace_size.argtypes = [P(PointsDesc), P(View), P(NSelectDesc)]
         lib.dsx_points_nselect_workspace_size.restype = sz

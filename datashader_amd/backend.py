@@ -421,7 +421,17 @@ def row_index_leaf(lib, frame, schema, glyph, view, red, passes, results, H, W):
     d, keep = _pass_desc(frame, schema, glyph, view, p, device, catc)
     if key is not None and d.val_dtype != (_abi.F32 if res.keyb == 4 else _abi.F64):
         raise _abi.DsxError("key plane width does not match the value column")
-    _abi.check(lib.dsx_points_rowindex(d, _view_struct(view), r, dev.stream_handle()),
+    vw = _view_struct(view)
+    if key is None and getattr(frame, "persistent", False):
+        # the LDS-window pass keeps its hot window per (frame, canvas, view) like the
+        # points pass (a window of one 32-bit row per cell)
+        wkey = ("rowindex", glyph.x, glyph.y, valcol, None if cat is None else cat._key(),
+                W, H, tuple(view[k] for k in ("xmin", "xmax", "ymin", "ymax")))
+        win, reuse = _window_slot(frame, wkey, device)
+        d.window, d.reuse_window = dev.ptr(win), reuse
+    ws = 0 if key is not None else int(lib.dsx_points_rowindex_workspace_size(d, vw, r))
+    work = t.empty(ws, dtype=t.uint8, device=device) if ws else None
+    _abi.check(lib.dsx_points_rowindex(d, vw, r, dev.ptr(work), ws, dev.stream_handle()),
                "dsx_points_rowindex")
     shape = (H, W, ncat) if isinstance(red, rd.by) else (H, W)
     if lookup is None:

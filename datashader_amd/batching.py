@@ -66,8 +66,8 @@ def _device_count():
 class BatchEngine:
     """One batch engine over a :class:`serving.PointIndex` (float32 columns)."""
 
-    def __init__(self, index, planes=("count", "mean"), max_requests=32, max_tiles=16384,
-                 max_regions=768, threads=1024, workgroups=None, use_graph=True):
+    def __init__(self, index, planes=("count", "mean"), max_requests=32, max_tiles=65536,
+                 max_regions=4096, threads=1024, workgroups=None, use_graph=True):
         t = dev.torch()
         dev.require_gpu()
         frame = index.frame
@@ -122,26 +122,34 @@ class BatchEngine:
         except Exception:
             pass
 
-    def tiles_of(self, vw, tile_h):
-        """Tiles a request takes: its tile shape narrowed as dsx_batch_submit narrows it
-        (at most the LDS tile's cells)."""
+    def shape(self, vw, tile_h, tile_cells):
+        """(tile_w, tile_h) as dsx_batch_submit narrows them: at most the LDS tile's
+        cells, at most the canvas."""
         W, H = vw["W"], vw["H"]
-        th = max(1, min(tile_h if tile_h > 0 else 64, H, self.tile_cells))
-        tw = max(1, min(self.tile_cells // th, W))
-        return -(-W // tw) * -(-H // th)
+        cells = max(1, min(int(tile_cells or self.tile_cells), self.tile_cells))
+        th = max(1, min(int(tile_h) if tile_h > 0 else 64, H, cells))
+        tw = max(1, min(cells // th, W))
+        return tw, th
+
+    def tiles_of(self, vw, tile_h, tile_cells=0):
+        """Tiles a request takes with this tile shape."""
+        tw, th = self.shape(vw, tile_h, tile_cells)
+        return -(-vw["W"] // tw) * -(-vw["H"] // th)
 
     def submit(self, jobs):
         """Launch one batch on the engine's stream.  ``jobs``: list of
-        (view dict, {plane: device tensor}, tile_h, part_rows); returns at once."""
+        (view dict, {plane: device tensor}, knobs dict); returns at once."""
         if len(jobs) > self.max_requests:
             raise ValueError("more requests than the engine's table holds")
         s = self._slots
-        for k, (vw, outs, tile_h, part_rows) in enumerate(jobs):
+        for k, (vw, outs, knobs) in enumerate(jobs):
+            tw, th = self.shape(vw, knobs["tile_h"], knobs.get("tile_cells", 0))
+            part_rows = knobs["part_rows"]
             q = s[k]
             q.sx, q.tx, q.sy, q.ty = vw["sx"], vw["tx"], vw["sy"], vw["ty"]
             q.xmin, q.xmax, q.ymin, q.ymax = vw["xmin"], vw["xmax"], vw["ymin"], vw["ymax"]
             q.width, q.height = vw["W"], vw["H"]
-            q.tile_h, q.tile_w = int(tile_h), 0
+            q.tile_h, q.tile_w = int(th), int(tw)
             q.part_rows = int(part_rows)
             q.count = outs["count"].data_ptr() if "count" in outs else None
             q.countv = outs["countv"].data_ptr() if "countv" in outs else None
@@ -160,11 +168,13 @@ class BatchEngine:
 
 
 class Tuning:
-    """The tuned knobs: global (threads, workgroups) and per request class
-    (tile_h, part_rows).  Classes: canvas cells (log2 bucket) x candidate rows per canvas
-    cell (log4 bucket)."""
+    """The tuned knobs: global (threads, workgroups) and per request class (tile_h,
+    tile_cells -- the tile's pixels, at most the LDS tile --, part_rows).  Classes: canvas
+    cells (log2 bucket) x candidate rows per canvas cell (log4 bucket): dense views want
+    small tiles (each reads only the lattice cells under it) and long parts, sparse ones
+    large tiles."""
 
-    DEFAULT = {"tile_h": 64, "part_rows": 1 << 16}
+    DEFAULT = {"tile_h": 64, "tile_cells": 0, "part_rows": 1 << 18}
 
     def __init__(self, threads=1024, workgroups=None, per_class=None):
         self.threads, self.workgroups = threads, workgroups
@@ -225,13 +235,15 @@ class BatchTuner:
     """Autotuner of the batch engine, keyed on canvas size and point skew (candidate rows
     per canvas cell of a request).
 
-    For every class of the sample requests it times each candidate (tile_h, part_rows)
-    on the class's requests, one request per batch (HIP events on the engine's stream),
-    and keeps the fastest; then it times the whole sample in batches under each candidate
-    (threads, workgroups) and keeps the fastest.  The result is a :class:`Tuning`."""
+    For every class of the sample requests it times each candidate (tile_h, tile_cells,
+    part_rows) on the class's requests, one request per batch (HIP events on the engine's
+    stream), and keeps the fastest; then it times the whole sample in batches under each
+    candidate (threads, workgroups) and keeps the fastest.  The result is a
+    :class:`Tuning`."""
 
-    TILE_H = (16, 32, 64, 128)
-    PART_ROWS = (1 << 15, 1 << 16, 1 << 17, 1 << 18)
+    TILE_H = (16, 64)
+    TILE_DIV = (1, 4, 16)          # tile_cells = LDS tile cells / TILE_DIV
+    PART_ROWS = (1 << 17, 1 << 19, 1 << 21)
     THREADS = (512, 1024)
     WG_PER_CU = (1, 2)
 
@@ -266,12 +278,14 @@ class BatchTuner:
         per_class = {}
         for key, idx in sorted(classes.items()):
             best, best_t = None, float("inf")
-            for th in self.TILE_H:
-                for pr in self.PART_ROWS:
-                    jl = [[(views[i], outs[i], th, pr)] for i in idx]
-                    ms = self._time(eng, jl)
-                    if ms < best_t:
-                        best, best_t = {"tile_h": th, "part_rows": pr}, ms
+            for div in self.TILE_DIV:
+                for th in self.TILE_H:
+                    for pr in self.PART_ROWS:
+                        kn = {"tile_h": th, "tile_cells": eng.tile_cells // div,
+                              "part_rows": pr}
+                        ms = self._time(eng, [[(views[i], outs[i], kn)] for i in idx])
+                        if ms < best_t:
+                            best, best_t = kn, ms
             per_class[key] = best
             if verbose:
                 print(f"[tune] class {key}: {len(idx)} requests -> {best} ({best_t:.3f} ms)",
@@ -284,12 +298,17 @@ class BatchTuner:
             for k in self.WG_PER_CU:
                 e = BatchEngine(self.index, self.planes, max_requests=max_batch, threads=th,
                                 workgroups=k * cus)
-                jl = []
-                for s in range(0, len(views), max_batch):
-                    jl.append([(views[i], outs[i],
-                                tuning.knobs(views[i]["W"], views[i]["H"], rows[i])["tile_h"],
-                                tuning.knobs(views[i]["W"], views[i]["H"], rows[i])["part_rows"])
-                               for i in range(s, min(s + max_batch, len(views)))])
+                jl, cur, ntl = [], [], 0
+                for i in range(len(views)):  # batches within the table and the tile budget
+                    kn = tuning.knobs(views[i]["W"], views[i]["H"], rows[i])
+                    nt = e.tiles_of(views[i], kn["tile_h"], kn.get("tile_cells", 0))
+                    if cur and (len(cur) == max_batch or ntl + nt > e.max_tiles):
+                        jl.append(cur)
+                        cur, ntl = [], 0
+                    cur.append((views[i], outs[i], kn))
+                    ntl += nt
+                if cur:
+                    jl.append(cur)
                 ms = self._time(e, jl)
                 e.close()
                 if verbose:
@@ -338,8 +357,7 @@ class BatchServer:
     def _job(self, req):
         vw = _view_of(req)
         W, H = vw["W"], vw["H"]
-        knobs = self.tuning.knobs(W, H, self.est.rows(req.x_range, req.y_range))
-        return vw, knobs["tile_h"], knobs["part_rows"]
+        return vw, self.tuning.knobs(W, H, self.est.rows(req.x_range, req.y_range))
 
     def result(self, req, vw, outs):
         """The Canvas.points result (DataArray / Dataset) of a served request."""
@@ -383,9 +401,9 @@ class BatchServer:
                 ids, jobs, outs_l, views, nbytes, tiles = [], [], [], [], 0, 0
                 while waiting and len(ids) < self.max_batch:
                     i = waiting[0]
-                    vw, th, pr = self._job(requests[i])
+                    vw, kn = self._job(requests[i])
                     need = vw["W"] * vw["H"] * sum(_PLANE_BYTES[p] for p in self.planes)
-                    nt = eng.tiles_of(vw, th)
+                    nt = eng.tiles_of(vw, kn["tile_h"], kn.get("tile_cells", 0))
                     if ids and (held + nbytes + need > self.mem_budget or
                                 tiles + nt > eng.max_tiles):
                         break
@@ -393,7 +411,7 @@ class BatchServer:
                     with t.cuda.stream(eng.stream):
                         outs = _alloc_outputs(self.planes, vw["W"], vw["H"], self.device)
                     ids.append(i)
-                    jobs.append((vw, outs, th, pr))
+                    jobs.append((vw, outs, kn))
                     outs_l.append(outs)
                     views.append(vw)
                     nbytes += need

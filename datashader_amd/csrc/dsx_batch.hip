@@ -401,24 +401,35 @@ __global__ __launch_bounds__(NT) void b_agg_kernel(BArgs A) {
         const int32_t nr = nrow;
         int32_t top = 1;  // binary search over the table: the largest power of two < nr
         while (2 * top < nr) top *= 2;
-        for (unsigned long long c0 = lo; c0 < hi; c0 += (unsigned long long)(NT * kBItems)) {
-            float xs[kBItems], ys[kBItems], vs[kBItems];
-#pragma unroll
-            for (int it = 0; it < kBItems; ++it) {
-                unsigned long long j = c0 + (unsigned long long)(it * NT + threadIdx.x);
-                j = j < hi ? j : lo;  // past the end: re-read the first row, dropped below
-                int32_t k = 0;        // the last lattice row starting at or before j
-                for (int32_t step = top; step >= 1; step >>= 1)
-                    if (k + step < nr && (unsigned long long)s_pre[k + step] <= j) k += step;
-                const int64_t row = (int64_t)s_start[k] + (int64_t)(j - s_pre[k]);
-                xs[it] = A.x[row];
-                ys[it] = A.y[row];
-                vs[it] = L.has_val ? A.v[row] : 0.0f;
+        // the last lattice row starting at or before j
+        auto row_of = [&](unsigned long long j) {
+            int32_t k = 0;
+            for (int32_t step = top; step >= 1; step >>= 1)
+                if (k + step < nr && (unsigned long long)s_pre[k + step] <= j) k += step;
+            return k;
+        };
+        // Each thread keeps a run of rows that land in one cell in registers and adds it to
+        // the LDS region when the cell changes: the rows are sorted by lattice cell, so in
+        // a dense view a thread's rows (NT apart) fall in one pixel for long stretches, and
+        // the run replaces per-row LDS atomics that would all hit the same address.
+        uint32_t run_lc = 0, run_n = 0, run_nan = 0;
+        double run_s = 0.0;
+        auto flush = [&]() {
+            if (run_n == 0u) return;
+            atomicAdd(reinterpret_cast<uint32_t *>(smem + L.o_cnt) + run_lc, run_n);
+            if (L.has_val) {
+                if (run_nan) atomicAdd(reinterpret_cast<uint32_t *>(smem + L.o_nan) + run_lc, run_nan);
+                if (run_n > run_nan) atomicAdd(reinterpret_cast<double *>(smem + L.o_sum) + run_lc, run_s);
             }
+            run_n = 0u;
+            run_nan = 0u;
+            run_s = 0.0;
+        };
+        // project kBItems loaded rows and add the tile's ones to the thread's run
+        auto bin = [&](const float *xs, const float *ys, const float *vs, const bool *ok) {
 #pragma unroll
             for (int it = 0; it < kBItems; ++it) {
-                const unsigned long long j = c0 + (unsigned long long)(it * NT + threadIdx.x);
-                if (j >= hi) continue;
+                if (!ok[it]) continue;
                 const double x = (double)xs[it], y = (double)ys[it];
                 if (!((q.xmin <= x && x <= q.xmax) && (q.ymin <= y && y <= q.ymax))) continue;
                 int32_t xx = proj(x, q.sx, q.tx), yy = proj(y, q.sy, q.ty);
@@ -428,14 +439,61 @@ __global__ __launch_bounds__(NT) void b_agg_kernel(BArgs A) {
                 if (yy < 0) yy += q.H;
                 if (xx < px0 || xx >= px1 || yy < py0 || yy >= py1) continue;  // another tile's
                 const uint32_t lc = (uint32_t)((yy - py0) * q.tw + (xx - px0));
-                atomicAdd(reinterpret_cast<uint32_t *>(smem + L.o_cnt) + lc, 1u);
+                if (lc != run_lc) {
+                    flush();
+                    run_lc = lc;
+                }
+                ++run_n;
                 if (L.has_val) {
                     const double v = (double)vs[it];
-                    if (v != v) atomicAdd(reinterpret_cast<uint32_t *>(smem + L.o_nan) + lc, 1u);
-                    else atomicAdd(reinterpret_cast<double *>(smem + L.o_sum) + lc, v);
+                    if (v != v) ++run_nan;
+                    else run_s += v;
                 }
             }
+        };
+        // Rows [lo, hi) of the tile's concatenated lattice rows, NT * kBItems a chunk, the
+        // next chunk's loads issued before this chunk is binned.  A chunk that lies inside
+        // one lattice row (the common case for a wide view, whose lattice rows are long)
+        // is one contiguous run of sorted rows, found by one search for the whole chunk;
+        // any other chunk finds each row's lattice row by a search of the table.
+        struct Set {
+            float x[kBItems], y[kBItems], v[kBItems];
+            bool ok[kBItems];
+        };
+        auto load = [&](unsigned long long c, Set &S) {
+            const int32_t kc = row_of(c < tot ? c : (tot > 0 ? tot - 1 : 0));  // uniform
+            const unsigned long long kend = kc + 1 < nr ? (unsigned long long)s_pre[kc + 1] : tot;
+            const bool one = c + (unsigned long long)(NT * kBItems) <= kend;   // uniform
+#pragma unroll
+            for (int it = 0; it < kBItems; ++it) {
+                const unsigned long long j = c + (unsigned long long)(it * NT + threadIdx.x);
+                S.ok[it] = j < hi;
+                int64_t row;
+                if (one) {  // j < kend: a row of lattice row kc (read even past hi, dropped)
+                    row = (int64_t)s_start[kc] + (int64_t)(j - s_pre[kc]);
+                } else {
+                    const unsigned long long jj = S.ok[it] ? j : lo;  // past hi: row lo, dropped
+                    const int32_t k = row_of(jj);
+                    row = (int64_t)s_start[k] + (int64_t)(jj - s_pre[k]);
+                }
+                S.x[it] = A.x[row];
+                S.y[it] = A.y[row];
+                S.v[it] = L.has_val ? A.v[row] : 0.0f;
+            }
+        };
+        if (lo < hi) {
+            constexpr unsigned long long kStep = (unsigned long long)NT * kBItems;
+            Set sa, sb;
+            load(lo, sa);
+            // no exit between the two halves: a step past hi loads row lo and bins nothing
+            for (unsigned long long c = lo; c < hi; c += 2 * kStep) {
+                load(c + kStep, sb);
+                bin(sa.x, sa.y, sa.v, sa.ok);
+                load(c + 2 * kStep, sa);
+                bin(sb.x, sb.y, sb.v, sb.ok);
+            }
         }
+        flush();
         __syncthreads();
         if (d.nparts > 1) {
             // a split tile's part: its region goes to scratch, the merge kernel folds it

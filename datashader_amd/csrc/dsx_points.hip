@@ -1912,8 +1912,204 @@ static int rowindex_launch(const RiArgs &r, int val_dtype, hipStream_t st) {
     return launch_status();
 }
 
+// ---- row-index reductions with an LDS window ---------------------------------------------
+// FIRST / LAST: the rows that land in the call's hot window (picked from a sample of the
+// rows, as for dsx_points_agg, but holding one 32-bit row per cell, so four times the
+// cells) keep their cell's smallest / largest row in a per-workgroup LDS copy of the
+// window (an LDS atomic min / max); the other rows take the global pass's atomic with its
+// skip test.  A merge then folds the workgroups' windows into the index plane (the global
+// atomics never touch a window cell).  Rows are < 2^32 (checked on the host), so a row fits
+// 32 bits: LAST keeps row + 1, so that 0 is the empty cell.
+constexpr int kRiwThreads = 1024;
+constexpr int kRiwItems = 4;
+constexpr int kRiwChunk = kRiwThreads * kRiwItems;
+
+struct RiwArgs {
+    PointArgs a;
+    int32_t last, nan_check;
+    unsigned long long *index;
+    const int32_t *window;
+    uint32_t cap;                 // window cells (pixels x categories)
+    uint32_t *partials;           // [G][cap]
+};
+
+struct RiwPlan {
+    uint32_t cap;
+    int groups, bshift, cw, ch;
+    size_t lds, o_window, o_hist, o_partials, total;
+};
+
+static int riw_plan(const dsx_points_desc *d, const dsx_view *v, RiwPlan &P) {
+    const int ncat = d->cat.kind == DSX_CAT_NONE ? 1 : d->cat.ncat;
+    P.lds = (size_t)kLdsBytes - 1024;  // the window, less the kernel's static words
+    P.cap = (uint32_t)(P.lds / 4);
+    if ((int64_t)P.cap / ncat < 1) return DSX_ERR_SHAPE;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0)
+            cus = c;
+    }
+    const int64_t chunks = (d->n + kRiwChunk - 1) / kRiwChunk;
+    P.groups = (int)(chunks < cus ? (chunks < 1 ? 1 : chunks) : cus);
+    P.bshift = 0;
+    while ((int64_t)((v->width + (1 << P.bshift) - 1) >> P.bshift) *
+               ((v->height + (1 << P.bshift) - 1) >> P.bshift) > kSelBins)
+        ++P.bshift;
+    P.cw = (v->width + (1 << P.bshift) - 1) >> P.bshift;
+    P.ch = (v->height + (1 << P.bshift) - 1) >> P.bshift;
+    size_t o = 0;
+    P.o_window = o; o = align_up(o + 16, 256);
+    P.o_hist = o; o = align_up(o + 4 * (size_t)kSelBins, 256);
+    P.o_partials = o; o = align_up(o + 4 * (size_t)P.cap * (size_t)P.groups, 256);
+    P.total = o;
+    return DSX_OK;
+}
+
+template <typename TC, typename TV>
+__global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r) {
+    extern __shared__ __align__(16) uint32_t win[];
+    const PointArgs &a = r.a;
+    const int ncat = a.cat.ncat;
+    const Window w = load_window(r.window, a.W, a.H, (int64_t)(r.cap / (uint32_t)ncat));
+    const uint32_t empty = r.last ? 0u : 0xFFFFFFFFu;
+    for (uint32_t c = threadIdx.x; c < r.cap; c += kRiwThreads) win[c] = empty;
+    __syncthreads();
+    const TC *X = reinterpret_cast<const TC *>(a.x);
+    const TC *Y = reinterpret_cast<const TC *>(a.y);
+    const TV *V = reinterpret_cast<const TV *>(a.val);
+    const int64_t nchunks = (a.n + kRiwChunk - 1) / kRiwChunk;
+    // FIRST visits the chunks in increasing row order and LAST in decreasing order, so
+    // that after the first sweep most rows fail the global skip test
+    struct Set {
+        TC x[kRiwItems], y[kRiwItems];
+        TV v[kRiwItems];
+    };
+    // chunk k of this workgroup's sequence (past the last: re-reads the last row, dropped)
+    auto base_of = [&](int64_t k) {
+        const int64_t ck = r.last ? nchunks - 1 - k : k;
+        return ck >= 0 && ck < nchunks ? ck * kRiwChunk : a.n;
+    };
+    auto load = [&](int64_t k, Set &S) {
+        const int64_t b = base_of(k);
+#pragma unroll
+        for (int it = 0; it < kRiwItems; ++it) {
+            int64_t i = b + it * kRiwThreads + threadIdx.x;
+            i = i < a.n ? i : a.n - 1;
+            S.x[it] = stream_load(X + i);
+            S.y[it] = stream_load(Y + i);
+            S.v[it] = r.nan_check ? stream_load(V + i) : TV(0);
+        }
+    };
+    // rows of chunk k (a chunk past the last has base a.n: every row dropped)
+    auto bin = [&](int64_t k, const Set &S) {
+        const int64_t b = base_of(k);
+#pragma unroll
+        for (int it = 0; it < kRiwItems; ++it) {
+            const int64_t row = b + it * kRiwThreads + threadIdx.x;
+            if (row >= a.n) continue;
+            int32_t px, py, pc;
+            if (!locate<TC>(a, row, S.x[it], S.y[it], px, py, pc)) continue;
+            if (r.nan_check && S.v[it] != S.v[it]) continue;
+            if (in_window(w, px, py)) {
+                const uint32_t lc = (uint32_t)(((py - w.y0) * w.w + (px - w.x0)) * ncat + pc);
+                if (r.last) atomicMax(&win[lc], (uint32_t)row + 1u);
+                else atomicMin(&win[lc], (uint32_t)row);
+                continue;
+            }
+            unsigned long long *slot = r.index + ((int64_t)py * a.W + px) * ncat + pc;
+            const long long cur = *reinterpret_cast<volatile long long *>(slot);
+            if (r.last) {
+                if ((long long)row > cur) atomicMax(reinterpret_cast<long long *>(slot), (long long)row);
+            } else if ((unsigned long long)row < (unsigned long long)cur) {
+                atomicMin(slot, (unsigned long long)row);  // -1 (empty) is the largest
+            }
+        }
+    };
+    // two register sets alternate: the next chunk's loads are in flight while one bins
+    // (no copy between them: a copy would wait for its loads)
+    Set sa, sb;
+    const int64_t G = gridDim.x;
+    load(blockIdx.x, sa);
+    for (int64_t k = blockIdx.x; k < nchunks; k += 2 * G) {
+        load(k + G, sb);
+        bin(k, sa);
+        load(k + 2 * G, sa);
+        bin(k + G, sb);
+    }
+    __syncthreads();
+    uint32_t *mine = r.partials + (size_t)blockIdx.x * r.cap;
+    const uint32_t used = (uint32_t)(w.w * w.h * ncat);
+    for (uint32_t c = threadIdx.x; c < used; c += kRiwThreads) mine[c] = win[c];
+}
+
+// The window cells of the index plane: min (FIRST) / max (LAST) over the workgroups.
+__global__ __launch_bounds__(256) void rowindex_window_merge(RiwArgs r, int groups) {
+    const PointArgs &a = r.a;
+    const int ncat = a.cat.ncat;
+    const Window w = load_window(r.window, a.W, a.H, (int64_t)(r.cap / (uint32_t)ncat));
+    const uint32_t used = (uint32_t)(w.w * w.h * ncat);
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < used; c += gridDim.x * blockDim.x) {
+        uint32_t m = r.last ? 0u : 0xFFFFFFFFu;
+        for (int g = 0; g < groups; ++g) {
+            const uint32_t x = r.partials[(size_t)g * r.cap + c];
+            m = r.last ? (x > m ? x : m) : (x < m ? x : m);
+        }
+        const uint32_t pix = c / (uint32_t)ncat, cat = c - pix * (uint32_t)ncat;
+        const int32_t ly = (int32_t)(pix / (uint32_t)w.w), lx = (int32_t)(pix - (uint32_t)ly * w.w);
+        const int64_t g = ((int64_t)(w.y0 + ly) * a.W + (w.x0 + lx)) * ncat + cat;
+        long long out;
+        if (r.last) out = m == 0u ? -1ll : (long long)(m - 1u);
+        else out = m == 0xFFFFFFFFu ? -1ll : (long long)m;
+        r.index[g] = (unsigned long long)out;
+    }
+}
+
+template <typename TC>
+static int riw_launch(const dsx_points_desc *d, const RiwArgs &r, const RiwPlan &P, uint8_t *ws,
+                      hipStream_t st) {
+    int32_t *window = const_cast<int32_t *>(r.window);
+    uint32_t *hist_sel = reinterpret_cast<uint32_t *>(ws + P.o_hist);
+    int s;
+    if (d->window && d->reuse_window) {
+        // the caller's stored window (sanitised by every kernel that reads it)
+        hipLaunchKernelGGL(copy_window_kernel, dim3(1), dim3(64), 0, st,
+                           (const int32_t *)d->window, window);
+    } else {
+        if ((s = fill_async(hist_sel, DSX_FILL_ZERO32, kSelBins, st))) return s;
+        hipLaunchKernelGGL((select_hist_kernel<TC>), dim3(64), dim3(1024), 0, st, r.a, P.bshift,
+                           P.cw, P.ch, hist_sel);
+        hipLaunchKernelGGL(select_pick_kernel, dim3(1), dim3(1024), 0, st,
+                           (const uint32_t *)hist_sel, P.bshift, P.cw, P.ch, r.a.W, r.a.H,
+                           (int64_t)(P.cap / (uint32_t)r.a.cat.ncat), window);
+        if (d->window)
+            hipLaunchKernelGGL(copy_window_kernel, dim3(1), dim3(64), 0, st,
+                               (const int32_t *)window, d->window);
+    }
+    if ((s = launch_status())) return s;
+    auto kern = d->val_dtype == DSX_F32 ? rowindex_window_kernel<TC, float>
+                                        : rowindex_window_kernel<TC, double>;
+    const hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds);
+    if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
+    hipLaunchKernelGGL(kern, dim3(P.groups), dim3(kRiwThreads), P.lds, st, r);
+    if ((s = launch_status())) return s;
+    hipLaunchKernelGGL(rowindex_window_merge, dim3(grid_for(P.cap, 256, 4)), dim3(256), 0, st, r,
+                       P.groups);
+    return launch_status();
+}
+
+extern "C" size_t dsx_points_rowindex_workspace_size(const dsx_points_desc *d, const dsx_view *v,
+                                                     const dsx_rowindex_desc *r) {
+    if (check_points(d, v) || !r) return 0;
+    RiwPlan P;
+    if (riw_plan(d, v, P)) return 0;
+    return P.total;
+}
+
 extern "C" int dsx_points_rowindex(const dsx_points_desc *d, const dsx_view *v,
-                                   const dsx_rowindex_desc *r, void *stream) {
+                                   const dsx_rowindex_desc *r, void *workspace,
+                                   size_t workspace_bytes, void *stream) {
     int st = check_points(d, v);
     if (st) return st;
     if (!r || !r->index) return DSX_ERR_NULL;
@@ -1932,6 +2128,26 @@ extern "C" int dsx_points_rowindex(const dsx_points_desc *d, const dsx_view *v,
     ra.key = r->key;
     ra.index = reinterpret_cast<unsigned long long *>(r->index);
     const int vdt = has_value(d) ? d->val_dtype : DSX_F64;
+    if (!arg && workspace) {
+        // FIRST / LAST with a workspace: the LDS-window pass
+        RiwPlan P;
+        if ((st = riw_plan(d, v, P))) return st;
+        if (workspace_bytes < P.total) return DSX_ERR_WORKSPACE;
+        uint8_t *ws = reinterpret_cast<uint8_t *>(workspace);
+        RiwArgs rw;
+        rw.a = ra.a;
+        rw.last = r->mode == DSX_RI_LAST ? 1 : 0;
+        rw.nan_check = ra.nan_check;
+        rw.index = ra.index;
+        rw.window = reinterpret_cast<const int32_t *>(ws + P.o_window);
+        rw.cap = P.cap;
+        rw.partials = reinterpret_cast<uint32_t *>(ws + P.o_partials);
+        if (!has_value(d)) { rw.a.val = rw.a.x; rw.nan_check = 0; }
+        dsx_points_desc dd = *d;
+        if (!has_value(d)) dd.val_dtype = d->xy_dtype;
+        if (d->xy_dtype == DSX_F32) return riw_launch<float>(&dd, rw, P, ws, s);
+        return riw_launch<double>(&dd, rw, P, ws, s);
+    }
     if (d->xy_dtype == DSX_F32) return rowindex_launch<float>(ra, vdt, s);
     return rowindex_launch<double>(ra, vdt, s);
 }

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define DSX_ABI_VERSION 4
+#define DSX_ABI_VERSION 5
 
 /* ---- status codes --------------------------------------------------------------- */
 typedef enum {
@@ -226,8 +226,15 @@ typedef struct {
     int64_t *index;               /* out, H*W*ncat: the selected row, -1 where none       */
 } dsx_rowindex_desc;
 
+/* FIRST and LAST with a workspace of dsx_points_rowindex_workspace_size bytes take an LDS
+ * pass: the rows in a hot window of the canvas (picked from a sample of the rows) keep
+ * their cells' rows in LDS, the others take the global atomics; with workspace == NULL (and
+ * always for ARGMAX / ARGMIN) one global-atomic pass.  Same results either way. */
+size_t dsx_points_rowindex_workspace_size(const dsx_points_desc *d, const dsx_view *v,
+                                          const dsx_rowindex_desc *r);
 int dsx_points_rowindex(const dsx_points_desc *d, const dsx_view *v,
-                        const dsx_rowindex_desc *r, void *stream);
+                        const dsx_rowindex_desc *r, void *workspace, size_t workspace_bytes,
+                        void *stream);
 
 /* ---- n-value reductions (max_n / min_n / first_n / last_n) ------------------------ *
  * For every cell the rows of the reference's n-slot list (reductions.py:1457-1805,

@@ -124,13 +124,14 @@ def test_graph_replay_equals_direct_launches(resident):
     b.close()
 
 
-@pytest.mark.parametrize("tile_h,part_rows", [(1, 1 << 10), (16, 1 << 12), (128, 1 << 20)])
-def test_results_do_not_depend_on_the_knobs(resident, tile_h, part_rows):
-    """Narrow tiles, many small parts (heavy tiles split and merged) or none."""
+@pytest.mark.parametrize("tile_h,tile_cells,part_rows", [(1, 0, 1 << 10), (16, 0, 1 << 12),
+                                                         (128, 0, 1 << 20), (8, 300, 1 << 14)])
+def test_results_do_not_depend_on_the_knobs(resident, tile_h, tile_cells, part_rows):
+    """Narrow or small tiles, many small parts (heavy tiles split and merged) or none."""
     frame, index = resident
     reqs = _requests(index, 8, seed=25, sizes=((512, 512), (1024, 768)))
     tuning = batching.Tuning(threads=512, per_class={})
-    tuning.DEFAULT = {"tile_h": tile_h, "part_rows": part_rows}
+    tuning.DEFAULT = {"tile_h": tile_h, "tile_cells": tile_cells, "part_rows": part_rows}
     server = batching.BatchServer(index, agg=reqs[0].agg, engines=1, max_batch=8,
                                   tuning=tuning)
     got = server.serve(reqs, realtime=False, keep_results=True)["results"]
@@ -160,7 +161,7 @@ def test_tuner_picks_knobs_that_keep_results(resident):
     frame, index = resident
     sample = _requests(index, 12, seed=26)
     tuner = batching.BatchTuner(index, planes=("count", "countv", "sum", "mean"))
-    tuner.TILE_H, tuner.PART_ROWS = (32, 64), (1 << 15, 1 << 17)
+    tuner.TILE_H, tuner.TILE_DIV, tuner.PART_ROWS = (32, 64), (1, 8), (1 << 15, 1 << 17)
     tuner.THREADS, tuner.WG_PER_CU = (512, 1024), (1,)
     tuning = tuner.tune(sample, max_batch=8)
     assert tuning.per_class and tuning.threads in (512, 1024)

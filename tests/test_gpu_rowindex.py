@@ -129,3 +129,80 @@ def test_n_reductions_vs_oracle(vdt, by, n):
         idx = ri.select_n_closed_form(cell, ncells, n, mode).reshape(shape)
         np.testing.assert_array_equal(got[name].values, idx)
         assert got[name].values.dtype == np.int64
+
+
+@pytest.mark.parametrize("by", [False, True], ids=["plain", "by-cat"])
+@pytest.mark.parametrize("xdt", ["f4", "f8"])
+def test_windowed_first_last_vs_global_pass_and_oracle(xdt, by):
+    """first / last / the row-index selectors on a canvas much larger than the LDS window
+    (1920 x 1080, a hot 0.01-sigma blob plus uniform rows): the LDS-window pass (the
+    default, with a workspace) and the global-atomic pass (no workspace) both equal the
+    oracle's closed form."""
+    import ctypes
+
+    import torch
+
+    from datashader_amd import _abi, backend
+    from datashader_amd import _device as dev
+    from datashader_amd.glyphs import Point
+
+    rng = np.random.default_rng(17)
+    n = 3_000_000
+    hot = rng.random(n) < 0.9
+    x = np.where(hot, rng.normal(0.5, 0.01, n), rng.uniform(0, 1, n)).astype(xdt)
+    y = np.where(hot, rng.normal(0.5, 0.01, n), rng.uniform(0, 1, n)).astype(xdt)
+    v = rng.lognormal(0, 2, n).astype(xdt)
+    v[rng.random(n) < 0.02] = np.nan
+    cat = pd.Categorical(rng.choice(list("pqrs"), n), categories=list("pqrs"))
+    df = pd.DataFrame({"x": x, "y": y, "v": v, "cat": cat})
+    W, H = 1920, 1080
+    xr = (float(np.nanmin(x)), float(np.nanmax(x)))
+    yr = (float(np.nanmin(y)), float(np.nanmax(y)))
+    cvs = dsa.Canvas(W, H, x_range=xr, y_range=yr)
+    src = dsa.DeviceFrame.from_pandas(df)
+    view = G.view_for(W, H, xr, yr)
+    bins = cf.point_bins(x.astype(np.float64), y.astype(np.float64), view, W, H)
+    ncat = 4 if by else 1
+    cell = ri.cells(bins, df["cat"].cat.codes.to_numpy() if by else None, ncat)
+    ncells = W * H * ncat
+    shape = (H, W, ncat) if by else (H, W)
+
+    def wrap(r):
+        return dsa.by("cat", r) if by else r
+
+    got = cvs.points(src, "x", "y", dsa.summary(
+        f=wrap(dsa.first("v")), l=wrap(dsa.last("v")), rmin=wrap(dsa._min_row_index()),
+        rmax=wrap(dsa._max_row_index())))
+    vv = v.astype(np.float64)
+    want_f = ri.selected_rows_closed_form(cell, ncells, "first", vv).reshape(shape)
+    want_l = ri.selected_rows_closed_form(cell, ncells, "last", vv).reshape(shape)
+    G.assert_same(got["f"].values, ri.take(want_f, vv))
+    G.assert_same(got["l"].values, ri.take(want_l, vv))
+    np.testing.assert_array_equal(
+        got["rmin"].values, ri.selected_rows_closed_form(cell, ncells, "first").reshape(shape))
+    np.testing.assert_array_equal(
+        got["rmax"].values, ri.selected_rows_closed_form(cell, ncells, "last").reshape(shape))
+    if by:
+        return
+    # the same selections through the C ABI with and without the workspace
+    lib = _abi.load()
+    p = backend.Pass("v", None)
+    catc, _, _ = backend._categorizer(None, src, None)
+    mview = backend.make_view(cvs, xr, yr)
+    d, keep = backend._pass_desc(src, None, Point("x", "y"), mview, p, src.device, catc)
+    vw = backend._view_struct(mview)
+    for mode, want in ((_abi.RI_FIRST, want_f), (_abi.RI_LAST, want_l)):
+        outs = []
+        for windowed in (True, False):
+            r = _abi.RowIndexDesc()
+            r.mode, r.nan_check = mode, 1
+            idx = torch.empty(ncells, dtype=torch.int64, device="cuda")
+            r.index = dev.ptr(idx)
+            ws = int(lib.dsx_points_rowindex_workspace_size(d, vw, r)) if windowed else 0
+            work = torch.empty(max(ws, 1), dtype=torch.uint8, device="cuda")
+            print(f"rowindex mode {mode} windowed {windowed} ws {ws}", flush=True)
+            _abi.check(lib.dsx_points_rowindex(d, vw, r, dev.ptr(work) if windowed else None, ws,
+                                               dev.stream_handle()), "dsx_points_rowindex")
+            outs.append(idx.cpu().numpy().reshape(shape))
+        np.testing.assert_array_equal(outs[0], want)
+        np.testing.assert_array_equal(outs[1], want)
