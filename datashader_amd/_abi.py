@@ -45,7 +45,8 @@ vp = ctypes.c_void_p
 
 
 class Planes(ctypes.Structure):
-    _fields_ = [("count", vp), ("countv", vp), ("sum", vp), ("min", vp), ("max", vp)]
+    _fields_ = [("count", vp), ("countv", vp), ("sum", vp), ("min", vp), ("max", vp),
+                ("mean", vp)]
 
 
 class View(ctypes.Structure):

@@ -68,6 +68,7 @@ class Pass:
         self.cat = cat
         self.planes = set()
         self.m2 = False
+        self.leaves = []  # the (inner) reductions finalized from this pass
 
     def __repr__(self):
         return f"Pass(val={self.val!r}, cat={self.cat!r}, planes={sorted(self.planes)}, m2={self.m2})"
@@ -99,12 +100,15 @@ def plan(agg):
             p = passes[key] = Pass(inner.column, cat)
         p.planes.update(inner.planes())
         p.m2 |= inner.needs_m2
+        p.leaves.append(inner)
         leaves.append((name, red, key))
     # a pass with no value column rides along with a value pass of the same categorizer
     for key in [k for k in passes if k[0] is None and k in passes]:
         host = next((k for k in passes if k[0] is not None and k[1] == key[1]), None)
         if host is not None:
-            passes[host].planes.update(passes.pop(key).planes)
+            gone = passes.pop(key)
+            passes[host].planes.update(gone.planes)
+            passes[host].leaves.extend(gone.leaves)
             leaves = [(n, r, host if k == key else k) for (n, r, k) in leaves]
     return passes, leaves
 
@@ -265,9 +269,27 @@ def m2_pass(lib, frame, schema, glyph, view, p, ncells, mean, grid=None):
     return m2
 
 
-def run_pass(lib, frame, schema, glyph, view, p, H, W, with_m2=True):
+def _fused_planes(p):
+    """The planes a points pass writes when mean / var / std take the kernel's finalized
+    mean plane: the sum and count(v) planes only where another leaf reads them."""
+    if not any(isinstance(r, (rd.mean, rd.var, rd.std)) for r in p.leaves):
+        return set(p.planes)
+    need_sum = any(type(r) is rd.sum for r in p.leaves)
+    need_countv = p.m2 or any(type(r) is rd.sum or (isinstance(r, (rd.count, rd.any)) and
+                                                    r.column is not None) for r in p.leaves)
+    planes = set(p.planes) | {"mean"}
+    if not need_sum:
+        planes.discard("sum")
+    if not need_countv:
+        planes.discard("countv")
+    return planes
+
+
+def run_pass(lib, frame, schema, glyph, view, p, H, W, with_m2=True, fuse_mean=True):
     """One kernel pass over ``frame``: its planes (and, for var / std, the m2 plane
-    against the pass's own means unless ``with_m2`` is False)."""
+    against the pass's own means unless ``with_m2`` is False).  Points passes with a
+    mean / var / std leaf take the kernel's finalized mean plane (``fuse_mean``) instead
+    of finalizing it from the sum and count(v) planes."""
     t = dev.torch()
     device = frame.device
     catc, catcol, ncat = _categorizer(p.cat, frame, schema)
@@ -280,6 +302,7 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W, with_m2=True):
     points = isinstance(glyph, Point)
     if not points and not isinstance(glyph, _LineLike):
         raise NotImplementedError(f"glyph {type(glyph).__name__} is not supported")
+    want = _fused_planes(p) if (points and fuse_mean) else set(p.planes)
     planes = {}
     ps = _abi.Planes()
     # points write every cell; lines accumulate into planes filled with the initial values
@@ -289,8 +312,9 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W, with_m2=True):
                              ("min", "u32" if f32 else "u64",
                               _abi.FILL_ONES32 if f32 else _abi.FILL_ONES64),
                              ("max", "u32" if f32 else "u64",
-                              _abi.FILL_ZERO32 if f32 else _abi.FILL_ZERO64)):
-        if name in p.planes:
+                              _abi.FILL_ZERO32 if f32 else _abi.FILL_ZERO64),
+                             ("mean", "f64", None)):
+        if name in want:
             planes[name] = _alloc(ncells, kind, device)
             if not points:
                 _fill(lib, planes[name], init)
@@ -309,8 +333,11 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W, with_m2=True):
     else:
         _abi.check(_timed(lambda: lib.dsx_lines_agg(d, vw, ps, stream)), "dsx_lines_agg")
     if p.m2 and with_m2:
-        mean = t.empty(ncells, dtype=t.float64, device=device)
-        _finalize(lib, _abi.FIN_MEAN, keyb, ncells, planes, mean)
+        if "mean" in planes:
+            mean = planes["mean"]
+        else:
+            mean = t.empty(ncells, dtype=t.float64, device=device)
+            _finalize(lib, _abi.FIN_MEAN, keyb, ncells, planes, mean)
         planes["m2"] = m2_pass(lib, frame, schema, glyph, view, p, ncells, mean, grid)
     return PassResult(ncat, keyb, planes)
 
@@ -346,8 +373,10 @@ def finalize_leaf(lib, red, res, H, W):
         out = t.empty(ncells, dtype=t.uint8, device=device)
         _finalize(lib, _abi.FIN_ANY, res.keyb, ncells, res.planes, out, counts=counts)
         return out.view(t.bool).reshape(shape)
-    out = t.empty(ncells, dtype=t.float64, device=device)
     mode = inner.finalize_mode
+    if mode == _abi.FIN_MEAN and "mean" in res.planes:  # finalized by the kernel
+        return res.planes["mean"].reshape(shape)
+    out = t.empty(ncells, dtype=t.float64, device=device)
     key = res.planes.get("min") if mode == _abi.FIN_MIN else res.planes.get("max")
     _finalize(lib, mode, res.keyb, ncells, res.planes, out, key=key)
     return out.reshape(shape)

@@ -99,7 +99,7 @@ class BatchEngine:
         self.tile_cells = int(lib.dsx_batch_tile_cells(c))
         self._ws = t.empty(ws, dtype=t.uint8, device=x.device)
         tb = int(lib.dsx_batch_table_bytes(max_requests))
-        self._table = t.empty(tb, dtype=t.uint8, pin_memory=True)  # device-mapped host memory
+        self._table = t.empty(tb, dtype=t.uint8, pin_memory=True)  # page-locked: copied per batch
         h = ctypes.c_void_p()
         _abi.check(lib.dsx_batch_create(c, dev.ptr(self._ws), ws,
                                         ctypes.c_void_p(self._table.data_ptr()),
@@ -242,7 +242,7 @@ class BatchTuner:
     :class:`Tuning`."""
 
     TILE_H = (16, 64)
-    TILE_DIV = (1, 4, 16)          # tile_cells = LDS tile cells / TILE_DIV
+    TILE_DIV = (1, 4, 16, 64)      # tile_cells = LDS tile cells / TILE_DIV
     PART_ROWS = (1 << 17, 1 << 19, 1 << 21)
     THREADS = (512, 1024)
     WG_PER_CU = (1, 2)

@@ -8,9 +8,10 @@
 // of reductions.py (554-558, 582-584, 958-962) and the sum / mean finalizers (1092-1096,
 // 1294-1297), so each request gets the reference's result for the whole set.
 //
-// One batch is a fixed sequence of four kernels, captured once per engine and replayed:
-//   stage   copies the request table from pinned host memory into the workspace, resets
-//           the work-queue counters and maps every tile to its request
+// One batch is a fixed sequence, captured once per engine and replayed: a copy of the
+// request table from pinned host memory into the workspace, then five kernels:
+//   stage   expands the requests (tile grid of each), resets the work-queue counters and
+//           maps every tile to its request
 //   tiles   one workgroup per canvas tile: the lattice cells whose rows can land in the
 //           tile's pixels (the tile's coordinate range, widened by a rounding margin,
 //           mapped through the index's own cell formula -- monotone, so no cell is
@@ -77,8 +78,8 @@ struct BItem {
 };
 
 struct BArgs {
-    const dsx_batch_req *host;     // device-mapped pinned table (after the header)
-    BHeader *hhdr;
+    const dsx_batch_req *host;     // the workspace's copy of the request table
+    const BHeader *hhdr;           // ... and of its header
     BReq *reqs;
     BTile *tiles;
     int32_t *tile_req;
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(256) void b_stage_kernel(BArgs A) {
     if (blockIdx.x == 0 && threadIdx.x == 4) A.ctr[4] = (uint32_t)nreq;
     if (blockIdx.x == 0 && threadIdx.x == 5) A.ctr[5] = (uint32_t)A.hhdr->ntiles;
     for (int32_t r = blockIdx.x; r < nreq && r < A.max_req; r += gridDim.x) {
-        if (threadIdx.x == 0) sq = A.host[r];  // one read of the host table per request
+        if (threadIdx.x == 0) sq = A.host[r];
         __syncthreads();
         const dsx_batch_req q = sq;
         __syncthreads();
@@ -496,11 +497,18 @@ __global__ __launch_bounds__(NT) void b_agg_kernel(BArgs A) {
         flush();
         __syncthreads();
         if (d.nparts > 1) {
-            // a split tile's part: its region goes to scratch, the merge kernel folds it
+            // a split tile's part: the tile's cells of its region go to scratch (same
+            // layout), the merge kernel folds them
             if (d.region + m.part < (uint32_t)A.max_regions) {
-                uint4 *dst = reinterpret_cast<uint4 *>(A.scratch + (size_t)(d.region + m.part) * L.bytes);
-                const uint4 *src = reinterpret_cast<const uint4 *>(smem);
-                for (uint32_t j = threadIdx.x; j < L.bytes / 16u; j += NT) dst[j] = src[j];
+                uint8_t *dst = A.scratch + (size_t)(d.region + m.part) * L.bytes;
+                const uint32_t used = (uint32_t)(q.tw * q.th);
+                for (uint32_t c = threadIdx.x; c < used; c += NT) {
+                    reinterpret_cast<uint32_t *>(dst + L.o_cnt)[c] = reinterpret_cast<const uint32_t *>(smem + L.o_cnt)[c];
+                    if (L.has_val) {
+                        reinterpret_cast<uint32_t *>(dst + L.o_nan)[c] = reinterpret_cast<const uint32_t *>(smem + L.o_nan)[c];
+                        reinterpret_cast<double *>(dst + L.o_sum)[c] = reinterpret_cast<const double *>(smem + L.o_sum)[c];
+                    }
+                }
             }
         } else {
             b_store_tile(smem, L, q, d.t, NT);
@@ -521,10 +529,30 @@ __global__ __launch_bounds__(kBMergeThreads) void b_merge_kernel(BArgs A) {
         const BReq q = A.reqs[d.req];
         const uint32_t np = d.region + d.nparts <= (uint32_t)A.max_regions ? d.nparts : 0u;
         const uint8_t *parts = A.scratch + (size_t)d.region * L.bytes;
-        for (uint32_t c = threadIdx.x; c < L.cap; c += kBMergeThreads) {
+        const uint32_t used = (uint32_t)(q.tw * q.th);  // the tile's cells of each region
+        constexpr uint32_t kU = 4;                     // parts whose loads are in flight together
+        for (uint32_t c = threadIdx.x; c < used; c += kBMergeThreads) {
             uint32_t n = 0, nn = 0;
             double sum = 0.0;
-            for (uint32_t p = 0; p < np; ++p) {
+            uint32_t p = 0;
+            for (; p + kU <= np; p += kU) {
+                uint32_t cn[kU], cnn[kU];
+                double cs[kU];
+#pragma unroll
+                for (uint32_t u = 0; u < kU; ++u) {
+                    const uint8_t *r = parts + (size_t)(p + u) * L.bytes;
+                    cn[u] = reinterpret_cast<const uint32_t *>(r + L.o_cnt)[c];
+                    cnn[u] = L.has_val ? reinterpret_cast<const uint32_t *>(r + L.o_nan)[c] : 0u;
+                    cs[u] = L.has_val ? reinterpret_cast<const double *>(r + L.o_sum)[c] : 0.0;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kU; ++u) {
+                    n += cn[u];
+                    nn += cnn[u];
+                    sum += cs[u];
+                }
+            }
+            for (; p < np; ++p) {
                 const uint8_t *r = parts + (size_t)p * L.bytes;
                 n += reinterpret_cast<const uint32_t *>(r + L.o_cnt)[c];
                 if (L.has_val) {
@@ -562,7 +590,8 @@ static size_t b_align(size_t x) { return (x + 255) / 256 * 256; }
 
 struct BLayout {
     BLay L;
-    size_t o_reqs, o_tiles, o_treq, o_items, o_split, o_ctr, o_scratch, total;
+    size_t o_table, table_bytes, o_reqs, o_tiles, o_treq, o_items, o_split, o_ctr, o_scratch,
+        total;
     int32_t max_items, max_regions;
 };
 
@@ -585,6 +614,8 @@ static int b_plan(const dsx_batch_config *c, BLayout &P) {
     P.max_regions = c->max_regions;
     P.max_items = c->max_tiles + c->max_regions;
     size_t o = 0;
+    P.table_bytes = (size_t)kBHeader + sizeof(dsx_batch_req) * (size_t)c->max_requests;
+    P.o_table = o; o = b_align(o + P.table_bytes);
     P.o_reqs = o; o = b_align(o + sizeof(BReq) * (size_t)c->max_requests);
     P.o_tiles = o; o = b_align(o + sizeof(BTile) * (size_t)c->max_tiles);
     P.o_treq = o; o = b_align(o + 4 * (size_t)c->max_tiles);
@@ -604,6 +635,8 @@ struct dsx_batch {
     dsx_batch_config cfg;
     BLayout P;
     BArgs A;
+    void *htable;              // the caller's pinned table
+    void *dtable;              // its copy in the workspace
     dsx_batch_req *table;      // host view of the pinned table (after the header)
     BHeader *hdr;
     hipGraph_t graph;
@@ -614,6 +647,11 @@ struct dsx_batch {
 static int b_launch_all(dsx_batch *b, hipStream_t st) {
     const BArgs &A = b->A;
     const dsx_batch_config &c = b->cfg;
+    // the request table: pinned host memory -> workspace (a copy node of the graph), so
+    // that no kernel reads host memory
+    const hipError_t ec = hipMemcpyAsync(b->dtable, b->htable, b->P.table_bytes,
+                                         hipMemcpyHostToDevice, st);
+    if (ec != hipSuccess) { record_hip_error(ec); return DSX_ERR_HIP; }
     hipLaunchKernelGGL(b_stage_kernel, dim3(c.max_requests < 256 ? c.max_requests : 256), dim3(256),
                        0, st, A);
     const int tg = c.max_tiles < 4096 ? c.max_tiles : 4096;
@@ -656,9 +694,8 @@ extern "C" int dsx_batch_create(const dsx_batch_config *c, void *workspace, size
     if (st) return st;
     if (!workspace || workspace_bytes < P.total) return DSX_ERR_WORKSPACE;
     if (!host_table) return DSX_ERR_NULL;
-    void *dtable = nullptr;
-    hipError_t e = hipHostGetDevicePointer(&dtable, host_table, 0);
-    if (e != hipSuccess || !dtable) { record_hip_error(e != hipSuccess ? e : hipErrorInvalidValue); return DSX_ERR_HIP; }
+    uint8_t *dtable = reinterpret_cast<uint8_t *>(workspace) + P.o_table;
+    hipError_t e = hipSuccess;
     dsx_batch *b = new (std::nothrow) dsx_batch();
     if (!b) return DSX_ERR_HIP;
     b->cfg = *c;
@@ -668,8 +705,10 @@ extern "C" int dsx_batch_create(const dsx_batch_config *c, void *workspace, size
     memset(host_table, 0, dsx_batch_table_bytes(c->max_requests));
     uint8_t *ws = reinterpret_cast<uint8_t *>(workspace);
     BArgs &A = b->A;
-    A.hhdr = reinterpret_cast<BHeader *>(dtable);
-    A.host = reinterpret_cast<const dsx_batch_req *>(reinterpret_cast<uint8_t *>(dtable) + kBHeader);
+    b->htable = host_table;
+    b->dtable = dtable;
+    A.hhdr = reinterpret_cast<const BHeader *>(dtable);
+    A.host = reinterpret_cast<const dsx_batch_req *>(dtable + kBHeader);
     A.reqs = reinterpret_cast<BReq *>(ws + P.o_reqs);
     A.tiles = reinterpret_cast<BTile *>(ws + P.o_tiles);
     A.tile_req = reinterpret_cast<int32_t *>(ws + P.o_treq);

@@ -173,21 +173,30 @@ template <> struct KeyOf<8> {
 // Which planes a call produces.
 enum : uint32_t { PL_CNT = 1, PL_CNTV = 2, PL_SUM = 4, PL_MIN = 8, PL_MAX = 16 };
 
+// The planes a call accumulates: the requested ones, plus sum and count(v) behind a
+// finalized mean.
 inline uint32_t plane_flags(const dsx_planes *p) {
     return (p->count ? PL_CNT : 0) | (p->countv ? PL_CNTV : 0) | (p->sum ? PL_SUM : 0) |
-           (p->min ? PL_MIN : 0) | (p->max ? PL_MAX : 0);
+           (p->min ? PL_MIN : 0) | (p->max ? PL_MAX : 0) | (p->mean ? (PL_SUM | PL_CNTV) : 0);
 }
 
 struct Planes {
     uint32_t *count, *countv;
     double *sum;
     void *min, *max;
+    double *mean;
 };
 
 inline Planes planes_of(const dsx_planes *a) {
     Planes p;
     p.count = a->count; p.countv = a->countv; p.sum = a->sum; p.min = a->min; p.max = a->max;
+    p.mean = a->mean;
     return p;
+}
+
+// mean's finalize (reductions.py:1294-1297): NaN where the cell has no value
+__device__ __forceinline__ double mean_of(double sum, uint32_t nv) {
+    return nv > 0u ? sum / (double)nv : (double)NAN;
 }
 
 // Global-atomic append of one row (or line pixel) at `cell` (used by the line kernel,

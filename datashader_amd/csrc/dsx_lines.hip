@@ -377,6 +377,7 @@ extern "C" int dsx_lines_agg(const dsx_lines_desc *d, const dsx_view *v, const d
     if (d->val && d->val_dtype != DSX_F32 && d->val_dtype != DSX_F64) return DSX_ERR_DTYPE;
     const int cst = check_cat(d->cat);
     if (cst) return cst;
+    if (acc->mean) return DSX_ERR_ACCUM;  // lines accumulate: a mean is finalized afterwards
     const bool needs_val = acc->countv || acc->sum || acc->min || acc->max;
     if (needs_val && !d->val) return DSX_ERR_ACCUM;
     if (d->mean && (!d->val || !acc->sum || acc->count || acc->countv || acc->min || acc->max))

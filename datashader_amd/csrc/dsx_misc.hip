@@ -336,6 +336,7 @@ extern "C" int dsx_combine(const dsx_combine_desc *c, void *stream) {
     if (!c) return DSX_ERR_NULL;
     if (c->ncells < 0) return DSX_ERR_SHAPE;
     if (c->key_bytes != 4 && c->key_bytes != 8) return DSX_ERR_DTYPE;
+    if (c->into.mean || c->part.mean) return DSX_ERR_ACCUM;  // finalized means do not combine
     // the same planes must be present in `into` and `part`
     if ((c->into.count == nullptr) != (c->part.count == nullptr) ||
         (c->into.countv == nullptr) != (c->part.countv == nullptr) ||

@@ -24,6 +24,7 @@
 // accumulated in float64 (LDS ds_add_f64), so only their last-bit rounding depends on
 // the order of the LDS atomics.
 #include <math.h>
+#include <string.h>
 
 #include "dsx_project.h"
 
@@ -127,17 +128,23 @@ template <int KEYB>
 __device__ __forceinline__ void region_store(const uint8_t *r, const Lay &L, uint32_t c,
                                              const Planes &out, int64_t g) {
     using K = KeyOf<KEYB>;
-    if (L.flags & PL_SUM) out.sum[g] = reinterpret_cast<const double *>(r + L.o_sum)[c];
     if (L.flags & PL_MIN)
         reinterpret_cast<typename K::T *>(out.min)[g] = reinterpret_cast<const typename K::T *>(r + L.o_min)[c];
     if (L.flags & PL_MAX)
         reinterpret_cast<typename K::T *>(out.max)[g] = reinterpret_cast<const typename K::T *>(r + L.o_max)[c];
+    uint32_t nv = 0u;  // count(v), when kept
     if (L.flags & PL_CNT) {
         const uint32_t n = reinterpret_cast<const uint32_t *>(r + L.o_cnt)[c];
         if (out.count) out.count[g] = n;
-        if (L.flags & PL_NAN) out.countv[g] = n - reinterpret_cast<const uint32_t *>(r + L.o_nan)[c];
+        if (L.flags & PL_NAN) nv = n - reinterpret_cast<const uint32_t *>(r + L.o_nan)[c];
     }
-    if (L.flags & PL_CNTV) out.countv[g] = reinterpret_cast<const uint32_t *>(r + L.o_cntv)[c];
+    if (L.flags & PL_CNTV) nv = reinterpret_cast<const uint32_t *>(r + L.o_cntv)[c];
+    if (out.countv) out.countv[g] = nv;
+    if (L.flags & PL_SUM) {
+        const double sum = reinterpret_cast<const double *>(r + L.o_sum)[c];
+        if (out.sum) out.sum[g] = sum;
+        if (out.mean) out.mean[g] = mean_of(sum, nv);
+    }
 }
 
 // Add LDS region cell c (a tile with a NaN plane) to the output planes at global cell g,
@@ -152,8 +159,10 @@ __device__ __forceinline__ void region_add_to(const uint8_t *r, const Lay &L, ui
     if (n == 0) return;
     const uint32_t nn = reinterpret_cast<const uint32_t *>(r + L.o_nan)[c];
     if (out.count) out.count[g] += n;
-    out.countv[g] += n - nn;
-    if (L.flags & PL_SUM) out.sum[g] += reinterpret_cast<const double *>(r + L.o_sum)[c];
+    // (the rows a tile holds for a window cell are rows with a NaN value -- rows with a
+    // value stay in the window --, so count(v), the sum and a finalized mean are final)
+    if (out.countv) out.countv[g] += n - nn;
+    if ((L.flags & PL_SUM) && out.sum) out.sum[g] += reinterpret_cast<const double *>(r + L.o_sum)[c];
     if (L.flags & PL_MIN) {
         T &m = reinterpret_cast<T *>(out.min)[g];
         const T x = reinterpret_cast<const T *>(r + L.o_min)[c];
@@ -637,18 +646,22 @@ __global__ __launch_bounds__(kMergeCells * kMergeWays) void merge_window_kernel(
     const uint32_t pix = c / (uint32_t)ncat, cat = c - pix * (uint32_t)ncat;
     const int32_t ly = (int32_t)(pix / (uint32_t)w.w), lx = (int32_t)(pix - (uint32_t)ly * w.w);
     const int64_t g = ((int64_t)(w.y0 + ly) * W + (w.x0 + lx)) * ncat + cat;
-    if (L.flags & PL_SUM) out.sum[g] = s_sum[0][cl];
     if (L.flags & PL_MIN) reinterpret_cast<typename K::T *>(out.min)[g] = s_min[0][cl];
     if (L.flags & PL_MAX) reinterpret_cast<typename K::T *>(out.max)[g] = s_max[0][cl];
+    uint32_t nv = 0u;
     if (L.flags & PL_CNT) {
         if (out.count) out.count[g] = s_cnt[0][cl];
-        if (L.flags & PL_NAN) out.countv[g] = s_cnt[0][cl] - s_nan[0][cl];
         // count() with count(v): the window holds the rows with a value only (rows with
-        // a NaN value are spilled), so its count is also its countv; the tile adds the
+        // a NaN value are spilled), so its count is also its count(v); the tile adds the
         // spilled rows of the cell afterwards
-        else if (out.countv) out.countv[g] = s_cnt[0][cl];
+        nv = (L.flags & PL_NAN) ? s_cnt[0][cl] - s_nan[0][cl] : s_cnt[0][cl];
     }
-    if (L.flags & PL_CNTV) out.countv[g] = s_cntv[0][cl];
+    if (L.flags & PL_CNTV) nv = s_cntv[0][cl];
+    if (out.countv) out.countv[g] = nv;
+    if (L.flags & PL_SUM) {
+        if (out.sum) out.sum[g] = s_sum[0][cl];
+        if (out.mean) out.mean[g] = mean_of(s_sum[0][cl], nv);
+    }
 }
 
 // ---- partition the spill list by tile --------------------------------------------------------
@@ -1197,7 +1210,11 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
         // more cells than kMaxTiles tiles of LDS hold (e.g. 8192 x 8192 count + mean):
         // one pass of global atomics instead, slower but exact
         P.global = true;
+        // a finalized mean accumulates its sum and count(v) in the workspace unless the
+        // caller asked for those planes too
         P.total = 256;
+        if (pl->mean && !pl->sum) P.total += align_up(8 * (size_t)P.ncells, 256);
+        if (pl->mean && !pl->countv) P.total += align_up(4 * (size_t)P.ncells, 256);
         return DSX_OK;
     }
     if (tcap % kTileThreads) return DSX_ERR_SHAPE;  // never: tiles hold >= 1024 cells
@@ -1416,11 +1433,23 @@ __global__ __launch_bounds__(256) void points_global_kernel(PointArgs a, Planes 
 }
 
 template <typename TC>
-static int run_global(const dsx_points_desc *d, const PointArgs &a, const dsx_planes *pl,
-                      const Planes &out, const double *mean, hipStream_t s) {
+static int run_global(const dsx_points_desc *d, const PointArgs &a, const dsx_planes *pl_in,
+                      const Planes &out_in, const double *mean, uint8_t *ws, hipStream_t s) {
     const int64_t ncells = (int64_t)a.W * a.H * a.cat.ncat;
     const bool f32 = has_value(d) && d->val_dtype == DSX_F32;
     int st;
+    // a finalized mean: its sum and count(v) planes (the caller's, or the workspace's)
+    dsx_planes plv = *pl_in;
+    Planes out = out_in;
+    if (pl_in->mean) {
+        size_t o = 256;
+        if (!plv.sum) { plv.sum = reinterpret_cast<double *>(ws + o); o += align_up(8 * (size_t)ncells, 256); }
+        if (!plv.countv) plv.countv = reinterpret_cast<uint32_t *>(ws + o);
+        out.sum = plv.sum;
+        out.countv = plv.countv;
+        out.mean = nullptr;  // finalized after the pass
+    }
+    const dsx_planes *pl = &plv;
     if (pl->count && (st = fill_async(pl->count, DSX_FILL_ZERO32, ncells, s))) return st;
     if (pl->countv && (st = fill_async(pl->countv, DSX_FILL_ZERO32, ncells, s))) return st;
     if (pl->sum && (st = fill_async(pl->sum, DSX_FILL_ZERO64, ncells, s))) return st;
@@ -1428,9 +1457,10 @@ static int run_global(const dsx_points_desc *d, const PointArgs &a, const dsx_pl
         return st;
     if (pl->max && (st = fill_async(pl->max, f32 ? DSX_FILL_ZERO32 : DSX_FILL_ZERO64, ncells, s)))
         return st;
-    if (d->n == 0) return DSX_OK;
-    const int grid = grid_for(d->n, 256, 8);
-    if (!has_value(d))
+    const int grid = grid_for(d->n < 1 ? 1 : d->n, 256, 8);
+    if (d->n == 0) {
+        // no rows: the filled planes are the result (a mean is finalized below)
+    } else if (!has_value(d))
         hipLaunchKernelGGL((points_global_kernel<TC, float, false, false>), dim3(grid), dim3(256), 0, s, a, out, mean);
     else if (f32 && mean)
         hipLaunchKernelGGL((points_global_kernel<TC, float, true, true>), dim3(grid), dim3(256), 0, s, a, out, mean);
@@ -1440,7 +1470,18 @@ static int run_global(const dsx_points_desc *d, const PointArgs &a, const dsx_pl
         hipLaunchKernelGGL((points_global_kernel<TC, double, true, true>), dim3(grid), dim3(256), 0, s, a, out, mean);
     else
         hipLaunchKernelGGL((points_global_kernel<TC, double, true, false>), dim3(grid), dim3(256), 0, s, a, out, mean);
-    return launch_status();
+    if ((st = launch_status())) return st;
+    if (pl_in->mean) {
+        dsx_finalize_desc f;
+        memset(&f, 0, sizeof f);
+        f.mode = DSX_FIN_MEAN;
+        f.ncells = ncells;
+        f.countv = plv.countv;
+        f.sum = plv.sum;
+        f.out = pl_in->mean;
+        return dsx_finalize(&f, s);
+    }
+    return DSX_OK;
 }
 
 template <typename TC>
@@ -1739,9 +1780,9 @@ static int idx_check(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     if (g->grid < 1 || g->grid > kIdxMaxRows) return DSX_ERR_SHAPE;
     if (!(g->x1 > g->x0) || !(g->y1 > g->y0)) return DSX_ERR_SHAPE;
     if (!g->cell_start) return DSX_ERR_NULL;
-    const bool needs_val = p->countv || p->sum || p->min || p->max;
+    const bool needs_val = p->countv || p->sum || p->min || p->max || p->mean;
     if (needs_val && !has_value(d)) return DSX_ERR_ACCUM;
-    if (mean && (!has_value(d) || !p->sum || p->count || p->countv || p->min || p->max))
+    if (mean && (!has_value(d) || !p->sum || p->count || p->countv || p->min || p->max || p->mean))
         return DSX_ERR_ACCUM;
     return DSX_OK;
 }
@@ -1796,10 +1837,11 @@ extern "C" int dsx_points_agg(const dsx_points_desc *d, const dsx_view *v, const
     int st = check_points(d, v);
     if (st) return st;
     if (!p) return DSX_ERR_NULL;
-    const bool needs_val = p->countv || p->sum || p->min || p->max;
+    const bool needs_val = p->countv || p->sum || p->min || p->max || p->mean;
     if (needs_val && !has_value(d)) return DSX_ERR_ACCUM;
-    if (mean && (!has_value(d) || !p->sum || p->count || p->countv || p->min || p->max))
+    if (mean && (!has_value(d) || !p->sum || p->count || p->countv || p->min || p->max || p->mean))
         return DSX_ERR_ACCUM;
+    if (p->mean && !has_value(d)) return DSX_ERR_ACCUM;
     Plan P;
     if ((st = make_plan(d, v, p, P))) return st;
     if (!workspace) return DSX_ERR_WORKSPACE;
@@ -1809,8 +1851,8 @@ extern "C" int dsx_points_agg(const dsx_points_desc *d, const dsx_view *v, const
     hipStream_t s = as_stream(stream);
     uint8_t *ws = reinterpret_cast<uint8_t *>(workspace);
     if (P.global) {
-        if (d->xy_dtype == DSX_F32) return run_global<float>(d, a, p, out, mean, s);
-        return run_global<double>(d, a, p, out, mean, s);
+        if (d->xy_dtype == DSX_F32) return run_global<float>(d, a, p, out, mean, ws, s);
+        return run_global<double>(d, a, p, out, mean, ws, s);
     }
     if (d->xy_dtype == DSX_F32) return dispatch_val<float>(d, a, P, out, mean, ws, s);
     return dispatch_val<double>(d, a, P, out, mean, ws, s);

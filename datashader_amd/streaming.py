@@ -320,7 +320,8 @@ def pipeline(source, schema, canvas, glyph, agg, *, antialias=False, to_host=Tru
     try:
         for frame in up.frames(cache):
             for key, p in passes.items():
-                r = backend.run_pass(lib, frame, schema, glyph, view, p, H, W, with_m2=False)
+                r = backend.run_pass(lib, frame, schema, glyph, view, p, H, W, with_m2=False,
+                                     fuse_mean=False)
                 if key in totals:
                     _combine(lib, totals[key], r)
                 else:

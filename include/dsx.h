@@ -78,6 +78,9 @@ typedef struct {
     double *sum;
     void *min;
     void *max;
+    double *mean;     /* f64, already finalized: sum / count(v) of the non-NaN values, NaN
+                         where a cell has none (mean's finalize, reductions.py:1294-1297);
+                         dsx_points_agg / dsx_points_agg_indexed only                        */
 } dsx_planes;
 
 /* ---- view: the extend() arguments vt + bounds (points.py:223, line.py:362) --------- */
@@ -416,9 +419,9 @@ typedef struct {
 
 typedef struct dsx_batch dsx_batch;
 
-/* Device workspace of an engine, the pinned host table it reads requests from
- * (dsx_batch_table_bytes, allocated device-mapped by the caller, e.g. hipHostMalloc), and
- * the cells of one LDS tile. */
+/* Device workspace of an engine, the pinned host table the caller writes requests into
+ * (dsx_batch_table_bytes of page-locked memory, e.g. from hipHostMalloc: every batch
+ * copies it to the workspace asynchronously), and the cells of one LDS tile. */
 size_t dsx_batch_workspace_size(const dsx_batch_config *c);
 size_t dsx_batch_table_bytes(int32_t max_requests);
 int32_t dsx_batch_tile_cells(const dsx_batch_config *c);
