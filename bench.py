@@ -615,6 +615,9 @@ def main():
                     help="serving: sample requests the autotuner times")
     ap.add_argument("--max-batch", type=int, default=32,
                     help="serving: requests per batch")
+    ap.add_argument("--strategy", default="auto", choices=["auto", "window", "tiles"],
+                    help="points: force how a call splits its rows (auto: picked from a "
+                         "sample by the first call)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -636,8 +639,13 @@ def main():
     if args.workload == "rowindex":
         run_rowindex(args, torch)
         return
+    from datashader_amd import _abi
+    backend.FORCE_STRATEGY = {"auto": None, "window": _abi.STRATEGY_WINDOW,
+                              "tiles": _abi.STRATEGY_TILES}[args.strategy]
     _log(f"[bench] building workload {args.workload} (scale {args.scale})")
     step, rows, algo, cfg, cpu, check = build_workload(args.workload, torch, args.scale)
+    if args.strategy != "auto":
+        cfg["strategy"] = args.strategy
     torch.cuda.synchronize()
     for i in range(args.warmup):
         res = step()

@@ -12,7 +12,7 @@ import os
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libdsx.so")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 # dtype codes
 F32, F64, I8, U8, I16, I32, I64 = 1, 2, 3, 4, 5, 6, 7
@@ -60,6 +60,10 @@ class View(ctypes.Structure):
 
 
 CAT_NONE, CAT_CODES, CAT_MODULO, CAT_BINNING = 0, 1, 2, 3
+# points: how a call splits the rows between its passes (dsx_strategy)
+STRATEGY_AUTO, STRATEGY_WINDOW, STRATEGY_TILES = 0, 1, 2
+# words of a window record {x0, y0, w, h, strategy, 0, 0, 0}
+WINDOW_RECORD_WORDS = 8
 
 
 class Categorizer(ctypes.Structure):
@@ -73,7 +77,8 @@ class Categorizer(ctypes.Structure):
 class PointsDesc(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("x", vp), ("y", vp), ("xy_dtype", ctypes.c_int32),
                 ("val_dtype", ctypes.c_int32), ("val", vp), ("cat", Categorizer),
-                ("window", vp), ("reuse_window", ctypes.c_int32)]
+                ("window", vp), ("reuse_window", ctypes.c_int32),
+                ("strategy", ctypes.c_int32)]
 
 
 class LinesDesc(ctypes.Structure):

@@ -155,15 +155,31 @@ class PassResult:
 
 _WINDOW_CACHE_MAX = 64
 
+# How points calls split their rows between passes: None lets each call pick (AUTO, then
+# the strategy the first call on a resident frame picked); _abi.STRATEGY_WINDOW or
+# _abi.STRATEGY_TILES forces one (tests and benchmarks compare them; results agree).
+FORCE_STRATEGY = None
+
+
+class _WindowSlot:
+    """One cached window record: the device int32[8] {x0, y0, w, h, strategy, ...} a
+    points call picks and later calls reuse, and the strategy it picked once read back."""
+    __slots__ = ("rec", "host", "event", "strategy")
+
+    def __init__(self, rec):
+        self.rec, self.host, self.event, self.strategy = rec, None, None, None
+
 
 def _window_slot(frame, key, device):
-    """(device int32[4] window, reuse flag) of the hot-window cache kept on a
-    DeviceFrame.  The window only steers performance (which canvas cells are kept in
-    LDS during the scan); any stored window gives the same result.  A call copies the
-    stored window into its own workspace before any of its kernels reads it, so calls
-    on several streams may share a slot; the slot is marked as used by the current
-    stream, so that evicting it cannot hand its memory to another allocation while a
-    call on that stream still reads or writes it."""
+    """(slot, reuse flag, strategy) of the hot-window cache kept on a DeviceFrame.  The
+    record only steers performance (which canvas cells are kept in LDS during the scan,
+    and whether the rows are bucketed by tile instead); any stored record gives the same
+    result.  A call copies the stored record into its own workspace before any of its
+    kernels reads it, so calls on several streams may share a slot; the slot is marked as
+    used by the current stream, so that evicting it cannot hand its memory to another
+    allocation while a call on that stream still reads or writes it.  The strategy is
+    AUTO until the one the first call picked has been read back (``_window_readback``):
+    later calls then pass it and launch that strategy's kernels only."""
     t = dev.torch()
     cache = frame.__dict__.setdefault("_window_cache", OrderedDict())
     slot = cache.get(key)
@@ -171,12 +187,29 @@ def _window_slot(frame, key, device):
     if slot is None:
         while len(cache) >= _WINDOW_CACHE_MAX:  # oldest first: a server sees many views
             cache.popitem(last=False)
-        slot = cache[key] = t.zeros(4, dtype=t.int32, device=device)
+        slot = cache[key] = _WindowSlot(
+            t.zeros(_abi.WINDOW_RECORD_WORDS, dtype=t.int32, device=device))
         reuse = 0
     else:
         cache.move_to_end(key)
-    slot.record_stream(t.cuda.current_stream(slot.device))
-    return slot, reuse
+        if slot.strategy is None and slot.event is not None and slot.event.query():
+            picked = int(slot.host[4])
+            if picked in (_abi.STRATEGY_WINDOW, _abi.STRATEGY_TILES):
+                slot.strategy = picked
+    slot.rec.record_stream(t.cuda.current_stream(slot.rec.device))
+    return slot, reuse, (_abi.STRATEGY_AUTO if slot.strategy is None else slot.strategy)
+
+
+def _window_readback(slot):
+    """After the call that picked a slot's record: start copying its strategy word to
+    pinned host memory, without waiting (read by a later ``_window_slot``)."""
+    if slot.event is not None:
+        return
+    t = dev.torch()
+    slot.host = t.empty(_abi.WINDOW_RECORD_WORDS, dtype=t.int32, pin_memory=True)
+    slot.host.copy_(slot.rec, non_blocking=True)
+    slot.event = t.cuda.Event()
+    slot.event.record()
 
 
 def _grid_ref(frame, glyph, p, view):
@@ -233,6 +266,7 @@ def _pass_desc(frame, schema, glyph, view, p, device, catc):
         d.xy_dtype = dev.dtype_code(x)
         d.val, d.val_dtype = dev.ptr(val), vdt
         d.cat = catc
+        d.strategy = _abi.STRATEGY_AUTO if FORCE_STRATEGY is None else FORCE_STRATEGY
         return d, (x, y, val)
     lay = glyph.vertex_layout(frame)
     d = _abi.LinesDesc()
@@ -327,9 +361,14 @@ def run_pass(lib, frame, schema, glyph, view, p, H, W, with_m2=True, fuse_mean=T
             key = (glyph.x, glyph.y, p.val, None if p.cat is None else p.cat._key(),
                    tuple(sorted(p.planes)),
                    W, H, tuple(view[k] for k in ("xmin", "xmax", "ymin", "ymax")))
-            win, reuse = _window_slot(frame, key, device)
-            d.window, d.reuse_window = dev.ptr(win), reuse
-        _points_call(lib, d, vw, ps, None, device, stream, grid)
+            slot, reuse, strategy = _window_slot(frame, key, device)
+            d.window, d.reuse_window = dev.ptr(slot.rec), reuse
+            if FORCE_STRATEGY is None:
+                d.strategy = strategy
+            _points_call(lib, d, vw, ps, None, device, stream, grid)
+            _window_readback(slot)
+        else:
+            _points_call(lib, d, vw, ps, None, device, stream, grid)
     else:
         _abi.check(_timed(lambda: lib.dsx_lines_agg(d, vw, ps, stream)), "dsx_lines_agg")
     if p.m2 and with_m2:
@@ -456,8 +495,8 @@ def row_index_leaf(lib, frame, schema, glyph, view, red, passes, results, H, W):
         # points pass (a window of one 32-bit row per cell)
         wkey = ("rowindex", glyph.x, glyph.y, valcol, None if cat is None else cat._key(),
                 W, H, tuple(view[k] for k in ("xmin", "xmax", "ymin", "ymax")))
-        win, reuse = _window_slot(frame, wkey, device)
-        d.window, d.reuse_window = dev.ptr(win), reuse
+        slot, reuse, _ = _window_slot(frame, wkey, device)
+        d.window, d.reuse_window = dev.ptr(slot.rec), reuse
     ws = 0 if key is not None else int(lib.dsx_points_rowindex_workspace_size(d, vw, r))
     work = t.empty(ws, dtype=t.uint8, device=device) if ws else None
     _abi.check(lib.dsx_points_rowindex(d, vw, r, dev.ptr(work), ws, dev.stream_handle()),

@@ -331,13 +331,13 @@ __device__ __forceinline__ void b_region_init(uint8_t *r, const BLay &L, int tid
 __device__ __forceinline__ void b_store_cell(const uint8_t *r, const BLay &L, uint32_t c,
                                              const BReq &q, int64_t g) {
     const uint32_t n = reinterpret_cast<const uint32_t *>(r + L.o_cnt)[c];
-    if (q.count) q.count[g] = n;
+    if (q.count) global_ptr(q.count)[g] = n;
     if (!L.has_val) return;
     const uint32_t nv = n - reinterpret_cast<const uint32_t *>(r + L.o_nan)[c];
     const double s = reinterpret_cast<const double *>(r + L.o_sum)[c];
-    if (q.countv) q.countv[g] = nv;
-    if (q.sum) q.sum[g] = nv > 0u ? s : (double)NAN;
-    if (q.mean) q.mean[g] = nv > 0u ? s / (double)nv : (double)NAN;
+    if (q.countv) global_ptr(q.countv)[g] = nv;
+    if (q.sum) global_ptr(q.sum)[g] = nv > 0u ? s : (double)NAN;
+    if (q.mean) global_ptr(q.mean)[g] = nv > 0u ? s / (double)nv : (double)NAN;
 }
 
 __device__ __forceinline__ void b_store_tile(const uint8_t *r, const BLay &L, const BReq &q,

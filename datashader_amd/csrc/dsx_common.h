@@ -194,6 +194,14 @@ inline Planes planes_of(const dsx_planes *a) {
     return p;
 }
 
+// A device pointer that came from memory (a request table, a descriptor array) as a
+// global-memory pointer: the compiler cannot prove its address space, and a store through
+// a generic pointer is a flat store, which LDS waits after it also wait for.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *global_ptr(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // mean's finalize (reductions.py:1294-1297): NaN where the cell has no value
 __device__ __forceinline__ double mean_of(double sum, uint32_t nv) {
     return nv > 0u ? sum / (double)nv : (double)NAN;

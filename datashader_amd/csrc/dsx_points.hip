@@ -207,13 +207,14 @@ struct Window {
     int32_t x0, y0, w, h;
 };
 
-// The window of a call, sanitised: anything outside the canvas or larger than the LDS
-// capacity (a stale or foreign hint) becomes the empty window.
+// The window of a call's record {x0, y0, w, h, strategy, ...}, sanitised: anything
+// outside the canvas or larger than the LDS capacity (a stale or foreign hint) becomes
+// the empty window, and so does any window under the tile strategy.
 __device__ __forceinline__ Window load_window(const int32_t *p, int32_t W, int32_t H,
                                               int64_t cap_px) {
     Window w{p[0], p[1], p[2], p[3]};
     if (w.x0 < 0 || w.y0 < 0 || w.w < 0 || w.h < 0 || (int64_t)w.x0 + w.w > W ||
-        (int64_t)w.y0 + w.h > H || (int64_t)w.w * w.h > cap_px)
+        (int64_t)w.y0 + w.h > H || (int64_t)w.w * w.h > cap_px || p[4] == DSX_STRATEGY_TILES)
         w = Window{0, 0, 0, 0};
     return w;
 }
@@ -222,14 +223,28 @@ __device__ __forceinline__ bool in_window(const Window &w, int32_t px, int32_t p
     return (uint32_t)(px - w.x0) < (uint32_t)w.w && (uint32_t)(py - w.y0) < (uint32_t)w.h;
 }
 
-__global__ void set_window_kernel(int32_t *w, int32_t x0, int32_t y0, int32_t ww, int32_t wh) {
-    if (threadIdx.x == 0) { w[0] = x0; w[1] = y0; w[2] = ww; w[3] = wh; }
+// A call's window record: {x0, y0, w, h, strategy, 0, 0, 0} (kRecWords int32 words).
+constexpr int kRecWords = 8;
+
+__global__ void set_window_kernel(int32_t *w, int32_t x0, int32_t y0, int32_t ww, int32_t wh,
+                                  int32_t strategy) {
+    if (threadIdx.x < kRecWords) {
+        const int32_t r[4] = {x0, y0, ww, wh};
+        w[threadIdx.x] = threadIdx.x < 4 ? r[threadIdx.x] : (threadIdx.x == 4 ? strategy : 0);
+    }
 }
 
-// Copy a window [x0, y0, w, h] (a call's snapshot of a stored window, or the window a
-// call picked back into the caller's store).
-__global__ void copy_window_kernel(const int32_t *src, int32_t *dst) {
-    if (threadIdx.x < 4) dst[threadIdx.x] = src[threadIdx.x];
+// Copy a window record (a call's snapshot of a stored record, or the record a call picked
+// back into the caller's store); strategy != 0 overrides the copied strategy.
+__global__ void copy_window_kernel(const int32_t *src, int32_t *dst, int32_t strategy) {
+    if (threadIdx.x < kRecWords)
+        dst[threadIdx.x] = threadIdx.x > 4 ? 0 : (threadIdx.x == 4 && strategy != 0) ? strategy
+                                                                               : src[threadIdx.x];
+}
+
+// The tile strategy runs (see "the tile strategy" below); any other word: the window one.
+__device__ __forceinline__ bool tiles_run(const int32_t *rec) {
+    return rec[4] == DSX_STRATEGY_TILES;
 }
 
 // Coarse density histogram of a sample of rows: kSampleRuns runs of kSampleRun
@@ -261,10 +276,12 @@ __global__ __launch_bounds__(1024) void select_hist_kernel(PointArgs a, int32_t 
 
 // Pick the window: for a few aspect ratios, the position (in coarse bins) of the
 // rectangle of at most cap_px pixels covering the most sampled rows.
+// With tiles_ok, a window holding less than 70% of the sampled rows picks the tile
+// strategy instead (no window).
 __global__ __launch_bounds__(1024) void select_pick_kernel(const uint32_t *hist, int32_t bshift,
                                                            int32_t cw, int32_t ch, int32_t W,
                                                            int32_t H, int64_t cap_px,
-                                                           int32_t *win) {
+                                                           int32_t tiles_ok, int32_t *win) {
     // (cw+1) x (ch+1) inclusive prefix sums; cw*ch <= kSelBins, so cw + ch <= kSelBins + 1
     __shared__ uint32_t ps[2 * kSelBins + 2];
     __shared__ unsigned long long best[1024 / 64];
@@ -312,6 +329,14 @@ __global__ __launch_bounds__(1024) void select_pick_kernel(const uint32_t *hist,
     if (threadIdx.x == 0) {
         unsigned long long k = 0ull;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) k = best[w] > k ? best[w] : k;
+        for (int i = 4; i < kRecWords; ++i) win[i] = 0;
+        win[4] = DSX_STRATEGY_WINDOW;
+        const uint32_t sampled = ps[ch * pw + cw];
+        if (tiles_ok && (k >> 32) * 10ull < (unsigned long long)sampled * 7ull) {
+            win[0] = 0; win[1] = 0; win[2] = 0; win[3] = 0;
+            win[4] = DSX_STRATEGY_TILES;
+            return;
+        }
         if (k == 0ull) {  // no shape fits a coarse bin: no window, every row spills
             win[0] = 0; win[1] = 0; win[2] = 0; win[3] = 0;
             return;
@@ -383,7 +408,8 @@ struct ScanArgs {
 
 // How the scan reads a row's category: no categorizer, a one-byte code column (int8 /
 // uint8), or any other categorizer column.
-enum : int { CK_NONE = 0, CK_BYTE = 1, CK_ANY = 2 };
+// CK_CODE8: a one-byte code column read by category codes (no modulo / binning).
+enum : int { CK_NONE = 0, CK_BYTE = 1, CK_ANY = 2, CK_CODE8 = 3 };
 
 // Rows of chunk ck handled by this thread.  Every load is issued unconditionally (a row
 // past the end reads the last row and is dropped when binned), so the loads of a chunk
@@ -419,17 +445,32 @@ struct Rows {
             if constexpr (HAS_VAL) v[it] = stream_load(V + j);
             else v[it] = TV(0);
             // (a byte column's 64-byte wave reads share cache lines between waves: cached)
-            if constexpr (CK == CK_BYTE) c[it] = reinterpret_cast<const uint8_t *>(a.cat.col)[b + j];
+            if constexpr (CK == CK_BYTE || CK == CK_CODE8)
+                c[it] = reinterpret_cast<const uint8_t *>(a.cat.col)[b + j];
             else if constexpr (CK == CK_ANY) c[it] = load_cat_raw(a.cat, b + j);
             else c[it] = 0;
         }
     }
 };
 
+// a * b + c in 32 bits, the multiply kept apart from the add: fused, they become a 64-bit
+// multiply-add whose unused high addend may share a register with a load still in flight,
+// and the wait for that load drains the scan's prefetch.
+__device__ __forceinline__ uint32_t mul_add32(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t m = a * b;
+    asm volatile("" : "+v"(m));
+    return m + c;
+}
+
 // Category of a scanned row from its loaded raw input; -1 when dropped.
 template <int CK>
 __device__ __forceinline__ int32_t scan_category(const Cat &c, int64_t raw) {
     if constexpr (CK == CK_NONE) return 0;
+    if constexpr (CK == CK_CODE8) {  // category_raw's codes branch, in 32 bits
+        int32_t k = c.dtype == DSX_I8 ? (int32_t)(int8_t)(uint8_t)raw : (int32_t)raw;
+        if (k < 0) k += c.ncat;
+        return (uint32_t)k < (uint32_t)c.ncat ? k : -1;
+    }
     if constexpr (CK == CK_BYTE) raw = c.dtype == DSX_I8 ? (int64_t)(int8_t)(uint8_t)raw : raw;
     return (int32_t)category_raw(c, raw);
 }
@@ -439,6 +480,7 @@ __device__ __forceinline__ int32_t scan_category(const Cat &c, int64_t raw) {
 template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV, int CK, bool LIN = false,
           uint32_t WF = 0>
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArgs s) {
+    if (tiles_run(s.window)) return;  // uniform: the tile strategy's scan runs instead
     extern __shared__ __align__(16) uint8_t smem[];
     uint8_t *win = smem;
     uint32_t *thist = reinterpret_cast<uint32_t *>(smem + s.o_thist);  // [ntiles]
@@ -687,6 +729,8 @@ struct PartArgs {
     int32_t max_parts;
     uint32_t *part_len;               // [1]
     uint8_t *scratch;                 // one tile-layout region per part of a split tile
+    const int32_t *rec;               // the call's window record (its strategy)
+    uint32_t min_part_tiles;          // fewest blocks a tile part is given (tile strategy)
 };
 
 // Block-wide exclusive scan of one value per thread (1024 threads).
@@ -779,9 +823,11 @@ __global__ __launch_bounds__(kPartThreads) void spill_tilescan_kernel(PartArgs p
     // evenly over its parts): at most ntiles + 2 * ntiles / 3 parts in all, and a tile
     // heavier than the mean (the edge of a dense cluster just outside the window) is
     // spread over several workgroups
+    // (under the tile strategy the counts are blocks of entries, not entries)
+    const unsigned long long minp = tiles_run(p.rec) ? p.min_part_tiles : (unsigned long long)kMinPart;
     const unsigned long long mean = (carry + p.ntiles - 1) / p.ntiles;
     const unsigned long long e15 = mean + (mean + 1) / 2;
-    const unsigned long long E = e15 > kMinPart ? e15 : (unsigned long long)kMinPart;
+    const unsigned long long E = e15 > minp ? e15 : minp;
     if (threadIdx.x == 0) *p.part_len = (uint32_t)(E < 0xFFFFFFFFull ? E : 0xFFFFFFFFull);
     unsigned long long pcarry = 0;
     for (int t0 = 0; t0 < p.ntiles; t0 += blockDim.x) {
@@ -835,6 +881,7 @@ __global__ __launch_bounds__(kPartThreads) void spill_scatter_kernel(PartArgs p)
     static_assert(kPer * kPartThreads == kSub && kWv % 64 == 0, "round shape");
     static_assert(kTpt * kPartThreads == kMaxTiles, "tile split");
     static_assert(3 * kWv <= kScatterPad, "loads past a sub-segment stay in the slack");
+    if (tiles_run(p.rec)) return;  // uniform: no spill list under the tile strategy
     __shared__ uint32_t cnt[kMaxTiles];  // round counts, then the round start of each tile
     __shared__ uint32_t wb[kMaxTiles];   // sorted position of round slot 0, per tile
     __shared__ uint32_t s_cell[kSub];
@@ -1020,6 +1067,7 @@ __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay 
                                                                int32_t H, int64_t wcap_px,
                                                                int32_t ncat, int64_t ncells,
                                                                const double *mean, Planes out) {
+    if (tiles_run(window)) return;  // uniform: the tile strategy's tile pass runs instead
     extern __shared__ __align__(16) uint8_t smem[];
     using R = TileRows<TV, HAS_VAL>;
     constexpr unsigned long long kStep = (unsigned long long)R::kU * kTileThreads;
@@ -1153,6 +1201,411 @@ __global__ __launch_bounds__(kTileThreads) void tile_merge_kernel(PartArgs p, La
     tile_store<KEYB>(smem, L, p, t, window, W, H, wcap_px, ncat, ncells, out);
 }
 
+// ---- the tile strategy ---------------------------------------------------------------------
+// With rows spread over the canvas (many categories, uniform data) a hot window holds few
+// of them, and the window strategy spills nearly every row: 4 + value bytes written by
+// the scan, read and written again by the scatter, read by the tile pass.  The tile
+// strategy keeps no window.  Its scan gives every tile an LDS bucket of C = 2^cshift
+// entries (a 16-bit tile-local cell offset, and the value) and writes a bucket that fills
+// as one block of its workgroup's own block stream (nothing is reserved across
+// workgroups); rows that arrive at a full bucket go straight into the next blocks of
+// their tile.  A list kernel orders the blocks by tile, and the tile pass aggregates each
+// tile's blocks in LDS as it does the spill list.  A row costs one write and one read of
+// 2 + value bytes.
+constexpr int kBktMinC = 16;              // smallest bucket worth a block
+constexpr int kBktMaxC = 256;
+constexpr uint32_t kBktMaxTileCells = 32768;  // tile offsets < 2^15: 0xFFFF marks unused slots
+constexpr uint16_t kBktUnused = 0xFFFF;   // an unused slot of a partial block
+constexpr int kBktBatch = 1024;           // block ids the tile pass stages in LDS at a time
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
+struct BktArgs {
+    const int32_t *rec;            // the call's window record (its strategy)
+    int32_t nb;                    // buckets (= tiles)
+    uint32_t tile_shift, cshift;   // tile cells = 1 << tile_shift, block entries = 1 << cshift
+    uint32_t o_bval;               // byte offset of a block's values
+    uint32_t blk_bytes;
+    uint32_t bcap;                 // blocks one scan workgroup may write
+    uint8_t *pool;                 // [G * bcap] blocks
+    unsigned long long *hdr;       // [G * bcap] tile | rank << 32 (among its workgroup's
+                                   // blocks of that tile)
+    uint32_t *nblk;                // [G] blocks each scan workgroup wrote
+    uint32_t *hist;                // [G][nb] blocks of each tile
+    uint32_t *list;                // block ids, ordered by tile
+    uint32_t o_cnt, o_lo, o_tblk, o_base, o_bk, o_fl, o_off, o_val, o_ctrl;  // scan LDS offsets
+};
+
+// The tile strategy's scan.  Bucket b's slots are numbered from the start of the scan:
+// its LDS block holds slots [lo[b], lo[b] + C) at LDS position slot % C.  Each row takes
+// the next slot of its tile's bucket (an LDS atomic); a row whose slot is past the LDS
+// block keeps it in registers.  After every kSteps chunks (three for float32 coordinates
+// without a value column, else one: the rows held cost registers) one barrier reads
+// whether any row did, and if so the workgroup flushes: every bucket holding C slots or
+// more past lo writes its LDS block and the full blocks its overflowing rows fill (those
+// rows write theirs directly), lo moves past them, and the rows left over take their LDS
+// positions (the next chunks' rows insert at later slots, whose positions cannot collide
+// with theirs).  A flush is two more barriers and a few LDS round trips during which no
+// row is binned, so they are batched: few checks, every LDS read of a phase issued
+// together, each bucket's block copied by its owner lane, all of them in parallel.  (A
+// barrier-free form, in which the lane completing a block writes it out alone, measured
+// slower: those copies run with one lane of a wave active.)  The buckets left at the end
+// are written as partial blocks (unused slots marked).
+constexpr int kBktOwn = 4;                      // buckets a thread owns (nb <= 4 * 1024)
+static_assert(kBktOwn * kScanThreads >= kMaxTiles, "every bucket has an owner");
+constexpr int kBktSpare = 64;                   // spare bucket words (one per lane)
+
+template <typename TC, typename TV, bool HAS_VAL, int CK, bool LIN>
+__global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArgs s) {
+    if (!tiles_run(s.rec)) return;  // uniform: the window strategy's scan runs instead
+    constexpr int kSteps = (HAS_VAL || sizeof(TC) == 8) ? 1 : 3;  // chunks between checks
+    constexpr int kBktPend = kSteps * kItems;    // rows a thread may hold past their LDS block
+    extern __shared__ __align__(16) uint8_t smem[];
+    // per bucket (+ kBktSpare spare words, which dropped rows and idle lanes address)
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(smem + s.o_cnt);     // slots taken
+    uint32_t *lo = reinterpret_cast<uint32_t *>(smem + s.o_lo);       // first slot in LDS
+    uint32_t *bbase = reinterpret_cast<uint32_t *>(smem + s.o_base);  // first block of a flush
+    uint32_t *bk = reinterpret_cast<uint32_t *>(smem + s.o_bk);       // blocks of that flush
+    uint32_t *tblk = reinterpret_cast<uint32_t *>(smem + s.o_tblk);   // blocks written
+    uint2 *fl = reinterpret_cast<uint2 *>(smem + s.o_fl);             // end: (bucket, block)
+    uint16_t *boff = reinterpret_cast<uint16_t *>(smem + s.o_off);    // [nb][C] offsets
+    TV *bval = reinterpret_cast<TV *>(smem + s.o_val);                // [nb][C] values
+    // [0] blocks written, [1] buckets listed, [2 + k % 3] the flag of check k
+    uint32_t *ctrl = reinterpret_cast<uint32_t *>(smem + s.o_ctrl);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint32_t cs = s.cshift, C = 1u << cs, cmask = C - 1u;
+    const uint32_t omask = (1u << s.tile_shift) - 1u;
+    const uint32_t spare = (uint32_t)s.nb + (uint32_t)lane;
+    const int ncat = a.cat.ncat;
+    for (int b = tid; b < s.nb + kBktSpare; b += kScanThreads) {
+        cnt[b] = 0u; lo[b] = 0u; bbase[b] = 0u; bk[b] = 0u;
+        if (b < s.nb) tblk[b] = 0u;
+    }
+    if (tid < 5) ctrl[tid] = 0u;
+    __syncthreads();
+    const size_t g0 = (size_t)blockIdx.x * s.bcap;  // this workgroup's first block
+    auto blk = [&](uint32_t q) { return s.pool + (g0 + q) * (size_t)s.blk_bytes; };
+    // rows held past their bucket's LDS block: slot - lo, tile << 16 | tile offset, value
+    uint32_t pend[kBktPend], ptag[kBktPend];
+    TV pv[kBktPend];
+#pragma unroll
+    for (int i = 0; i < kBktPend; ++i) { pend[i] = kNoSlot; ptag[i] = 0u; pv[i] = TV(0); }
+    auto lds_at = [&](uint32_t t, uint32_t slot) { return (t << cs) | (slot & cmask); };
+    auto flush = [&]() {  // every thread calls it, after a check's barrier (lo, cnt stable)
+        // A. every bucket holding C slots or more past lo gets k = (count - lo) / C
+        //    consecutive blocks of the stream, its headers, and its owner lane's copy of its
+        //    LDS block, 16 bytes at a time
+        uint32_t newlo[kBktOwn];
+#pragma unroll
+        for (int o = 0; o < kBktOwn; ++o) newlo[o] = kNoSlot;
+#pragma unroll
+        for (int o = 0; o < kBktOwn; ++o) {
+            if (o * kScanThreads >= s.nb) break;  // uniform
+            const int b = tid + o * kScanThreads;
+            uint32_t l = 0u, k = 0u;
+            if (b < s.nb) {
+                l = lo[b];
+                const uint32_t n = cnt[b] - l;
+                k = n >= C ? n >> cs : 0u;
+            }
+            // blocks: the lanes with one block take consecutive ones after one atomic for
+            // the wave (a lane-mask prefix, no LDS round trip); a lane with more takes its own
+            const unsigned long long m1 = __ballot(k == 1u);
+            uint32_t q0 = 0u;
+            if (m1) {  // uniform
+                const int first = __ffsll((long long)m1) - 1;
+                uint32_t b1 = 0u;
+                if (lane == first) b1 = atomicAdd(&ctrl[0], (uint32_t)__popcll(m1));
+                b1 = (uint32_t)__builtin_amdgcn_readlane((int)b1, first);
+                q0 = b1 + __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u));
+            }
+            if (k >= 2u) q0 = atomicAdd(&ctrl[0], k);
+            if (k == 0u) continue;
+            const uint32_t r0 = tblk[b];
+            bbase[b] = q0;
+            bk[b] = k;
+            tblk[b] = r0 + k;
+            newlo[o] = l + (k << cs);
+            for (uint32_t q = 0; q < k; ++q)
+                if (q0 + q < s.bcap)  // always: bcap bounds a workgroup's blocks
+                    s.hdr[g0 + q0 + q] = (unsigned long long)b | ((unsigned long long)(r0 + q) << 32);
+            if (q0 >= s.bcap) continue;
+            uint8_t *dst = blk(q0);
+            const uint4 *so = reinterpret_cast<const uint4 *>(boff + ((uint32_t)b << cs));
+            for (uint32_t w = 0; w < (C * 2u) / 16u; ++w) reinterpret_cast<uint4 *>(dst)[w] = so[w];
+            if constexpr (HAS_VAL) {
+                const uint4 *sv = reinterpret_cast<const uint4 *>(bval + ((uint32_t)b << cs));
+                for (uint32_t w = 0; w < (C * (uint32_t)sizeof(TV)) / 16u; ++w)
+                    reinterpret_cast<uint4 *>(dst + s.o_bval)[w] = sv[w];
+            }
+        }
+        __syncthreads();
+        // B. lo moves past the blocks written; a row held (its slot rel = slot - lo >= C)
+        //    goes to its full block rel / C (straight to the stream) when that is < k, else
+        //    to the front of the bucket's next LDS block (at position rel % C)
+#pragma unroll
+        for (int o = 0; o < kBktOwn; ++o) {
+            const int b = tid + o * kScanThreads;
+            if (b < s.nb && newlo[o] != kNoSlot) lo[b] = newlo[o];
+        }
+        uint32_t bw[kBktPend], kw[kBktPend];
+#pragma unroll
+        for (int i = 0; i < kBktPend; ++i) {
+            const uint32_t idx = pend[i] != kNoSlot ? ptag[i] >> 16 : spare;
+            bw[i] = bbase[idx];
+            kw[i] = bk[idx];
+        }
+        // (two loops: one store site that could write either memory would be a flat store,
+        // and the LDS waits after it would also wait for its write to global memory)
+#pragma unroll
+        for (int i = 0; i < kBktPend; ++i) {
+            if (pend[i] == kNoSlot || (pend[i] >> cs) >= kw[i]) continue;  // (pend >> cs >= 1)
+            const uint32_t qb = bw[i] + (pend[i] >> cs);
+            if (qb < s.bcap) {
+                uint8_t *dst = blk(qb);
+                reinterpret_cast<uint16_t *>(dst)[pend[i] & cmask] = (uint16_t)ptag[i];
+                if constexpr (HAS_VAL) reinterpret_cast<TV *>(dst + s.o_bval)[pend[i] & cmask] = pv[i];
+            }
+            pend[i] = kNoSlot;
+        }
+#pragma unroll
+        for (int i = 0; i < kBktPend; ++i) {
+            if (pend[i] == kNoSlot) continue;
+            const uint32_t at = lds_at(ptag[i] >> 16, pend[i]);
+            boff[at] = (uint16_t)ptag[i];
+            if constexpr (HAS_VAL) bval[at] = pv[i];
+            pend[i] = kNoSlot;
+        }
+        // the next chunks' rows read the new lo, and insert at slots past every one just
+        // placed: their LDS positions cannot collide
+        __syncthreads();
+    };
+    const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
+    using R = Rows<TC, TV, HAS_VAL, CK>;
+    R ra, rb, rc;
+    ra.load(a, blockIdx.x, tid);
+    rb.load(a, (int64_t)blockIdx.x + gridDim.x, tid);
+    int64_t ck = blockIdx.x;
+    uint32_t fk = 0;  // this check's flag: ctrl[2 + fk]
+    bool over = false;
+    // pb: the first of this chunk's kItems entries of pend (a constant at every call)
+    auto step = [&](R &cur, R &far, const int pb) {
+        far.load(a, ck + 2 * (int64_t)gridDim.x, tid);
+        const int64_t rem64 = a.n - ck * kChunk;
+        const uint32_t rem = rem64 >= kChunk ? (uint32_t)kChunk
+                                             : (rem64 > 0 ? (uint32_t)rem64 : 0u);
+        // three passes over the chunk's rows, so that the LDS round trips of all its rows
+        // are in flight together: bin every row (a dropped row takes its lane's spare
+        // bucket, whose slots are never stored), take every slot, then store
+        uint32_t tt[kItems], ss[kItems], ll[kItems];
+        uint16_t oo[kItems];
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            const uint32_t j = (uint32_t)(it * kScanThreads + tid);
+            int32_t px = 0, py = 0, pc = 0;
+            bool keep;
+            if constexpr (LIN) {  // as in scan_kernel
+                const double x = (double)cur.x[it], y = (double)cur.y[it];
+                const bool inr = (a.xmin <= x) & (x <= a.xmax) & (a.ymin <= y) & (y <= a.ymax);
+                const double fx = inr ? __dadd_rn(__dmul_rn(x, a.sx), a.tx) : 0.0;
+                const double fy = inr ? __dadd_rn(__dmul_rn(y, a.sy), a.ty) : 0.0;
+                const int32_t xx = (int32_t)fx, yy = (int32_t)fy;
+                px = xx < a.W - 1 ? xx : a.W - 1;
+                py = yy < a.H - 1 ? yy : a.H - 1;
+                keep = (j < rem) & inr;
+            } else {
+                keep = j < rem && locate_xy<TC>(a, cur.x[it], cur.y[it], px, py);
+            }
+            if constexpr (CK != CK_NONE) {
+                pc = scan_category<CK>(a.cat, cur.c[it]);
+                keep = keep && pc >= 0;
+            }
+            // cells < 2^32 (checked on the host), so 32-bit arithmetic is exact; py and W
+            // are < 2^24, so the pixel is a 24-bit multiply-add
+            const uint32_t pix = __umul24((uint32_t)py, (uint32_t)a.W) + (uint32_t)px;
+            const uint32_t cell = CK == CK_NONE ? pix : mul_add32(pix, (uint32_t)ncat, (uint32_t)pc);
+            tt[it] = keep ? cell >> s.tile_shift : spare;
+            oo[it] = (uint16_t)(cell & omask);
+        }
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            ll[it] = lo[tt[it]];  // current: lo changes only inside a flush
+            ss[it] = atomicAdd(&cnt[tt[it]], 1u);
+        }
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            const uint32_t t = tt[it];
+            if (t >= (uint32_t)s.nb) continue;  // dropped
+            if (ss[it] - ll[it] < C) {
+                const uint32_t at = lds_at(t, ss[it]);
+                boff[at] = oo[it];
+                if constexpr (HAS_VAL) bval[at] = cur.v[it];
+            } else {
+                pend[pb + it] = ss[it] - ll[it];  // relative to lo: >= C
+                ptag[pb + it] = (t << 16) | oo[it];
+                if constexpr (HAS_VAL) pv[pb + it] = cur.v[it];
+                over = true;
+            }
+        }
+        ck += gridDim.x;
+    };
+    auto check = [&]() {
+        if (over) ctrl[2 + fk] = 1u;
+        __syncthreads();
+        const bool any = ctrl[2 + fk] != 0u;
+        // the previous check's flag: every wave read it before this barrier, and the
+        // chunks that raise it next start after the next check's barrier
+        if (tid == 0) ctrl[2 + (fk == 0u ? 2u : fk - 1u)] = 0u;
+        if (any) flush();
+        over = false;
+        fk = fk == 2u ? 0u : fk + 1u;
+    };
+    while (ck < nchunks) {  // three steps per trip, as in scan_kernel
+        step(ra, rc, 0);
+        if constexpr (kSteps == 1) check();
+        step(rb, ra, kSteps == 1 ? 0 : kItems);
+        if constexpr (kSteps == 1) check();
+        step(rc, rb, kSteps == 1 ? 0 : 2 * kItems);
+        check();
+    }
+    __syncthreads();
+    // the buckets left: one partial block each (every bucket holds < C slots past lo)
+    for (int b = tid; b < s.nb; b += kScanThreads) {
+        const uint32_t n = cnt[b] - lo[b];
+        if (n > 0u) {
+            const uint32_t q0 = atomicAdd(&ctrl[0], 1u);
+            if (q0 < s.bcap) s.hdr[g0 + q0] = (unsigned long long)b | ((unsigned long long)tblk[b] << 32);
+            tblk[b] += 1u;
+            fl[atomicAdd(&ctrl[1], 1u)] = make_uint2((uint32_t)b, q0);
+        }
+    }
+    __syncthreads();
+    const uint32_t nfl = ctrl[1];
+    for (uint32_t d = tid; d < (nfl << cs); d += kScanThreads) {  // one entry per thread
+        const uint2 e = fl[d >> cs];
+        const uint32_t w = d & cmask, n = cnt[e.x] - lo[e.x];
+        if (e.y >= s.bcap) continue;
+        uint8_t *dst = blk(e.y);
+        reinterpret_cast<uint16_t *>(dst)[w] = w < n ? boff[(e.x << cs) + w] : kBktUnused;
+        if constexpr (HAS_VAL)
+            if (w < n) reinterpret_cast<TV *>(dst + s.o_bval)[w] = bval[(e.x << cs) + w];
+    }
+    for (int b = tid; b < s.nb; b += kScanThreads) s.hist[(size_t)blockIdx.x * s.nb + b] = tblk[b];
+    if (tid == 0) s.nblk[blockIdx.x] = ctrl[0] < s.bcap ? ctrl[0] : s.bcap;
+}
+
+// Order the blocks by tile: block q of scan workgroup g goes to tile_start[t] + (g's
+// exclusive prefix for t, from spill_colscan_kernel) + its rank.
+__global__ __launch_bounds__(1024) void bkt_list_kernel(BktArgs s, PartArgs p) {
+    if (!tiles_run(s.rec)) return;
+    const uint32_t g = blockIdx.x;
+    const uint32_t n = s.nblk[g];
+    const size_t g0 = (size_t)g * s.bcap;
+    for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
+        const unsigned long long h = s.hdr[g0 + q];
+        const uint32_t t = (uint32_t)h;
+        if (t >= (uint32_t)s.nb) continue;  // never, for blocks the scan wrote
+        const unsigned long long pos = p.tile_start[t] + p.hist[(size_t)g * p.ntiles + t] + (h >> 32);
+        s.list[pos] = (uint32_t)(g0 + q);
+    }
+}
+
+// Entries [j0, j0 + kStep) of a batch of staged blocks, kU per thread; an index past the
+// batch re-reads entry 0 and is dropped.
+template <typename TV, bool HAS_VAL>
+struct BktRows {
+    static constexpr int kU = 8;
+    uint16_t o[kU];
+    TV v[kU];
+
+    __device__ __forceinline__ void load(const BktArgs &s, const uint32_t *sblk, uint32_t j0,
+                                         uint32_t nent) {
+        const uint32_t cmask = (1u << s.cshift) - 1u;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            uint32_t j = j0 + (uint32_t)(u * kTileThreads + threadIdx.x);
+            j = j < nent ? j : 0u;
+            const uint8_t *b = s.pool + (size_t)sblk[j >> s.cshift] * s.blk_bytes;
+            o[u] = reinterpret_cast<const uint16_t *>(b)[j & cmask];
+            if constexpr (HAS_VAL) v[u] = reinterpret_cast<const TV *>(b + s.o_bval)[j & cmask];
+            else v[u] = TV(0);
+        }
+    }
+};
+
+// The tile strategy's tile pass: part `blockIdx.x` of tile t aggregates its share of
+// the tile's blocks in LDS (a split tile leaves its planes to tile_merge_kernel).
+template <typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
+__global__ __launch_bounds__(kTileThreads) void btile_agg_kernel(PartArgs p, BktArgs s, Lay L,
+                                                                const int32_t *window, int32_t W,
+                                                                int32_t H, int64_t wcap_px,
+                                                                int32_t ncat, int64_t ncells,
+                                                                const double *mean, Planes out) {
+    if (!tiles_run(window)) return;  // uniform: the window strategy's tile pass runs instead
+    extern __shared__ __align__(16) uint8_t smem[];
+    uint32_t *sblk = reinterpret_cast<uint32_t *>(smem + L.bytes);  // [kBktBatch]
+    using R = BktRows<TV, HAS_VAL>;
+    constexpr uint32_t kStep = (uint32_t)R::kU * kTileThreads;
+    const uint32_t part = blockIdx.x;
+    if (part >= p.part_start[p.ntiles] || part >= (uint32_t)p.max_parts) return;  // the grid is an upper bound
+    const int t = (int)p.part_tile[part];
+    region_init<KEYB>(smem, L, threadIdx.x, kTileThreads);
+    const uint32_t first = p.part_start[t], nparts = p.part_start[t + 1] - first;
+    const unsigned long long tb = p.tile_start[t], te = p.tile_start[t + 1];
+    const unsigned long long len = (te - tb + nparts - 1) / nparts;  // an even split of blocks
+    unsigned long long b = tb + (unsigned long long)(part - first) * len;
+    unsigned long long e = b + len;
+    b = b < te ? b : te;
+    e = e < te ? e : te;
+    const int64_t c0 = (int64_t)t * p.tile_cells;
+    for (unsigned long long bb = b; bb < e; bb += kBktBatch) {
+        const uint32_t nbb = (uint32_t)(e - bb < (unsigned long long)kBktBatch ? e - bb : kBktBatch);
+        __syncthreads();  // the region is initialised, the previous batch's ids are read
+        for (uint32_t i = threadIdx.x; i < nbb; i += kTileThreads) sblk[i] = s.list[bb + i];
+        __syncthreads();
+        const uint32_t nent = nbb << s.cshift;
+        R ra, rb, rc;  // three register sets rotate roles, as in the scan
+        ra.load(s, sblk, 0u, nent);
+        rb.load(s, sblk, kStep, nent);
+        uint32_t j0 = 0u;
+        auto step = [&](R &cur, R &far) {
+            far.load(s, sblk, j0 + 2u * kStep, nent);
+#pragma unroll
+            for (int u = 0; u < R::kU; ++u) {
+                if (j0 + (uint32_t)(u * kTileThreads + threadIdx.x) >= nent) continue;
+                const uint32_t lc = cur.o[u];
+                if (lc >= L.cap) continue;  // an unused slot of a partial block
+                double v = 0.0;
+                bool vnan = true;
+                if constexpr (HAS_VAL) {
+                    v = (double)cur.v[u];
+                    vnan = v != v;
+                }
+                double add = v;
+                if constexpr (SQDEV) {
+                    const double d = v - mean[c0 + lc];
+                    add = d * d;
+                }
+                region_add<KEYB>(smem, L, lc, vnan, v, add);
+            }
+            j0 += kStep;
+        };
+        while (j0 < nent) {  // steps past nent load entry 0 and add nothing
+            step(ra, rc);
+            step(rb, ra);
+            step(rc, rb);
+        }
+    }
+    __syncthreads();
+    if (nparts > 1) {  // a split tile, as in tile_agg_kernel
+        uint4 *mine = reinterpret_cast<uint4 *>(p.scratch + (size_t)part * L.bytes);
+        const uint4 *src = reinterpret_cast<const uint4 *>(smem);
+        for (uint32_t j = threadIdx.x; j < L.bytes / 16u; j += kTileThreads) mine[j] = src[j];
+        return;
+    }
+    tile_store<KEYB>(smem, L, p, t, window, W, H, wcap_px, ncat, ncells, out);
+}
+
 // ---- host side --------------------------------------------------------------------------------
 struct Plan {
     uint32_t flags;
@@ -1174,6 +1627,13 @@ struct Plan {
     bool global;         // the canvas is too large for the tiles: global-atomic pass
     int bshift, cw, ch;
     int tshift;          // tile_cells = 1 << tshift
+    // the tile strategy (tiles_ok: the tiles' buckets fit the scan's LDS)
+    bool tiles_ok;
+    bool run_window, run_tiles;  // the strategies this call launches (both under AUTO)
+    uint32_t cshift, o_bval, blk_bytes, bcap, min_part_tiles;
+    uint32_t bo_cnt, bo_lo, bo_tblk, bo_base, bo_bk, bo_fl, bo_off, bo_val, bo_ctrl;  // its scan's LDS
+    size_t bscan_lds, btile_lds;
+    size_t o_nblk, o_bhdr, o_blist, o_pool;
 };
 
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -1249,31 +1709,85 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     // scan workgroups: one per CU at most, fewer for small inputs
     const int64_t chunks = (d->n + kChunk - 1) / kChunk;
     P.scan_groups = (int)(chunks < cus ? (chunks < 1 ? 1 : chunks) : cus);
+    const int64_t group_rows = ((chunks + P.scan_groups - 1) / P.scan_groups) * (int64_t)kChunk;
     // each wave's spill sub-segment holds every row that wave could spill
-    P.seg_cap = ((chunks + P.scan_groups - 1) / P.scan_groups) * (int64_t)(kChunk / kWaves);
+    P.seg_cap = group_rows / kWaves;
     if (P.seg_cap < 1) P.seg_cap = 1;
     const size_t nseg = (size_t)P.scan_groups * kWaves * (size_t)P.seg_cap;
-    // workspace
+    // the tile strategy: the largest bucket C (a power of two, kBktMinC..kBktMaxC) whose
+    // nb buckets fit the scan's LDS next to seven words per bucket
+    P.tiles_ok = false;
+    if (tcap <= (int64_t)kBktMaxTileCells) {
+        const size_t vb = (size_t)P.vbytes, nb = (size_t)P.ntiles;
+        for (int cs = 8; (1 << cs) >= kBktMinC && !P.tiles_ok; --cs) {
+            if ((1 << cs) > kBktMaxC) continue;
+            const size_t C = (size_t)1 << cs;
+            size_t q = 0;
+            P.bo_val = (uint32_t)q; q = align_up(q + vb * C * nb, 16);   // values first (8-aligned)
+            P.bo_off = (uint32_t)q; q = align_up(q + 2 * C * nb, 16);
+            P.bo_cnt = (uint32_t)q; q = align_up(q + 4 * (nb + kBktSpare), 16);
+            P.bo_lo = (uint32_t)q; q = align_up(q + 4 * (nb + kBktSpare), 16);
+            P.bo_tblk = (uint32_t)q; q = align_up(q + 4 * nb, 16);
+            P.bo_base = (uint32_t)q; q = align_up(q + 4 * (nb + kBktSpare), 16);
+            P.bo_bk = (uint32_t)q; q = align_up(q + 4 * (nb + kBktSpare), 16);
+            P.bo_fl = (uint32_t)q; q = align_up(q + 8 * nb, 16);
+            P.bo_ctrl = (uint32_t)q; q += 32;
+            const int64_t bcap = (group_rows + (int64_t)C - 1) / (int64_t)C + (int64_t)nb;
+            // (512 bytes left for the kernel's static LDS words)
+            if (q <= (size_t)kLdsBytes - 512 && (int64_t)P.scan_groups * bcap < (1ll << 32)) {
+                P.tiles_ok = true;
+                P.cshift = (uint32_t)cs;
+                P.bscan_lds = q;
+                P.bcap = (uint32_t)bcap;
+                P.o_bval = (uint32_t)align_up(2 * C, 16);
+                P.blk_bytes = (uint32_t)align_up(P.o_bval + vb * C, 16);
+                const uint32_t mp = (uint32_t)(kMinPart >> cs);
+                P.min_part_tiles = mp < 16u ? 16u : mp;
+                P.btile_lds = (size_t)P.tl.bytes + 4 * (size_t)kBktBatch;
+            }
+        }
+    }
+    // which strategies run: the whole canvas in the window needs no choice; a forced
+    // strategy runs alone; AUTO launches both and the kernels of the one the window pick
+    // did not choose return at once
+    const int strat = d->strategy;
+    P.run_tiles = P.tiles_ok && !P.whole && strat != DSX_STRATEGY_WINDOW;
+    P.run_window = !(P.run_tiles && strat == DSX_STRATEGY_TILES);
+    // workspace: the shared part, then each strategy's buffers over the same bytes
     const int64_t n = d->n < 1 ? 1 : d->n;
     size_t o = 0;
-    P.o_window = o; o = align_up(o + 16, 256);
+    P.o_window = o; o = align_up(o + 4 * kRecWords, 256);
     P.o_hist_sel = o; o = align_up(o + 4 * (size_t)kSelBins, 256);
     P.o_count = o; o = align_up(o + 4 * (size_t)P.scan_groups * kWaves, 256);  // sub-segment lengths
-    P.o_partials = o; o = align_up(o + (size_t)P.scan_groups * P.wl.bytes, 256);
-    P.o_spill_cell = o; o = align_up(o + 4 * (nseg + kScatterPad), 256);
-    P.o_spill_val = o; o = align_up(o + (size_t)P.vbytes * (nseg + kScatterPad), 256);
     P.o_hist = o; o = align_up(o + 4 * (size_t)P.scan_groups * P.ntiles, 256);
     P.o_totals = o; o = align_up(o + 8 * (size_t)P.ntiles, 256);
     P.o_tile_start = o; o = align_up(o + 8 * (size_t)(P.ntiles + 1), 256);
-    const size_t nsorted = (size_t)n + 64 * (size_t)P.scan_groups;  // + the scatter's sink lines
-    P.o_sorted_cell = o; o = align_up(o + 4 * nsorted, 256);
-    P.o_sorted_val = o; o = align_up(o + (size_t)P.vbytes * nsorted, 256);
     P.max_parts = 2 * P.ntiles + 1;
     P.o_part_start = o; o = align_up(o + 4 * (size_t)(P.ntiles + 1), 256);
     P.o_part_len = o; o = align_up(o + 4, 256);
     P.o_part_tile = o; o = align_up(o + 4 * (size_t)P.max_parts, 256);
     P.o_scratch = o; o = align_up(o + (size_t)P.max_parts * P.tl.bytes, 256);
-    P.total = o;
+    const size_t shared = o;
+    P.o_partials = o; o = align_up(o + (size_t)P.scan_groups * P.wl.bytes, 256);
+    P.o_spill_cell = o; o = align_up(o + 4 * (nseg + kScatterPad), 256);
+    P.o_spill_val = o; o = align_up(o + (size_t)P.vbytes * (nseg + kScatterPad), 256);
+    const size_t nsorted = (size_t)n + 64 * (size_t)P.scan_groups;  // + the scatter's sink lines
+    P.o_sorted_cell = o; o = align_up(o + 4 * nsorted, 256);
+    P.o_sorted_val = o; o = align_up(o + (size_t)P.vbytes * nsorted, 256);
+    const size_t end_window = o;
+    size_t end_tiles = shared;
+    if (P.tiles_ok) {
+        const size_t nblocks = (size_t)P.scan_groups * P.bcap;
+        o = shared;
+        P.o_nblk = o; o = align_up(o + 4 * (size_t)P.scan_groups, 256);
+        P.o_bhdr = o; o = align_up(o + 8 * nblocks, 256);
+        P.o_blist = o; o = align_up(o + 4 * nblocks, 256);
+        P.o_pool = o; o = align_up(o + (size_t)P.blk_bytes * nblocks, 256);
+        end_tiles = o;
+    }
+    P.total = shared;
+    if (P.run_window && end_window > P.total) P.total = end_window;
+    if (P.run_tiles && end_tiles > P.total) P.total = end_tiles;
     return DSX_OK;
 }
 
@@ -1290,21 +1804,27 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     const int64_t wcap_px = (int64_t)(P.wl.cap / (uint32_t)ncat);
     int s;
     if (P.whole) {
-        hipLaunchKernelGGL(set_window_kernel, dim3(1), dim3(64), 0, st, window, 0, 0, a.W, a.H);
+        hipLaunchKernelGGL(set_window_kernel, dim3(1), dim3(64), 0, st, window, 0, 0, a.W, a.H,
+                           (int32_t)DSX_STRATEGY_WINDOW);
+    } else if (!P.run_window) {  // the tile strategy, forced: no window
+        hipLaunchKernelGGL(set_window_kernel, dim3(1), dim3(64), 0, st, window, 0, 0, 0, 0,
+                           (int32_t)DSX_STRATEGY_TILES);
     } else if (d->window && d->reuse_window) {
-        // the caller's stored window (sanitised by every kernel that reads it)
+        // the caller's stored record (its window sanitised by every kernel that reads it;
+        // its strategy kept only when this call launches both)
         hipLaunchKernelGGL(copy_window_kernel, dim3(1), dim3(64), 0, st,
-                           (const int32_t *)d->window, window);
+                           (const int32_t *)d->window, window,
+                           P.run_tiles ? 0 : (int32_t)DSX_STRATEGY_WINDOW);
     } else {
         if ((s = fill_async(hist_sel, DSX_FILL_ZERO32, kSelBins, st))) return s;
         hipLaunchKernelGGL((select_hist_kernel<TC>), dim3(64), dim3(1024), 0, st, a, P.bshift,
                            P.cw, P.ch, hist_sel);
         hipLaunchKernelGGL(select_pick_kernel, dim3(1), dim3(1024), 0, st, (const uint32_t *)hist_sel,
                            P.bshift, P.cw, P.ch, a.W, a.H, (int64_t)(P.wl.cap / (uint32_t)ncat),
-                           window);
+                           P.run_tiles ? 1 : 0, window);
         if (d->window)
             hipLaunchKernelGGL(copy_window_kernel, dim3(1), dim3(64), 0, st,
-                               (const int32_t *)window, d->window);
+                               (const int32_t *)window, d->window, 0);
     }
     if ((s = launch_status())) return s;
     ScanArgs sa;
@@ -1333,6 +1853,8 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     }
     auto scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_ANY>;
     if (a.cat.kind == DSX_CAT_NONE) scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE>;
+    else if ((a.cat.dtype == DSX_I8 || a.cat.dtype == DSX_U8) && a.cat.kind == DSX_CAT_CODES)
+        scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_CODE8>;
     else if (a.cat.dtype == DSX_I8 || a.cat.dtype == DSX_U8)
         scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_BYTE>;
     // the common plane sets over linear axes, without categorizer: compile-time planes
@@ -1355,15 +1877,59 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
                 scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE, true, PL_CNT>;
         }
     }
-    hipError_t e = hipFuncSetAttribute((const void *)scan,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.scan_lds);
-    if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
-    hipLaunchKernelGGL(scan, dim3(P.scan_groups), dim3(kScanThreads), P.scan_lds, st, sa_a, sa);
-    if ((s = launch_status())) return s;
-    hipLaunchKernelGGL((merge_window_kernel<KEYB>), dim3((P.wl.cap + kMergeCells - 1) / kMergeCells),
-                       dim3(kMergeCells * kMergeWays), 0, st, P.wl, (const uint8_t *)sa.partials,
-                       P.scan_groups, (const int32_t *)window, a.W, a.H, ncat, out);
-    if ((s = launch_status())) return s;
+    hipError_t e;
+    if (P.run_window) {
+        e = hipFuncSetAttribute((const void *)scan, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)P.scan_lds);
+        if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
+        hipLaunchKernelGGL(scan, dim3(P.scan_groups), dim3(kScanThreads), P.scan_lds, st, sa_a, sa);
+        if ((s = launch_status())) return s;
+    }
+    BktArgs ba;
+    if (P.run_tiles) {
+        ba.rec = window;
+        ba.nb = P.ntiles;
+        ba.tile_shift = (uint32_t)P.tshift;
+        ba.cshift = P.cshift;
+        ba.o_bval = P.o_bval;
+        ba.blk_bytes = P.blk_bytes;
+        ba.bcap = P.bcap;
+        ba.pool = ws + P.o_pool;
+        ba.hdr = reinterpret_cast<unsigned long long *>(ws + P.o_bhdr);
+        ba.nblk = reinterpret_cast<uint32_t *>(ws + P.o_nblk);
+        ba.hist = reinterpret_cast<uint32_t *>(ws + P.o_hist);
+        ba.list = reinterpret_cast<uint32_t *>(ws + P.o_blist);
+        ba.o_cnt = P.bo_cnt;
+        ba.o_lo = P.bo_lo;
+        ba.o_tblk = P.bo_tblk;
+        ba.o_base = P.bo_base;
+        ba.o_bk = P.bo_bk;
+        ba.o_fl = P.bo_fl;
+        ba.o_off = P.bo_off;
+        ba.o_val = P.bo_val;
+        ba.o_ctrl = P.bo_ctrl;
+        const bool lin = !a.xlog && !a.ylog && origin;
+        auto bscan = bscan_kernel<TC, TV, HAS_VAL, CK_ANY, false>;
+        if (a.cat.kind == DSX_CAT_NONE)
+            bscan = lin ? bscan_kernel<TC, TV, HAS_VAL, CK_NONE, true>
+                        : bscan_kernel<TC, TV, HAS_VAL, CK_NONE, false>;
+        else if ((a.cat.dtype == DSX_I8 || a.cat.dtype == DSX_U8) && a.cat.kind == DSX_CAT_CODES)
+            bscan = lin ? bscan_kernel<TC, TV, HAS_VAL, CK_CODE8, true>
+                        : bscan_kernel<TC, TV, HAS_VAL, CK_CODE8, false>;
+        else if (a.cat.dtype == DSX_I8 || a.cat.dtype == DSX_U8)
+            bscan = bscan_kernel<TC, TV, HAS_VAL, CK_BYTE, false>;
+        e = hipFuncSetAttribute((const void *)bscan, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)P.bscan_lds);
+        if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
+        hipLaunchKernelGGL(bscan, dim3(P.scan_groups), dim3(kScanThreads), P.bscan_lds, st, sa_a, ba);
+        if ((s = launch_status())) return s;
+    }
+    if (P.run_window) {
+        hipLaunchKernelGGL((merge_window_kernel<KEYB>), dim3((P.wl.cap + kMergeCells - 1) / kMergeCells),
+                           dim3(kMergeCells * kMergeWays), 0, st, P.wl, (const uint8_t *)sa.partials,
+                           P.scan_groups, (const int32_t *)window, a.W, a.H, ncat, out);
+        if ((s = launch_status())) return s;
+    }
     PartArgs pa;
     pa.cell = sa.spill_cell;
     pa.val = sa.spill_val;
@@ -1383,20 +1949,37 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     pa.part_tile = reinterpret_cast<uint32_t *>(ws + P.o_part_tile);
     pa.max_parts = P.max_parts;
     pa.scratch = ws + P.o_scratch;
+    pa.rec = window;
+    pa.min_part_tiles = P.tiles_ok ? P.min_part_tiles : (uint32_t)kMinPart;
     unsigned long long *totals = reinterpret_cast<unsigned long long *>(ws + P.o_totals);
     hipLaunchKernelGGL(spill_colscan_kernel, dim3(P.ntiles), dim3(kPartThreads), 0, st, pa, totals);
     hipLaunchKernelGGL(spill_tilescan_kernel, dim3(1), dim3(kPartThreads), 0, st, pa,
                        (const unsigned long long *)totals);
-    hipLaunchKernelGGL((spill_scatter_kernel<TV, HAS_VAL>), dim3(P.scan_groups),
-                       dim3(kPartThreads), 0, st, pa);
+    if (P.run_window)
+        hipLaunchKernelGGL((spill_scatter_kernel<TV, HAS_VAL>), dim3(P.scan_groups),
+                           dim3(kPartThreads), 0, st, pa);
+    if (P.run_tiles)
+        hipLaunchKernelGGL(bkt_list_kernel, dim3(P.scan_groups), dim3(1024), 0, st, ba, pa);
     if ((s = launch_status())) return s;
-    auto tile = tile_agg_kernel<TV, HAS_VAL, KEYB, SQDEV>;
-    e = hipFuncSetAttribute((const void *)tile, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)P.tile_lds);
-    if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
-    hipLaunchKernelGGL(tile, dim3(P.max_parts), dim3(kTileThreads), P.tile_lds, st, pa, P.tl,
-                       (const int32_t *)window, a.W, a.H, wcap_px, ncat, P.ncells, mean, out);
-    if ((s = launch_status())) return s;
+    if (P.run_window) {
+        auto tile = tile_agg_kernel<TV, HAS_VAL, KEYB, SQDEV>;
+        e = hipFuncSetAttribute((const void *)tile, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)P.tile_lds);
+        if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
+        hipLaunchKernelGGL(tile, dim3(P.max_parts), dim3(kTileThreads), P.tile_lds, st, pa, P.tl,
+                           (const int32_t *)window, a.W, a.H, wcap_px, ncat, P.ncells, mean, out);
+        if ((s = launch_status())) return s;
+    }
+    if (P.run_tiles) {
+        auto btile = btile_agg_kernel<TV, HAS_VAL, KEYB, SQDEV>;
+        e = hipFuncSetAttribute((const void *)btile, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)P.btile_lds);
+        if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
+        hipLaunchKernelGGL(btile, dim3(P.max_parts), dim3(kTileThreads), P.btile_lds, st, pa, ba,
+                           P.tl, (const int32_t *)window, a.W, a.H, wcap_px, ncat, P.ncells, mean,
+                           out);
+        if ((s = launch_status())) return s;
+    }
     auto merge = tile_merge_kernel<KEYB>;
     e = hipFuncSetAttribute((const void *)merge, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)P.tile_lds);
@@ -1921,8 +2504,11 @@ __global__ __launch_bounds__(256) void rowindex_kernel(RiArgs r) {
         }
         long long cur[kRiU];
 #pragma unroll
-        for (int u = 0; u < kRiU; ++u)  // a plain read first: most rows cannot improve the cell
-            cur[u] = ok[u] ? *reinterpret_cast<volatile long long *>(r.index + cell[u]) : 0;
+        // a relaxed device-scope read first (served by L2): most rows cannot improve the cell
+        for (int u = 0; u < kRiU; ++u)
+            cur[u] = ok[u] ? (long long)__hip_atomic_load(r.index + cell[u], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                           : 0;
 #pragma unroll
         for (int u = 0; u < kRiU; ++u) {
             if (!ok[u]) continue;
@@ -2001,7 +2587,7 @@ static int riw_plan(const dsx_points_desc *d, const dsx_view *v, RiwPlan &P) {
     P.cw = (v->width + (1 << P.bshift) - 1) >> P.bshift;
     P.ch = (v->height + (1 << P.bshift) - 1) >> P.bshift;
     size_t o = 0;
-    P.o_window = o; o = align_up(o + 16, 256);
+    P.o_window = o; o = align_up(o + 4 * kRecWords, 256);
     P.o_hist = o; o = align_up(o + 4 * (size_t)kSelBins, 256);
     P.o_partials = o; o = align_up(o + 4 * (size_t)P.cap * (size_t)P.groups, 256);
     P.total = o;
@@ -2060,7 +2646,9 @@ __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r)
                 continue;
             }
             unsigned long long *slot = r.index + ((int64_t)py * a.W + px) * ncat + pc;
-            const long long cur = *reinterpret_cast<volatile long long *>(slot);
+            // (a relaxed device-scope load: served by L2; a volatile one bypasses it)
+            const long long cur = (long long)__hip_atomic_load(slot, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT);
             if (r.last) {
                 if ((long long)row > cur) atomicMax(reinterpret_cast<long long *>(slot), (long long)row);
             } else if ((unsigned long long)row < (unsigned long long)cur) {
@@ -2116,17 +2704,17 @@ static int riw_launch(const dsx_points_desc *d, const RiwArgs &r, const RiwPlan 
     if (d->window && d->reuse_window) {
         // the caller's stored window (sanitised by every kernel that reads it)
         hipLaunchKernelGGL(copy_window_kernel, dim3(1), dim3(64), 0, st,
-                           (const int32_t *)d->window, window);
+                           (const int32_t *)d->window, window, (int32_t)DSX_STRATEGY_WINDOW);
     } else {
         if ((s = fill_async(hist_sel, DSX_FILL_ZERO32, kSelBins, st))) return s;
         hipLaunchKernelGGL((select_hist_kernel<TC>), dim3(64), dim3(1024), 0, st, r.a, P.bshift,
                            P.cw, P.ch, hist_sel);
         hipLaunchKernelGGL(select_pick_kernel, dim3(1), dim3(1024), 0, st,
                            (const uint32_t *)hist_sel, P.bshift, P.cw, P.ch, r.a.W, r.a.H,
-                           (int64_t)(P.cap / (uint32_t)r.a.cat.ncat), window);
+                           (int64_t)(P.cap / (uint32_t)r.a.cat.ncat), 0, window);
         if (d->window)
             hipLaunchKernelGGL(copy_window_kernel, dim3(1), dim3(64), 0, st,
-                               (const int32_t *)window, d->window);
+                               (const int32_t *)window, d->window, 0);
     }
     if ((s = launch_status())) return s;
     auto kern = d->val_dtype == DSX_F32 ? rowindex_window_kernel<TC, float>

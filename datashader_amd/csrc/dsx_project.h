@@ -57,6 +57,7 @@ static int check_points(const dsx_points_desc *d, const dsx_view *v) {
     if (d->n > 0 && (!d->x || !d->y)) return DSX_ERR_NULL;
     if (d->xy_dtype != DSX_F32 && d->xy_dtype != DSX_F64) return DSX_ERR_DTYPE;
     if (d->val && d->val_dtype != DSX_F32 && d->val_dtype != DSX_F64) return DSX_ERR_DTYPE;
+    if (d->strategy < DSX_STRATEGY_AUTO || d->strategy > DSX_STRATEGY_TILES) return DSX_ERR_MODE;
     return check_cat(d->cat);
 }
 

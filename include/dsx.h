@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define DSX_ABI_VERSION 5
+#define DSX_ABI_VERSION 6
 
 /* ---- status codes --------------------------------------------------------------- */
 typedef enum {
@@ -113,7 +113,24 @@ typedef struct {
     double bin0, binsize;
 } dsx_categorizer;
 
-/* ---- points --------------------------------------------------------------------- */
+/* ---- points --------------------------------------------------------------------- *
+ * How a call splits the rows between its passes (a performance choice; results do not
+ * depend on it):
+ *   WINDOW  a hot window of the canvas is aggregated in LDS during the scan; the other
+ *           rows are spilled (cell + value) and aggregated per tile after a scatter by
+ *           tile.  Best when a window holds most rows (clustered data).
+ *   TILES   no window: the scan buckets every row by tile in LDS and writes full
+ *           buckets as blocks of 16-bit tile-local offsets + values; the tile pass reads
+ *           each tile's blocks.  Best when rows spread over the canvas (many categories,
+ *           uniform data): one write and one read of 2 (+ value) bytes per row.
+ *   AUTO    the call picks from the sampled rows: TILES when the best window would hold
+ *           less than 70% of them and the canvas's tiles fit the scan's LDS buckets.
+ * The strategy picked is kept in the window store (window[4]), so a caller can read it
+ * back and pass it explicitly on later calls.  Another value: DSX_ERR_MODE.  A forced
+ * TILES on a canvas whose tiles do not fit the buckets runs WINDOW.                    */
+typedef enum { DSX_STRATEGY_AUTO = 0, DSX_STRATEGY_WINDOW = 1, DSX_STRATEGY_TILES = 2 }
+    dsx_strategy;
+
 typedef struct {
     int64_t n;                    /* rows                                                 */
     const void *x, *y;            /* device, n elements each                              */
@@ -121,14 +138,16 @@ typedef struct {
     int32_t val_dtype;            /* DSX_F32 | DSX_F64, 0 when val == NULL                */
     const void *val;              /* device value column or NULL                          */
     dsx_categorizer cat;
-    /* Hot window (a performance hint; results do not depend on it): a device int32[4]
-     * {x0, y0, w, h} in pixels, or NULL.  reuse_window = 0: the call picks the window
-     * from a sample of the rows and, when window != NULL, stores it there; 1: the call
-     * uses the stored window (an invalid one is treated as empty).  Either way every
-     * kernel of the call reads the call's own copy in its workspace (copied in, or picked
-     * there and copied out), so calls on several streams may share one store.          */
+    /* Hot window (a performance hint; results do not depend on it): a device int32[8]
+     * {x0, y0, w, h, strategy, 0, 0, 0} (pixels; the strategy a call ran), or NULL.
+     * reuse_window = 0: the call picks the window (and, under AUTO, the strategy) from a
+     * sample of the rows and, when window != NULL, stores it there; 1: the call uses the
+     * stored record (an invalid window is treated as empty).  Either way every kernel of
+     * the call reads the call's own copy in its workspace (copied in, or picked there and
+     * copied out), so calls on several streams may share one store.                     */
     int32_t *window;
     int32_t reuse_window;
+    int32_t strategy;             /* dsx_strategy                                         */
 } dsx_points_desc;
 
 /* Device workspace (bytes) dsx_points_agg needs for these inputs and planes. */

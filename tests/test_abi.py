@@ -116,6 +116,18 @@ def test_points_descriptor_validation():
     assert call(d, planes=pl, work=None) == -4                      # workspace missing
     assert call(d, planes=pl, size=16) == -4                        # workspace too small
     assert lib.dsx_points_workspace_size(d, _view(), pl) > 0
+    d.strategy = 3
+    assert call(d, planes=pl) == -7                                 # unknown strategy
+    assert lib.dsx_points_workspace_size(d, _view(), pl) == 0
+    # AUTO needs the workspace of either strategy (their buffers share the bytes)
+    d.n = 50_000_000
+    sizes = {}
+    for s in (_abi.STRATEGY_AUTO, _abi.STRATEGY_WINDOW, _abi.STRATEGY_TILES):
+        d.strategy = s
+        sizes[s] = lib.dsx_points_workspace_size(d, _view(1024, 1024), pl)
+    assert sizes[_abi.STRATEGY_AUTO] == max(sizes[_abi.STRATEGY_WINDOW], sizes[_abi.STRATEGY_TILES])
+    assert sizes[_abi.STRATEGY_TILES] < sizes[_abi.STRATEGY_WINDOW]
+    d.strategy, d.n = _abi.STRATEGY_AUTO, 10
     d.val, d.val_dtype = ctypes.c_void_p(0x3000), _abi.F32
     assert call(d, planes=pl, mean=ctypes.c_void_p(0x5000)) == -6   # m2 mode takes sum only
     d.cat.kind = _abi.CAT_CODES

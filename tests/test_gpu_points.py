@@ -346,10 +346,14 @@ def test_window_cache_reuse_and_stale_hint():
     agg = dsa.summary(n=dsa.count(), m=dsa.mean("v"), hi=dsa.max("v"))
     first = c.points(f, "x", "y", agg)
     (slot,) = f._window_cache.values()
-    assert int(slot[2]) > 0 and int(slot[3]) > 0   # a window was picked and stored
+    rec = slot.rec
+    assert int(rec[2]) > 0 and int(rec[3]) > 0   # a window was picked and stored
+    assert int(rec[4]) == 1                      # ... with the window strategy
     second = c.points(f, "x", "y", agg)
-    for bogus in ([-5, 10**6, 7, 9], [600, 400, 100, 100], [0, 0, 0, 0]):
-        slot.copy_(torch.tensor(bogus, dtype=torch.int32))
+    for bogus in ([-5, 10**6, 7, 9, 1], [600, 400, 100, 100, 2], [0, 0, 0, 0, 99],
+                  [10, 10, 50, 50, 2]):
+        rec.copy_(torch.tensor(bogus + [0, 0, 0], dtype=torch.int32))
+        slot.strategy = slot.event = None  # the stored record's own strategy word runs
         third = c.points(f, "x", "y", agg)
         for k in ("n", "hi"):
             np.testing.assert_array_equal(third[k].values, first[k].values)
