@@ -1231,7 +1231,9 @@ struct BktArgs {
                                    // blocks of that tile)
     uint32_t *nblk;                // [G] blocks each scan workgroup wrote
     uint32_t *hist;                // [G][nb] blocks of each tile
-    uint32_t *list;                // block ids, ordered by tile
+    uint32_t *wl;                  // [G][bcap]: each workgroup's block indices ordered by
+                                   // tile (tile t's run starts at lpre[g][t])
+    uint32_t *lpre;                // [G][nb]
     uint32_t o_cnt, o_lo, o_tblk, o_base, o_bk, o_fl, o_off, o_val, o_ctrl;  // scan LDS offsets
 };
 
@@ -1490,24 +1492,40 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
         if constexpr (HAS_VAL)
             if (w < n) reinterpret_cast<TV *>(dst + s.o_bval)[w] = bval[(e.x << cs) + w];
     }
-    for (int b = tid; b < s.nb; b += kScanThreads) s.hist[(size_t)blockIdx.x * s.nb + b] = tblk[b];
-    if (tid == 0) s.nblk[blockIdx.x] = ctrl[0] < s.bcap ? ctrl[0] : s.bcap;
-}
-
-// Order the blocks by tile: block q of scan workgroup g goes to tile_start[t] + (g's
-// exclusive prefix for t, from spill_colscan_kernel) + its rank.
-__global__ __launch_bounds__(1024) void bkt_list_kernel(BktArgs s, PartArgs p) {
-    if (!tiles_run(s.rec)) return;
-    const uint32_t g = blockIdx.x;
-    const uint32_t n = s.nblk[g];
-    const size_t g0 = (size_t)g * s.bcap;
-    for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
+    // this workgroup's blocks in tile order, for the tile pass: tile t's run starts at the
+    // exclusive prefix of the block counts over the tiles before it, and a block's rank
+    // places it within the run (lpre reuses the flush bookkeeping words, free now)
+    uint32_t *lp = bbase;
+    {
+        constexpr int kPer = kBktOwn;  // tiles per thread: 4 * 1024 >= nb
+        uint32_t v[kPer], sum = 0u;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int b = tid * kPer + i;
+            v[i] = b < s.nb ? tblk[b] : 0u;
+            sum += v[i];
+        }
+        uint32_t total;
+        uint32_t run = block_exscan32(sum, reinterpret_cast<uint32_t *>(bk), total);
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int b = tid * kPer + i;
+            if (b < s.nb) {
+                lp[b] = run;
+                s.lpre[(size_t)blockIdx.x * s.nb + b] = run;
+            }
+            run += v[i];
+        }
+    }
+    __syncthreads();
+    const uint32_t nq = ctrl[0] < s.bcap ? ctrl[0] : s.bcap;
+    for (uint32_t q = tid; q < nq; q += kScanThreads) {
         const unsigned long long h = s.hdr[g0 + q];
         const uint32_t t = (uint32_t)h;
-        if (t >= (uint32_t)s.nb) continue;  // never, for blocks the scan wrote
-        const unsigned long long pos = p.tile_start[t] + p.hist[(size_t)g * p.ntiles + t] + (h >> 32);
-        s.list[pos] = (uint32_t)(g0 + q);
+        if (t < (uint32_t)s.nb) s.wl[g0 + lp[t] + (uint32_t)(h >> 32)] = q;  // (always)
     }
+    for (int b = tid; b < s.nb; b += kScanThreads) s.hist[(size_t)blockIdx.x * s.nb + b] = tblk[b];
+    if (tid == 0) s.nblk[blockIdx.x] = nq;
 }
 
 // Entries [j0, j0 + kStep) of a batch of staged blocks, kU per thread; an index past the
@@ -1544,6 +1562,11 @@ __global__ __launch_bounds__(kTileThreads) void btile_agg_kernel(PartArgs p, Bkt
     if (!tiles_run(window)) return;  // uniform: the window strategy's tile pass runs instead
     extern __shared__ __align__(16) uint8_t smem[];
     uint32_t *sblk = reinterpret_cast<uint32_t *>(smem + L.bytes);  // [kBktBatch]
+    // the tile's blocks, in the order of their scan workgroups: workgroup g's run is
+    // [ex[g], ex[g + 1]) of the tile's blocks (ex from spill_colscan_kernel), found in its
+    // tile-ordered list at lp[g]
+    uint32_t *ex = sblk + kBktBatch;              // [G + 1]
+    uint32_t *lpg = ex + p.ngroups + 1;           // [G]
     using R = BktRows<TV, HAS_VAL>;
     constexpr uint32_t kStep = (uint32_t)R::kU * kTileThreads;
     const uint32_t part = blockIdx.x;
@@ -1558,10 +1581,25 @@ __global__ __launch_bounds__(kTileThreads) void btile_agg_kernel(PartArgs p, Bkt
     b = b < te ? b : te;
     e = e < te ? e : te;
     const int64_t c0 = (int64_t)t * p.tile_cells;
+    const int G = p.ngroups;
+    for (int g = threadIdx.x; g <= G; g += kTileThreads) {
+        ex[g] = g < G ? p.hist[(size_t)g * p.ntiles + t] : (uint32_t)(te - tb);
+        if (g < G) lpg[g] = s.lpre[(size_t)g * s.nb + t];
+    }
     for (unsigned long long bb = b; bb < e; bb += kBktBatch) {
         const uint32_t nbb = (uint32_t)(e - bb < (unsigned long long)kBktBatch ? e - bb : kBktBatch);
-        __syncthreads();  // the region is initialised, the previous batch's ids are read
-        for (uint32_t i = threadIdx.x; i < nbb; i += kTileThreads) sblk[i] = s.list[bb + i];
+        __syncthreads();  // the region and ex / lpg are written, the last batch's ids are read
+        for (uint32_t i = threadIdx.x; i < nbb; i += kTileThreads) {
+            const uint32_t v = (uint32_t)(bb - tb) + i;  // the block's index in the tile
+            int lo_ = 0, hi_ = G;  // the last workgroup whose run starts at or before v
+            while (hi_ - lo_ > 1) {
+                const int mid = (lo_ + hi_) >> 1;
+                if (ex[mid] <= v) lo_ = mid;
+                else hi_ = mid;
+            }
+            const size_t g0 = (size_t)lo_ * s.bcap;
+            sblk[i] = (uint32_t)(g0 + s.wl[g0 + lpg[lo_] + (v - ex[lo_])]);
+        }
         __syncthreads();
         const uint32_t nent = nbb << s.cshift;
         R ra, rb, rc;  // three register sets rotate roles, as in the scan
@@ -1633,7 +1671,7 @@ struct Plan {
     uint32_t cshift, o_bval, blk_bytes, bcap, min_part_tiles;
     uint32_t bo_cnt, bo_lo, bo_tblk, bo_base, bo_bk, bo_fl, bo_off, bo_val, bo_ctrl;  // its scan's LDS
     size_t bscan_lds, btile_lds;
-    size_t o_nblk, o_bhdr, o_blist, o_pool;
+    size_t o_nblk, o_bhdr, o_bwl, o_blpre, o_pool;
 };
 
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -1743,7 +1781,7 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
                 P.blk_bytes = (uint32_t)align_up(P.o_bval + vb * C, 16);
                 const uint32_t mp = (uint32_t)(kMinPart >> cs);
                 P.min_part_tiles = mp < 16u ? 16u : mp;
-                P.btile_lds = (size_t)P.tl.bytes + 4 * (size_t)kBktBatch;
+                P.btile_lds = (size_t)P.tl.bytes + 4 * ((size_t)kBktBatch + 2 * (size_t)P.scan_groups + 1);
             }
         }
     }
@@ -1781,7 +1819,8 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
         o = shared;
         P.o_nblk = o; o = align_up(o + 4 * (size_t)P.scan_groups, 256);
         P.o_bhdr = o; o = align_up(o + 8 * nblocks, 256);
-        P.o_blist = o; o = align_up(o + 4 * nblocks, 256);
+        P.o_bwl = o; o = align_up(o + 4 * nblocks, 256);
+        P.o_blpre = o; o = align_up(o + 4 * (size_t)P.scan_groups * P.ntiles, 256);
         P.o_pool = o; o = align_up(o + (size_t)P.blk_bytes * nblocks, 256);
         end_tiles = o;
     }
@@ -1898,7 +1937,8 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
         ba.hdr = reinterpret_cast<unsigned long long *>(ws + P.o_bhdr);
         ba.nblk = reinterpret_cast<uint32_t *>(ws + P.o_nblk);
         ba.hist = reinterpret_cast<uint32_t *>(ws + P.o_hist);
-        ba.list = reinterpret_cast<uint32_t *>(ws + P.o_blist);
+        ba.wl = reinterpret_cast<uint32_t *>(ws + P.o_bwl);
+        ba.lpre = reinterpret_cast<uint32_t *>(ws + P.o_blpre);
         ba.o_cnt = P.bo_cnt;
         ba.o_lo = P.bo_lo;
         ba.o_tblk = P.bo_tblk;
@@ -1958,8 +1998,6 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     if (P.run_window)
         hipLaunchKernelGGL((spill_scatter_kernel<TV, HAS_VAL>), dim3(P.scan_groups),
                            dim3(kPartThreads), 0, st, pa);
-    if (P.run_tiles)
-        hipLaunchKernelGGL(bkt_list_kernel, dim3(P.scan_groups), dim3(1024), 0, st, ba, pa);
     if ((s = launch_status())) return s;
     if (P.run_window) {
         auto tile = tile_agg_kernel<TV, HAS_VAL, KEYB, SQDEV>;
