@@ -1235,6 +1235,7 @@ struct BktArgs {
                                    // tile (tile t's run starts at lpre[g][t])
     uint32_t *lpre;                // [G][nb]
     uint32_t o_cnt, o_lo, o_tblk, o_base, o_bk, o_fl, o_off, o_val, o_ctrl;  // scan LDS offsets
+    uint32_t loc_cap;              // block indices the bucket area of LDS holds
 };
 
 // The tile strategy's scan.  Bucket b's slots are numbered from the start of the scan:
@@ -1519,10 +1520,21 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     }
     __syncthreads();
     const uint32_t nq = ctrl[0] < s.bcap ? ctrl[0] : s.bcap;
+    // the list is built in the bucket area of LDS (free now) when it fits, then written out
+    // in order: written straight to its scattered positions it moved ~9x its bytes
+    uint32_t *loc = reinterpret_cast<uint32_t *>(smem);
+    const bool in_lds = nq <= s.loc_cap;
     for (uint32_t q = tid; q < nq; q += kScanThreads) {
         const unsigned long long h = s.hdr[g0 + q];
         const uint32_t t = (uint32_t)h;
-        if (t < (uint32_t)s.nb) s.wl[g0 + lp[t] + (uint32_t)(h >> 32)] = q;  // (always)
+        if (t >= (uint32_t)s.nb) continue;  // never, for blocks this workgroup wrote
+        const uint32_t at = lp[t] + (uint32_t)(h >> 32);
+        if (in_lds) loc[at] = q;
+        else s.wl[g0 + at] = q;
+    }
+    if (in_lds) {  // uniform
+        __syncthreads();
+        for (uint32_t i = tid; i < nq; i += kScanThreads) s.wl[g0 + i] = loc[i];
     }
     for (int b = tid; b < s.nb; b += kScanThreads) s.hist[(size_t)blockIdx.x * s.nb + b] = tblk[b];
     if (tid == 0) s.nblk[blockIdx.x] = nq;
@@ -1948,6 +1960,7 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
         ba.o_off = P.bo_off;
         ba.o_val = P.bo_val;
         ba.o_ctrl = P.bo_ctrl;
+        ba.loc_cap = P.bo_cnt / 4u;  // values and offsets come first, from byte 0
         const bool lin = !a.xlog && !a.ylog && origin;
         auto bscan = bscan_kernel<TC, TV, HAS_VAL, CK_ANY, false>;
         if (a.cat.kind == DSX_CAT_NONE)
