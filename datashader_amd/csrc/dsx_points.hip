@@ -2575,9 +2575,101 @@ __global__ __launch_bounds__(256) void rowindex_kernel(RiArgs r) {
     }
 }
 
+// where(max / min): the first row whose value equals its cell's extreme (the key plane of
+// the max / min pass), as one streaming pass like the points scan (chunks of 4 096 rows,
+// three rotating register sets, the linear projection and byte-code categories
+// specialised).  Each kept row reads its cell's key (L2-resident plane); the reads are
+// issued before the loads of the chunk two ahead and waited for after them, so the wait
+// does not also wait for those loads.  A row matching the key takes the global atomic
+// min (rows matching an extreme are few: no skip read).
+template <typename TC, typename TV, int CK, bool LIN>
+__global__ __launch_bounds__(kScanThreads) void rowindex_arg_kernel(RiArgs r) {
+    const PointArgs &a = r.a;
+    const int tid = threadIdx.x;
+    const int ncat = a.cat.ncat;
+    using K = KeyOf<sizeof(TV)>;
+    const auto *const key = global_ptr(reinterpret_cast<const typename K::T *>(r.key));
+    auto *const index = global_ptr(r.index);
+    const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
+    using R = Rows<TC, TV, true, CK>;
+    auto step = [&](int64_t ck, const R &S, R &far) {
+        const int64_t b = ck * kChunk;
+        const int64_t rem64 = a.n - b;
+        const uint32_t rem = rem64 >= kChunk ? (uint32_t)kChunk : (rem64 > 0 ? (uint32_t)rem64 : 0u);
+        int64_t cell[kItems];
+        bool keep[kItems];
+        typename K::T kv[kItems];
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            const uint32_t j = (uint32_t)(it * kScanThreads + tid);
+            int32_t px = 0, py = 0, pc = 0;
+            bool k_;
+            if constexpr (LIN) {  // as in scan_kernel
+                const double x = (double)S.x[it], y = (double)S.y[it];
+                const bool inr = (a.xmin <= x) & (x <= a.xmax) & (a.ymin <= y) & (y <= a.ymax);
+                const double fx = inr ? __dadd_rn(__dmul_rn(x, a.sx), a.tx) : 0.0;
+                const double fy = inr ? __dadd_rn(__dmul_rn(y, a.sy), a.ty) : 0.0;
+                const int32_t xx = (int32_t)fx, yy = (int32_t)fy;
+                px = xx < a.W - 1 ? xx : a.W - 1;
+                py = yy < a.H - 1 ? yy : a.H - 1;
+                k_ = (j < rem) & inr;
+            } else {
+                k_ = j < rem && locate_xy<TC>(a, S.x[it], S.y[it], px, py);
+            }
+            if constexpr (CK != CK_NONE) {
+                pc = scan_category<CK>(a.cat, S.c[it]);
+                k_ = k_ && pc >= 0;
+            }
+            k_ = k_ && !(S.v[it] != S.v[it]);
+            keep[it] = k_;
+            cell[it] = k_ ? ((int64_t)py * a.W + px) * ncat + pc : 0;
+        }
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) kv[it] = keep[it] ? key[cell[it]] : (typename K::T)0;
+        far.load(a, ck + 2 * (int64_t)gridDim.x, tid);
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            if (!keep[it] || K::key((double)S.v[it]) != kv[it]) continue;
+            const unsigned long long row = (unsigned long long)(b + it * kScanThreads + tid);
+            __hip_atomic_fetch_min(index + cell[it], row, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    };
+    R sa, sb, sc;
+    const int64_t G = gridDim.x;
+    sa.load(a, blockIdx.x, tid);
+    sb.load(a, blockIdx.x + G, tid);
+    for (int64_t ck = blockIdx.x; ck < nchunks; ck += 3 * G) {  // steps past the end drop rows
+        step(ck, sa, sc);
+        step(ck + G, sb, sa);
+        step(ck + 2 * G, sc, sb);
+    }
+}
+
 template <typename TC>
 static int rowindex_launch(const RiArgs &r, int val_dtype, hipStream_t st) {
     const bool arg = r.mode == DSX_RI_ARGMAX || r.mode == DSX_RI_ARGMIN;
+    if (arg) {  // the streaming form
+        const PointArgs &a = r.a;
+        const bool lin = !a.xlog && !a.ylog && a.sx > 0 && a.sy > 0 &&
+                         a.tx == -(a.xmin * a.sx) && a.ty == -(a.ymin * a.sy);
+        const bool byte = (a.cat.dtype == DSX_I8 || a.cat.dtype == DSX_U8) &&
+                          a.cat.kind == DSX_CAT_CODES;
+        const int64_t chunks = (a.n + kChunk - 1) / kChunk;
+        const int groups = grid_for(chunks < 1 ? 1 : chunks, 1, 1);  // one per CU at most
+        auto kern = rowindex_arg_kernel<TC, double, CK_ANY, false>;
+        if (val_dtype == DSX_F32) {
+            kern = rowindex_arg_kernel<TC, float, CK_ANY, false>;
+            if (a.cat.kind == DSX_CAT_NONE) kern = lin ? rowindex_arg_kernel<TC, float, CK_NONE, true>
+                                                       : rowindex_arg_kernel<TC, float, CK_NONE, false>;
+            else if (byte) kern = rowindex_arg_kernel<TC, float, CK_CODE8, false>;
+        } else {
+            if (a.cat.kind == DSX_CAT_NONE) kern = lin ? rowindex_arg_kernel<TC, double, CK_NONE, true>
+                                                       : rowindex_arg_kernel<TC, double, CK_NONE, false>;
+            else if (byte) kern = rowindex_arg_kernel<TC, double, CK_CODE8, false>;
+        }
+        hipLaunchKernelGGL(kern, dim3(groups), dim3(kScanThreads), 0, st, r);
+        return launch_status();
+    }
     // four workgroups per CU: fewer rows in flight for the skip test (on 100M rows
     // first 0.97 -> 0.90 ms, where(max) 1.05 -> 1.01 ms against eight; two is slower)
     const int grid = grid_for(r.a.n < 1 ? 1 : r.a.n, 256, 4);
@@ -2645,83 +2737,113 @@ static int riw_plan(const dsx_points_desc *d, const dsx_view *v, RiwPlan &P) {
     return DSX_OK;
 }
 
-template <typename TC, typename TV>
+// NANCHK: rows whose value is NaN are skipped (the value column is read); CK, LIN: as
+// in scan_kernel (the category read with the row's other columns, the branch-free linear
+// projection).
+template <typename TC, typename TV, bool NANCHK, int CK, bool LIN>
 __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r) {
+    static_assert(kRiwThreads == kScanThreads && kRiwItems == kItems, "chunks as the scan's");
     extern __shared__ __align__(16) uint32_t win[];
     const PointArgs &a = r.a;
     const int ncat = a.cat.ncat;
+    const int tid = threadIdx.x;
     const Window w = load_window(r.window, a.W, a.H, (int64_t)(r.cap / (uint32_t)ncat));
     const uint32_t empty = r.last ? 0u : 0xFFFFFFFFu;
-    for (uint32_t c = threadIdx.x; c < r.cap; c += kRiwThreads) win[c] = empty;
+    for (uint32_t c = tid; c < r.cap; c += kRiwThreads) win[c] = empty;
     __syncthreads();
-    const TC *X = reinterpret_cast<const TC *>(a.x);
-    const TC *Y = reinterpret_cast<const TC *>(a.y);
-    const TV *V = reinterpret_cast<const TV *>(a.val);
-    const int64_t nchunks = (a.n + kRiwChunk - 1) / kRiwChunk;
+    const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
     // FIRST visits the chunks in increasing row order and LAST in decreasing order, so
-    // that after the first sweep most rows fail the global skip test
-    struct Set {
-        TC x[kRiwItems], y[kRiwItems];
-        TV v[kRiwItems];
-    };
-    // chunk k of this workgroup's sequence (past the last: re-reads the last row, dropped)
-    auto base_of = [&](int64_t k) {
-        const int64_t ck = r.last ? nchunks - 1 - k : k;
-        return ck >= 0 && ck < nchunks ? ck * kRiwChunk : a.n;
-    };
-    auto load = [&](int64_t k, Set &S) {
-        const int64_t b = base_of(k);
+    // that after the first sweep most rows fail the global skip test; a step past the
+    // last chunk reads chunk nchunks (rows past the end: re-reads the last row, dropped)
+    auto chunk_of = [&](int64_t k) { return k >= nchunks ? nchunks : (r.last ? nchunks - 1 - k : k); };
+    using R = Rows<TC, TV, NANCHK, CK>;
+    auto *const index = global_ptr(r.index);
+    auto *const sindex = global_ptr(reinterpret_cast<long long *>(r.index));
+    // Three register sets rotate roles (binned / in flight / being loaded), as in the
+    // points scan.  A row outside the window reads its cell of the index plane (a relaxed
+    // device-scope load, served by L2) before any atomic: most rows cannot improve the
+    // cell.  Those reads are issued before the loads of the chunk two ahead and waited for
+    // after them, so that wait (vector memory counts complete in order) does not also wait
+    // for those loads.
+    auto step = [&](int64_t k, const R &S, R &far) {
+        const int64_t ck = chunk_of(k);
+        const int64_t b = ck * kChunk;
+        const int64_t rem64 = a.n - b;
+        const uint32_t rem = rem64 >= kChunk ? (uint32_t)kChunk : (rem64 > 0 ? (uint32_t)rem64 : 0u);
+        int64_t cell[kRiwItems];
+        long long cur[kRiwItems];
+        bool out[kRiwItems];
 #pragma unroll
         for (int it = 0; it < kRiwItems; ++it) {
-            int64_t i = b + it * kRiwThreads + threadIdx.x;
-            i = i < a.n ? i : a.n - 1;
-            S.x[it] = stream_load(X + i);
-            S.y[it] = stream_load(Y + i);
-            S.v[it] = r.nan_check ? stream_load(V + i) : TV(0);
-        }
-    };
-    // rows of chunk k (a chunk past the last has base a.n: every row dropped)
-    auto bin = [&](int64_t k, const Set &S) {
-        const int64_t b = base_of(k);
-#pragma unroll
-        for (int it = 0; it < kRiwItems; ++it) {
-            const int64_t row = b + it * kRiwThreads + threadIdx.x;
-            if (row >= a.n) continue;
-            int32_t px, py, pc;
-            if (!locate<TC>(a, row, S.x[it], S.y[it], px, py, pc)) continue;
-            if (r.nan_check && S.v[it] != S.v[it]) continue;
+            const uint32_t j = (uint32_t)(it * kRiwThreads + tid);
+            int32_t px = 0, py = 0, pc = 0;
+            bool keep;
+            if constexpr (LIN) {  // as in scan_kernel
+                const double x = (double)S.x[it], y = (double)S.y[it];
+                const bool inr = (a.xmin <= x) & (x <= a.xmax) & (a.ymin <= y) & (y <= a.ymax);
+                const double fx = inr ? __dadd_rn(__dmul_rn(x, a.sx), a.tx) : 0.0;
+                const double fy = inr ? __dadd_rn(__dmul_rn(y, a.sy), a.ty) : 0.0;
+                const int32_t xx = (int32_t)fx, yy = (int32_t)fy;
+                px = xx < a.W - 1 ? xx : a.W - 1;
+                py = yy < a.H - 1 ? yy : a.H - 1;
+                keep = (j < rem) & inr;
+            } else {
+                keep = j < rem && locate_xy<TC>(a, S.x[it], S.y[it], px, py);
+            }
+            if constexpr (CK != CK_NONE) {
+                pc = scan_category<CK>(a.cat, S.c[it]);
+                keep = keep && pc >= 0;
+            }
+            if constexpr (NANCHK) keep = keep && !(S.v[it] != S.v[it]);
+            out[it] = false;
+            cell[it] = 0;
+            cur[it] = 0;
+            const uint32_t row = (uint32_t)(b + j);  // rows < 2^32 (checked on the host)
+            if (!keep) continue;
             if (in_window(w, px, py)) {
                 const uint32_t lc = (uint32_t)(((py - w.y0) * w.w + (px - w.x0)) * ncat + pc);
-                if (r.last) atomicMax(&win[lc], (uint32_t)row + 1u);
-                else atomicMin(&win[lc], (uint32_t)row);
+                if (r.last) atomicMax(&win[lc], row + 1u);
+                else atomicMin(&win[lc], row);
                 continue;
             }
-            unsigned long long *slot = r.index + ((int64_t)py * a.W + px) * ncat + pc;
-            // (a relaxed device-scope load: served by L2; a volatile one bypasses it)
-            const long long cur = (long long)__hip_atomic_load(slot, __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_AGENT);
+            out[it] = true;
+            cell[it] = ((int64_t)py * a.W + px) * ncat + pc;
+        }
+        // (global-memory pointers: through a generic one these would be flat operations,
+        // whose waits also wait for LDS)
+#pragma unroll
+        for (int it = 0; it < kRiwItems; ++it)
+            if (out[it]) cur[it] = (long long)__hip_atomic_load(index + cell[it], __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_AGENT);
+        far.load(a, chunk_of(k + 2 * (int64_t)gridDim.x), tid);
+#pragma unroll
+        for (int it = 0; it < kRiwItems; ++it) {
+            if (!out[it]) continue;
+            const long long row = (long long)(b + it * kRiwThreads + tid);
             if (r.last) {
-                if ((long long)row > cur) atomicMax(reinterpret_cast<long long *>(slot), (long long)row);
-            } else if ((unsigned long long)row < (unsigned long long)cur) {
-                atomicMin(slot, (unsigned long long)row);  // -1 (empty) is the largest
+                if (row > cur[it])
+                    __hip_atomic_fetch_max(sindex + cell[it], row, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            } else if ((unsigned long long)row < (unsigned long long)cur[it]) {
+                // -1 (empty) is the largest unsigned value
+                __hip_atomic_fetch_min(index + cell[it], (unsigned long long)row, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     };
-    // two register sets alternate: the next chunk's loads are in flight while one bins
-    // (no copy between them: a copy would wait for its loads)
-    Set sa, sb;
+    R sa, sb, sc;
     const int64_t G = gridDim.x;
-    load(blockIdx.x, sa);
-    for (int64_t k = blockIdx.x; k < nchunks; k += 2 * G) {
-        load(k + G, sb);
-        bin(k, sa);
-        load(k + 2 * G, sa);
-        bin(k + G, sb);
+    sa.load(a, chunk_of(blockIdx.x), tid);
+    sb.load(a, chunk_of(blockIdx.x + G), tid);
+    for (int64_t k = blockIdx.x; k < nchunks; k += 3 * G) {  // steps past the end drop rows
+        step(k, sa, sc);
+        step(k + G, sb, sa);
+        step(k + 2 * G, sc, sb);
     }
     __syncthreads();
     uint32_t *mine = r.partials + (size_t)blockIdx.x * r.cap;
     const uint32_t used = (uint32_t)(w.w * w.h * ncat);
-    for (uint32_t c = threadIdx.x; c < used; c += kRiwThreads) mine[c] = win[c];
+    for (uint32_t c = tid; c < used; c += kRiwThreads) mine[c] = win[c];
 }
 
 // The window cells of the index plane: min (FIRST) / max (LAST) over the workgroups.
@@ -2732,10 +2854,16 @@ __global__ __launch_bounds__(256) void rowindex_window_merge(RiwArgs r, int grou
     const uint32_t used = (uint32_t)(w.w * w.h * ncat);
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < used; c += gridDim.x * blockDim.x) {
         uint32_t m = r.last ? 0u : 0xFFFFFFFFu;
-        for (int g = 0; g < groups; ++g) {
-            const uint32_t x = r.partials[(size_t)g * r.cap + c];
-            m = r.last ? (x > m ? x : m) : (x < m ? x : m);
+        auto fold = [&](uint32_t x) { m = r.last ? (x > m ? x : m) : (x < m ? x : m); };
+        int q = 0;
+        for (; q + 8 <= groups; q += 8) {  // eight groups' loads in flight at a time
+            uint32_t x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = r.partials[(size_t)(q + u) * r.cap + c];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) fold(x[u]);
         }
+        for (; q < groups; ++q) fold(r.partials[(size_t)q * r.cap + c]);
         const uint32_t pix = c / (uint32_t)ncat, cat = c - pix * (uint32_t)ncat;
         const int32_t ly = (int32_t)(pix / (uint32_t)w.w), lx = (int32_t)(pix - (uint32_t)ly * w.w);
         const int64_t g = ((int64_t)(w.y0 + ly) * a.W + (w.x0 + lx)) * ncat + cat;
@@ -2768,8 +2896,30 @@ static int riw_launch(const dsx_points_desc *d, const RiwArgs &r, const RiwPlan 
                                (const int32_t *)window, d->window, 0);
     }
     if ((s = launch_status())) return s;
-    auto kern = d->val_dtype == DSX_F32 ? rowindex_window_kernel<TC, float>
-                                        : rowindex_window_kernel<TC, double>;
+    // (the linear fast path needs the reference's own translation, as in the points scan)
+    const PointArgs &a = r.a;
+    const bool lin = !a.xlog && !a.ylog && a.sx > 0 && a.sy > 0 && a.tx == -(a.xmin * a.sx) &&
+                     a.ty == -(a.ymin * a.sy);
+    const bool f32v = d->val_dtype == DSX_F32;
+    const bool nan = r.nan_check != 0;
+    const bool byte = (a.cat.dtype == DSX_I8 || a.cat.dtype == DSX_U8) && a.cat.kind == DSX_CAT_CODES;
+    auto kern = rowindex_window_kernel<TC, double, false, CK_ANY, false>;
+    if (a.cat.kind == DSX_CAT_NONE) {
+        if (lin) kern = nan ? (f32v ? rowindex_window_kernel<TC, float, true, CK_NONE, true>
+                                    : rowindex_window_kernel<TC, double, true, CK_NONE, true>)
+                            : rowindex_window_kernel<TC, double, false, CK_NONE, true>;
+        else kern = nan ? (f32v ? rowindex_window_kernel<TC, float, true, CK_NONE, false>
+                                : rowindex_window_kernel<TC, double, true, CK_NONE, false>)
+                        : rowindex_window_kernel<TC, double, false, CK_NONE, false>;
+    } else if (byte) {
+        kern = nan ? (f32v ? rowindex_window_kernel<TC, float, true, CK_CODE8, false>
+                           : rowindex_window_kernel<TC, double, true, CK_CODE8, false>)
+                   : rowindex_window_kernel<TC, double, false, CK_CODE8, false>;
+    } else {
+        kern = nan ? (f32v ? rowindex_window_kernel<TC, float, true, CK_ANY, false>
+                           : rowindex_window_kernel<TC, double, true, CK_ANY, false>)
+                   : rowindex_window_kernel<TC, double, false, CK_ANY, false>;
+    }
     const hipError_t e = hipFuncSetAttribute((const void *)kern,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds);
     if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
