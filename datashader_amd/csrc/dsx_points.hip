@@ -1293,6 +1293,12 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
 #pragma unroll
     for (int i = 0; i < kBktPend; ++i) { pend[i] = kNoSlot; ptag[i] = 0u; pv[i] = TV(0); }
     auto lds_at = [&](uint32_t t, uint32_t slot) { return (t << cs) | (slot & cmask); };
+    // a flush's list of the wave's full buckets (bucket, first block) in the end-of-scan
+    // list area, 64 entries a wave; log2 of the 16-byte pieces of a block's offsets and
+    // of its values
+    uint2 *const wfl = fl + (tid >> 6) * 64;
+    const uint32_t lco = cs - 3u;
+    const uint32_t lcv = HAS_VAL ? cs + (sizeof(TV) == 8 ? 3u : 2u) - 4u : 0u;
     auto flush = [&]() {  // every thread calls it, after a check's barrier (lo, cnt stable)
         // A. every bucket holding C slots or more past lo gets k = (count - lo) / C
         //    consecutive blocks of the stream, its headers, and its owner lane's copy of its
@@ -1323,24 +1329,44 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u));
             }
             if (k >= 2u) q0 = atomicAdd(&ctrl[0], k);
-            if (k == 0u) continue;
-            const uint32_t r0 = tblk[b];
-            bbase[b] = q0;
-            bk[b] = k;
-            tblk[b] = r0 + k;
-            newlo[o] = l + (k << cs);
-            for (uint32_t q = 0; q < k; ++q)
-                if (q0 + q < s.bcap)  // always: bcap bounds a workgroup's blocks
-                    s.hdr[g0 + q0 + q] = (unsigned long long)b | ((unsigned long long)(r0 + q) << 32);
-            if (q0 >= s.bcap) continue;
-            uint8_t *dst = blk(q0);
-            const uint4 *so = reinterpret_cast<const uint4 *>(boff + ((uint32_t)b << cs));
-            for (uint32_t w = 0; w < (C * 2u) / 16u; ++w) reinterpret_cast<uint4 *>(dst)[w] = so[w];
-            if constexpr (HAS_VAL) {
-                const uint4 *sv = reinterpret_cast<const uint4 *>(bval + ((uint32_t)b << cs));
-                for (uint32_t w = 0; w < (C * (uint32_t)sizeof(TV)) / 16u; ++w)
-                    reinterpret_cast<uint4 *>(dst + s.o_bval)[w] = sv[w];
+            const unsigned long long mf = __ballot(k != 0u);
+            if (k != 0u) {
+                const uint32_t r0 = tblk[b];
+                bbase[b] = q0;
+                bk[b] = k;
+                tblk[b] = r0 + k;
+                newlo[o] = l + (k << cs);
+                for (uint32_t q = 0; q < k; ++q)
+                    if (q0 + q < s.bcap)  // always: bcap bounds a workgroup's blocks
+                        s.hdr[g0 + q0 + q] = (unsigned long long)b | ((unsigned long long)(r0 + q) << 32);
+                const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(mf >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)mf, 0u));
+                wfl[r] = make_uint2((uint32_t)b, q0);
             }
+            if (mf == 0ull) continue;  // uniform
+            // the wave's full LDS blocks, copied by all its lanes 16 bytes each, consecutive
+            // lanes along a block, so that every store writes whole lines
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            const uint32_t nf = (uint32_t)__popcll(mf);
+            for (uint32_t d = (uint32_t)lane; d < (nf << lco); d += 64u) {
+                const uint2 e = wfl[d >> lco];
+                if (e.y >= s.bcap) continue;
+                const uint32_t w = d & ((1u << lco) - 1u);
+                reinterpret_cast<uint4 *>(blk(e.y))[w] =
+                    reinterpret_cast<const uint4 *>(boff + (e.x << cs))[w];
+            }
+            if constexpr (HAS_VAL) {
+                for (uint32_t d = (uint32_t)lane; d < (nf << lcv); d += 64u) {
+                    const uint2 e = wfl[d >> lcv];
+                    if (e.y >= s.bcap) continue;
+                    const uint32_t w = d & ((1u << lcv) - 1u);
+                    reinterpret_cast<uint4 *>(blk(e.y) + s.o_bval)[w] =
+                        reinterpret_cast<const uint4 *>(bval + (e.x << cs))[w];
+                }
+            }
+            // the list is rewritten by the next bucket set's owners only after every lane's
+            // reads above (one wave, in order)
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
         __syncthreads();
         // B. lo moves past the blocks written; a row held (its slot rel = slot - lo >= C)
@@ -1540,25 +1566,31 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     if (tid == 0) s.nblk[blockIdx.x] = nq;
 }
 
-// Entries [j0, j0 + kStep) of a batch of staged blocks, kU per thread; an index past the
-// batch re-reads entry 0 and is dropped.
+// Groups of four consecutive entries [g0, g0 + kStep) of a batch of staged blocks, kU
+// groups per thread; a group index past the batch re-reads group 0 and is dropped.  A
+// lane reads a group's four 16-bit offsets in one 8-byte load (a wave: four blocks), so
+// four times the bytes are in flight per load instruction.
 template <typename TV, bool HAS_VAL>
 struct BktRows {
-    static constexpr int kU = 8;
-    uint16_t o[kU];
-    TV v[kU];
+    static constexpr int kU = 4;
+    uint2 o[kU];
+    TV v[kU][4];
 
-    __device__ __forceinline__ void load(const BktArgs &s, const uint32_t *sblk, uint32_t j0,
-                                         uint32_t nent) {
-        const uint32_t cmask = (1u << s.cshift) - 1u;
+    __device__ __forceinline__ void load(const BktArgs &s, const uint32_t *sblk, uint32_t g0,
+                                         uint32_t ngrp) {
+        const uint32_t gs = s.cshift - 2u, gmask = (1u << gs) - 1u;  // groups in a block
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            uint32_t j = j0 + (uint32_t)(u * kTileThreads + threadIdx.x);
-            j = j < nent ? j : 0u;
-            const uint8_t *b = s.pool + (size_t)sblk[j >> s.cshift] * s.blk_bytes;
-            o[u] = reinterpret_cast<const uint16_t *>(b)[j & cmask];
-            if constexpr (HAS_VAL) v[u] = reinterpret_cast<const TV *>(b + s.o_bval)[j & cmask];
-            else v[u] = TV(0);
+            uint32_t g = g0 + (uint32_t)(u * kTileThreads + threadIdx.x);
+            g = g < ngrp ? g : 0u;
+            const uint8_t *b = s.pool + (size_t)sblk[g >> gs] * s.blk_bytes;
+            const uint32_t q = g & gmask;
+            o[u] = reinterpret_cast<const uint2 *>(b)[q];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if constexpr (HAS_VAL) v[u][e] = reinterpret_cast<const TV *>(b + s.o_bval)[4u * q + e];
+                else v[u][e] = TV(0);
+            }
         }
     }
 };
@@ -1613,34 +1645,37 @@ __global__ __launch_bounds__(kTileThreads) void btile_agg_kernel(PartArgs p, Bkt
             sblk[i] = (uint32_t)(g0 + s.wl[g0 + lpg[lo_] + (v - ex[lo_])]);
         }
         __syncthreads();
-        const uint32_t nent = nbb << s.cshift;
+        const uint32_t ngrp = nbb << (s.cshift - 2u);  // groups of four entries
         R ra, rb, rc;  // three register sets rotate roles, as in the scan
-        ra.load(s, sblk, 0u, nent);
-        rb.load(s, sblk, kStep, nent);
-        uint32_t j0 = 0u;
+        ra.load(s, sblk, 0u, ngrp);
+        rb.load(s, sblk, kStep, ngrp);
+        uint32_t g0 = 0u;
         auto step = [&](R &cur, R &far) {
-            far.load(s, sblk, j0 + 2u * kStep, nent);
+            far.load(s, sblk, g0 + 2u * kStep, ngrp);
 #pragma unroll
             for (int u = 0; u < R::kU; ++u) {
-                if (j0 + (uint32_t)(u * kTileThreads + threadIdx.x) >= nent) continue;
-                const uint32_t lc = cur.o[u];
-                if (lc >= L.cap) continue;  // an unused slot of a partial block
-                double v = 0.0;
-                bool vnan = true;
-                if constexpr (HAS_VAL) {
-                    v = (double)cur.v[u];
-                    vnan = v != v;
+                if (g0 + (uint32_t)(u * kTileThreads + threadIdx.x) >= ngrp) continue;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const uint32_t lc = ((e < 2 ? cur.o[u].x : cur.o[u].y) >> (16 * (e & 1))) & 0xFFFFu;
+                    if (lc >= L.cap) continue;  // an unused slot of a partial block
+                    double v = 0.0;
+                    bool vnan = true;
+                    if constexpr (HAS_VAL) {
+                        v = (double)cur.v[u][e];
+                        vnan = v != v;
+                    }
+                    double add = v;
+                    if constexpr (SQDEV) {
+                        const double d = v - mean[c0 + lc];
+                        add = d * d;
+                    }
+                    region_add<KEYB>(smem, L, lc, vnan, v, add);
                 }
-                double add = v;
-                if constexpr (SQDEV) {
-                    const double d = v - mean[c0 + lc];
-                    add = d * d;
-                }
-                region_add<KEYB>(smem, L, lc, vnan, v, add);
             }
-            j0 += kStep;
+            g0 += kStep;
         };
-        while (j0 < nent) {  // steps past nent load entry 0 and add nothing
+        while (g0 < ngrp) {  // steps past ngrp load group 0 and add nothing
             step(ra, rc);
             step(rb, ra);
             step(rc, rb);
@@ -1780,7 +1815,8 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
             P.bo_tblk = (uint32_t)q; q = align_up(q + 4 * nb, 16);
             P.bo_base = (uint32_t)q; q = align_up(q + 4 * (nb + kBktSpare), 16);
             P.bo_bk = (uint32_t)q; q = align_up(q + 4 * (nb + kBktSpare), 16);
-            P.bo_fl = (uint32_t)q; q = align_up(q + 8 * nb, 16);
+            // (the end-of-scan list; during the scan, the flushes' per-wave lists)
+            P.bo_fl = (uint32_t)q; q = align_up(q + 8 * (nb > (size_t)kScanThreads ? nb : (size_t)kScanThreads), 16);
             P.bo_ctrl = (uint32_t)q; q += 32;
             const int64_t bcap = (group_rows + (int64_t)C - 1) / (int64_t)C + (int64_t)nb;
             // (512 bytes left for the kernel's static LDS words)
