@@ -1247,9 +1247,9 @@ struct BktArgs {
 // more past lo writes its LDS block and the full blocks its overflowing rows fill (those
 // rows write theirs directly), lo moves past them, and the rows left over take their LDS
 // positions (the next chunks' rows insert at later slots, whose positions cannot collide
-// with theirs).  A flush is two more barriers and a few LDS round trips during which no
+// with theirs).  A flush is one more barrier and a few LDS round trips during which no
 // row is binned, so they are batched: few checks, every LDS read of a phase issued
-// together, each bucket's block copied by its owner lane, all of them in parallel.  (A
+// together, the full blocks of a wave's buckets copied by all its lanes.  (A
 // barrier-free form, in which the lane completing a block writes it out alone, measured
 // slower: those copies run with one lane of a wave active.)  The buckets left at the end
 // are written as partial blocks (unused slots marked).
@@ -1287,7 +1287,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     __syncthreads();
     const size_t g0 = (size_t)blockIdx.x * s.bcap;  // this workgroup's first block
     auto blk = [&](uint32_t q) { return s.pool + (g0 + q) * (size_t)s.blk_bytes; };
-    // rows held past their bucket's LDS block: slot - lo, tile << 16 | tile offset, value
+    // rows held past their bucket's LDS block: slot, tile << 16 | tile offset, value
     uint32_t pend[kBktPend], ptag[kBktPend];
     TV pv[kBktPend];
 #pragma unroll
@@ -1303,9 +1303,6 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
         // A. every bucket holding C slots or more past lo gets k = (count - lo) / C
         //    consecutive blocks of the stream, its headers, and its owner lane's copy of its
         //    LDS block, 16 bytes at a time
-        uint32_t newlo[kBktOwn];
-#pragma unroll
-        for (int o = 0; o < kBktOwn; ++o) newlo[o] = kNoSlot;
 #pragma unroll
         for (int o = 0; o < kBktOwn; ++o) {
             if (o * kScanThreads >= s.nb) break;  // uniform
@@ -1335,7 +1332,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
                 bbase[b] = q0;
                 bk[b] = k;
                 tblk[b] = r0 + k;
-                newlo[o] = l + (k << cs);
+                lo[b] = l + (k << cs);  // only owners read lo during a flush
                 for (uint32_t q = 0; q < k; ++q)
                     if (q0 + q < s.bcap)  // always: bcap bounds a workgroup's blocks
                         s.hdr[g0 + q0 + q] = (unsigned long long)b | ((unsigned long long)(r0 + q) << 32);
@@ -1369,27 +1366,25 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
         __syncthreads();
-        // B. lo moves past the blocks written; a row held (its slot rel = slot - lo >= C)
-        //    goes to its full block rel / C (straight to the stream) when that is < k, else
-        //    to the front of the bucket's next LDS block (at position rel % C)
-#pragma unroll
-        for (int o = 0; o < kBktOwn; ++o) {
-            const int b = tid + o * kScanThreads;
-            if (b < s.nb && newlo[o] != kNoSlot) lo[b] = newlo[o];
-        }
-        uint32_t bw[kBktPend], kw[kBktPend];
+        // B. a row held (rel = slot - the old lo >= C) goes to its full block rel / C
+        //    (straight to the stream) when that is < k, else to the front of the bucket's
+        //    next LDS block (position slot % C)
+        uint32_t bw[kBktPend], kw[kBktPend], rw[kBktPend];
 #pragma unroll
         for (int i = 0; i < kBktPend; ++i) {
             const uint32_t idx = pend[i] != kNoSlot ? ptag[i] >> 16 : spare;
             bw[i] = bbase[idx];
             kw[i] = bk[idx];
+            rw[i] = lo[idx];
         }
+#pragma unroll
+        for (int i = 0; i < kBktPend; ++i) rw[i] = pend[i] - (rw[i] - (kw[i] << cs));
         // (two loops: one store site that could write either memory would be a flat store,
         // and the LDS waits after it would also wait for its write to global memory)
 #pragma unroll
         for (int i = 0; i < kBktPend; ++i) {
-            if (pend[i] == kNoSlot || (pend[i] >> cs) >= kw[i]) continue;  // (pend >> cs >= 1)
-            const uint32_t qb = bw[i] + (pend[i] >> cs);
+            if (pend[i] == kNoSlot || (rw[i] >> cs) >= kw[i]) continue;  // (rw >> cs >= 1)
+            const uint32_t qb = bw[i] + (rw[i] >> cs);
             if (qb < s.bcap) {
                 uint8_t *dst = blk(qb);
                 reinterpret_cast<uint16_t *>(dst)[pend[i] & cmask] = (uint16_t)ptag[i];
@@ -1405,9 +1400,9 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
             if constexpr (HAS_VAL) bval[at] = pv[i];
             pend[i] = kNoSlot;
         }
-        // the next chunks' rows read the new lo, and insert at slots past every one just
-        // placed: their LDS positions cannot collide
-        __syncthreads();
+        // no barrier: the next chunks' rows insert at slots past every one just placed, and
+        // one that reads the old lo holds its row (slot - old lo >= k C), so no LDS position
+        // collides; the next check's barrier orders these writes before the next flush
     };
     const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
     using R = Rows<TC, TV, HAS_VAL, CK>;
@@ -1470,7 +1465,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
                 boff[at] = oo[it];
                 if constexpr (HAS_VAL) bval[at] = cur.v[it];
             } else {
-                pend[pb + it] = ss[it] - ll[it];  // relative to lo: >= C
+                pend[pb + it] = ss[it];  // the slot itself: lo may move before the flush
                 ptag[pb + it] = (t << 16) | oo[it];
                 if constexpr (HAS_VAL) pv[pb + it] = cur.v[it];
                 over = true;
