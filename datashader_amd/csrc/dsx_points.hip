@@ -1266,8 +1266,9 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     // per bucket (+ kBktSpare spare words, which dropped rows and idle lanes address)
     uint32_t *cnt = reinterpret_cast<uint32_t *>(smem + s.o_cnt);     // slots taken
     uint32_t *lo = reinterpret_cast<uint32_t *>(smem + s.o_lo);       // first slot in LDS
-    uint32_t *bbase = reinterpret_cast<uint32_t *>(smem + s.o_base);  // first block of a flush
-    uint32_t *bk = reinterpret_cast<uint32_t *>(smem + s.o_bk);       // blocks of that flush
+    uint32_t *bbase = reinterpret_cast<uint32_t *>(smem + s.o_base);  // a flush: block of slot v
+                                                                      // is bbase + v / C
+    uint32_t *xs = reinterpret_cast<uint32_t *>(smem + s.o_bk);       // end: a scan's scratch
     uint32_t *tblk = reinterpret_cast<uint32_t *>(smem + s.o_tblk);   // blocks written
     uint2 *fl = reinterpret_cast<uint2 *>(smem + s.o_fl);             // end: (bucket, block)
     uint16_t *boff = reinterpret_cast<uint16_t *>(smem + s.o_off);    // [nb][C] offsets
@@ -1280,7 +1281,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     const uint32_t spare = (uint32_t)s.nb + (uint32_t)lane;
     const int ncat = a.cat.ncat;
     for (int b = tid; b < s.nb + kBktSpare; b += kScanThreads) {
-        cnt[b] = 0u; lo[b] = 0u; bbase[b] = 0u; bk[b] = 0u;
+        cnt[b] = 0u; lo[b] = 0u; bbase[b] = 0u;
         if (b < s.nb) tblk[b] = 0u;
     }
     if (tid < 5) ctrl[tid] = 0u;
@@ -1329,8 +1330,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
             const unsigned long long mf = __ballot(k != 0u);
             if (k != 0u) {
                 const uint32_t r0 = tblk[b];
-                bbase[b] = q0;
-                bk[b] = k;
+                bbase[b] = q0 - (l >> cs);  // the block of slot v is bbase + v / C
                 tblk[b] = r0 + k;
                 lo[b] = l + (k << cs);  // only owners read lo during a flush
                 for (uint32_t q = 0; q < k; ++q)
@@ -1366,25 +1366,22 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
         __syncthreads();
-        // B. a row held (rel = slot - the old lo >= C) goes to its full block rel / C
-        //    (straight to the stream) when that is < k, else to the front of the bucket's
-        //    next LDS block (position slot % C)
-        uint32_t bw[kBktPend], kw[kBktPend], rw[kBktPend];
+        // B. a row held goes to the full block of its slot (straight to the stream) when
+        //    the slot is below the new lo, else to the front of the bucket's next LDS block
+        //    (position slot % C)
+        uint32_t bw[kBktPend], lw[kBktPend];
 #pragma unroll
         for (int i = 0; i < kBktPend; ++i) {
             const uint32_t idx = pend[i] != kNoSlot ? ptag[i] >> 16 : spare;
             bw[i] = bbase[idx];
-            kw[i] = bk[idx];
-            rw[i] = lo[idx];
+            lw[i] = lo[idx];
         }
-#pragma unroll
-        for (int i = 0; i < kBktPend; ++i) rw[i] = pend[i] - (rw[i] - (kw[i] << cs));
         // (two loops: one store site that could write either memory would be a flat store,
         // and the LDS waits after it would also wait for its write to global memory)
 #pragma unroll
         for (int i = 0; i < kBktPend; ++i) {
-            if (pend[i] == kNoSlot || (rw[i] >> cs) >= kw[i]) continue;  // (rw >> cs >= 1)
-            const uint32_t qb = bw[i] + (rw[i] >> cs);
+            if (pend[i] == kNoSlot || pend[i] >= lw[i]) continue;
+            const uint32_t qb = bw[i] + (pend[i] >> cs);
             if (qb < s.bcap) {
                 uint8_t *dst = blk(qb);
                 reinterpret_cast<uint16_t *>(dst)[pend[i] & cmask] = (uint16_t)ptag[i];
@@ -1528,7 +1525,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
             sum += v[i];
         }
         uint32_t total;
-        uint32_t run = block_exscan32(sum, reinterpret_cast<uint32_t *>(bk), total);
+        uint32_t run = block_exscan32(sum, xs, total);
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
             const int b = tid * kPer + i;
@@ -1809,7 +1806,7 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
             P.bo_lo = (uint32_t)q; q = align_up(q + 4 * (nb + kBktSpare), 16);
             P.bo_tblk = (uint32_t)q; q = align_up(q + 4 * nb, 16);
             P.bo_base = (uint32_t)q; q = align_up(q + 4 * (nb + kBktSpare), 16);
-            P.bo_bk = (uint32_t)q; q = align_up(q + 4 * (nb + kBktSpare), 16);
+            P.bo_bk = (uint32_t)q; q = align_up(q + 4 * kBktSpare, 16);  // >= 17 words
             // (the end-of-scan list; during the scan, the flushes' per-wave lists)
             P.bo_fl = (uint32_t)q; q = align_up(q + 8 * (nb > (size_t)kScanThreads ? nb : (size_t)kScanThreads), 16);
             P.bo_ctrl = (uint32_t)q; q += 32;
