@@ -1217,6 +1217,7 @@ constexpr int kBktMaxC = 256;
 constexpr uint32_t kBktMaxTileCells = 32768;  // tile offsets < 2^15: 0xFFFF marks unused slots
 constexpr uint16_t kBktUnused = 0xFFFF;   // an unused slot of a partial block
 constexpr int kBktBatch = 1024;           // block ids the tile pass stages in LDS at a time
+constexpr int kBktBatchMax = 4096;        // (more, where two workgroups still fit a CU)
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
 struct BktArgs {
@@ -1236,6 +1237,7 @@ struct BktArgs {
     uint32_t *lpre;                // [G][nb]
     uint32_t o_cnt, o_lo, o_tblk, o_base, o_bk, o_fl, o_off, o_val, o_ctrl;  // scan LDS offsets
     uint32_t loc_cap;              // block indices the bucket area of LDS holds
+    uint32_t batch;                // block ids the tile pass stages at a time
 };
 
 // The tile strategy's scan.  Bucket b's slots are numbered from the start of the scan:
@@ -1597,11 +1599,11 @@ __global__ __launch_bounds__(kTileThreads) void btile_agg_kernel(PartArgs p, Bkt
                                                                 const double *mean, Planes out) {
     if (!tiles_run(window)) return;  // uniform: the window strategy's tile pass runs instead
     extern __shared__ __align__(16) uint8_t smem[];
-    uint32_t *sblk = reinterpret_cast<uint32_t *>(smem + L.bytes);  // [kBktBatch]
+    uint32_t *sblk = reinterpret_cast<uint32_t *>(smem + L.bytes);  // [s.batch]
     // the tile's blocks, in the order of their scan workgroups: workgroup g's run is
     // [ex[g], ex[g + 1]) of the tile's blocks (ex from spill_colscan_kernel), found in its
     // tile-ordered list at lp[g]
-    uint32_t *ex = sblk + kBktBatch;              // [G + 1]
+    uint32_t *ex = sblk + s.batch;                // [G + 1]
     uint32_t *lpg = ex + p.ngroups + 1;           // [G]
     using R = BktRows<TV, HAS_VAL>;
     constexpr uint32_t kStep = (uint32_t)R::kU * kTileThreads;
@@ -1622,8 +1624,8 @@ __global__ __launch_bounds__(kTileThreads) void btile_agg_kernel(PartArgs p, Bkt
         ex[g] = g < G ? p.hist[(size_t)g * p.ntiles + t] : (uint32_t)(te - tb);
         if (g < G) lpg[g] = s.lpre[(size_t)g * s.nb + t];
     }
-    for (unsigned long long bb = b; bb < e; bb += kBktBatch) {
-        const uint32_t nbb = (uint32_t)(e - bb < (unsigned long long)kBktBatch ? e - bb : kBktBatch);
+    for (unsigned long long bb = b; bb < e; bb += s.batch) {
+        const uint32_t nbb = (uint32_t)(e - bb < (unsigned long long)s.batch ? e - bb : s.batch);
         __syncthreads();  // the region and ex / lpg are written, the last batch's ids are read
         for (uint32_t i = threadIdx.x; i < nbb; i += kTileThreads) {
             const uint32_t v = (uint32_t)(bb - tb) + i;  // the block's index in the tile
@@ -1710,6 +1712,7 @@ struct Plan {
     uint32_t cshift, o_bval, blk_bytes, bcap, min_part_tiles;
     uint32_t bo_cnt, bo_lo, bo_tblk, bo_base, bo_bk, bo_fl, bo_off, bo_val, bo_ctrl;  // its scan's LDS
     size_t bscan_lds, btile_lds;
+    uint32_t batch;  // the tile pass's staged block ids
     size_t o_nblk, o_bhdr, o_bwl, o_blpre, o_pool;
 };
 
@@ -1794,6 +1797,7 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     // the tile strategy: the largest bucket C (a power of two, kBktMinC..kBktMaxC) whose
     // nb buckets fit the scan's LDS next to seven words per bucket
     P.tiles_ok = false;
+    P.batch = kBktBatch;
     if (tcap <= (int64_t)kBktMaxTileCells) {
         const size_t vb = (size_t)P.vbytes, nb = (size_t)P.ntiles;
         for (int cs = 8; (1 << cs) >= kBktMinC && !P.tiles_ok; --cs) {
@@ -1821,7 +1825,16 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
                 P.blk_bytes = (uint32_t)align_up(P.o_bval + vb * C, 16);
                 const uint32_t mp = (uint32_t)(kMinPart >> cs);
                 P.min_part_tiles = mp < 16u ? 16u : mp;
-                P.btile_lds = (size_t)P.tl.bytes + 4 * ((size_t)kBktBatch + 2 * (size_t)P.scan_groups + 1);
+                // the tile pass stages kBktBatch block ids at a time, or up to
+                // kBktBatchMax where two of its workgroups still fit a CU: each batch
+                // starts its load pipeline over
+                P.batch = kBktBatch;
+                auto blds = [&](size_t nbat) {
+                    return (size_t)P.tl.bytes + 4 * (nbat + 2 * (size_t)P.scan_groups + 1);
+                };
+                while (P.batch < (uint32_t)kBktBatchMax && blds(2 * (size_t)P.batch) <= (size_t)kLdsBytes / 2)
+                    P.batch *= 2;
+                P.btile_lds = blds(P.batch);
             }
         }
     }
@@ -1989,6 +2002,7 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
         ba.o_val = P.bo_val;
         ba.o_ctrl = P.bo_ctrl;
         ba.loc_cap = P.bo_cnt / 4u;  // values and offsets come first, from byte 0
+        ba.batch = P.batch;
         const bool lin = !a.xlog && !a.ylog && origin;
         auto bscan = bscan_kernel<TC, TV, HAS_VAL, CK_ANY, false>;
         if (a.cat.kind == DSX_CAT_NONE)
