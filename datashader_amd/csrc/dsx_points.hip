@@ -1405,15 +1405,20 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     };
     const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
     using R = Rows<TC, TV, HAS_VAL, CK>;
-    R ra, rb, rc;
+    // chunks loaded ahead: three where the checks come every three chunks (four register
+    // sets: the loads in flight cover a flush), else two; the generic category column (a
+    // 64-bit raw value a row) keeps two, as a fourth set would spill registers
+    constexpr int kAhead = (kSteps == 3 && CK != CK_ANY) ? 3 : 2;
+    R ra, rb, rc, rd;
     ra.load(a, blockIdx.x, tid);
     rb.load(a, (int64_t)blockIdx.x + gridDim.x, tid);
+    if constexpr (kAhead == 3) rc.load(a, (int64_t)blockIdx.x + 2 * (int64_t)gridDim.x, tid);
     int64_t ck = blockIdx.x;
     uint32_t fk = 0;  // this check's flag: ctrl[2 + fk]
     bool over = false;
     // pb: the first of this chunk's kItems entries of pend (a constant at every call)
     auto step = [&](R &cur, R &far, const int pb) {
-        far.load(a, ck + 2 * (int64_t)gridDim.x, tid);
+        far.load(a, ck + kAhead * (int64_t)gridDim.x, tid);
         const int64_t rem64 = a.n - ck * kChunk;
         const uint32_t rem = rem64 >= kChunk ? (uint32_t)kChunk
                                              : (rem64 > 0 ? (uint32_t)rem64 : 0u);
@@ -1483,13 +1488,22 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
         over = false;
         fk = fk == 2u ? 0u : fk + 1u;
     };
-    while (ck < nchunks) {  // three steps per trip, as in scan_kernel
-        step(ra, rc, 0);
-        if constexpr (kSteps == 1) check();
-        step(rb, ra, kSteps == 1 ? 0 : kItems);
-        if constexpr (kSteps == 1) check();
-        step(rc, rb, kSteps == 1 ? 0 : 2 * kItems);
-        check();
+    if constexpr (kAhead == 3) {
+        while (ck < nchunks) {  // the four sets rotate through twelve steps, four checks
+            step(ra, rd, 0); step(rb, ra, kItems); step(rc, rb, 2 * kItems); check();
+            step(rd, rc, 0); step(ra, rd, kItems); step(rb, ra, 2 * kItems); check();
+            step(rc, rb, 0); step(rd, rc, kItems); step(ra, rd, 2 * kItems); check();
+            step(rb, ra, 0); step(rc, rb, kItems); step(rd, rc, 2 * kItems); check();
+        }
+    } else {
+        while (ck < nchunks) {  // three steps per trip, as in scan_kernel
+            step(ra, rc, 0);
+            if constexpr (kSteps == 1) check();
+            step(rb, ra, kSteps == 1 ? 0 : kItems);
+            if constexpr (kSteps == 1) check();
+            step(rc, rb, kSteps == 1 ? 0 : 2 * kItems);
+            check();
+        }
     }
     __syncthreads();
     // the buckets left: one partial block each (every bucket holds < C slots past lo)
