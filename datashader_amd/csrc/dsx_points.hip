@@ -1271,6 +1271,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     uint32_t *bbase = reinterpret_cast<uint32_t *>(smem + s.o_base);  // a flush: block of slot v
                                                                       // is bbase + v / C
     uint32_t *xs = reinterpret_cast<uint32_t *>(smem + s.o_bk);       // end: a scan's scratch
+                                                                      // (before: spare slots)
     uint32_t *tblk = reinterpret_cast<uint32_t *>(smem + s.o_tblk);   // blocks written
     uint2 *fl = reinterpret_cast<uint2 *>(smem + s.o_fl);             // end: (bucket, block)
     uint16_t *boff = reinterpret_cast<uint16_t *>(smem + s.o_off);    // [nb][C] offsets
@@ -1300,6 +1301,9 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     // list area, 64 entries a wave; log2 of the 16-byte pieces of a block's offsets and
     // of its values
     uint2 *const wfl = fl + (tid >> 6) * 64;
+    // each lane's spare offset and value positions, in the end-of-scan scratch words
+    const uint32_t off_spare = (s.o_bk - s.o_off) / 2u + (uint32_t)lane;
+    const uint32_t val_spare = (s.o_bk - s.o_val) / (uint32_t)sizeof(TV) + (uint32_t)lane;
     const uint32_t lco = cs - 3u;
     const uint32_t lcv = HAS_VAL ? cs + (sizeof(TV) == 8 ? 3u : 2u) - 4u : 0u;
     auto flush = [&]() {  // every thread calls it, after a check's barrier (lo, cnt stable)
@@ -1371,33 +1375,75 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
         // B. a row held goes to the full block of its slot (straight to the stream) when
         //    the slot is below the new lo, else to the front of the bucket's next LDS block
         //    (position slot % C)
-        uint32_t bw[kBktPend], lw[kBktPend];
+        if constexpr (!HAS_VAL) {
+            // a wave gathers its held rows (about one in ten) into its 64-entry list and
+            // places them 64 at a time with every lane busy, instead of testing each of its
+            // kBktPend registers in every lane
+            uint32_t n = 0u;  // uniform: rows in the wave's list
+            auto place = [&](uint32_t cnt) {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                if ((uint32_t)lane < cnt) {
+                    const uint2 e = wfl[lane];  // (slot, tile << 16 | tile offset)
+                    const uint32_t t = e.y >> 16;
+                    const uint32_t bw = bbase[t], lw = lo[t];
+                    // (two store sites: one that could write either memory would be a flat
+                    // store, whose LDS waits would also wait for its global write)
+                    if (e.x < lw) {
+                        const uint32_t qb = bw + (e.x >> cs);
+                        if (qb < s.bcap)
+                            global_ptr(reinterpret_cast<uint16_t *>(blk(qb)))[e.x & cmask] = (uint16_t)e.y;
+                    } else {
+                        boff[lds_at(t, e.x)] = (uint16_t)e.y;
+                    }
+                }
+                // the list is rewritten only after every lane's read above (one wave, in order)
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            };
 #pragma unroll
-        for (int i = 0; i < kBktPend; ++i) {
-            const uint32_t idx = pend[i] != kNoSlot ? ptag[i] >> 16 : spare;
-            bw[i] = bbase[idx];
-            lw[i] = lo[idx];
-        }
-        // (two loops: one store site that could write either memory would be a flat store,
-        // and the LDS waits after it would also wait for its write to global memory)
-#pragma unroll
-        for (int i = 0; i < kBktPend; ++i) {
-            if (pend[i] == kNoSlot || pend[i] >= lw[i]) continue;
-            const uint32_t qb = bw[i] + (pend[i] >> cs);
-            if (qb < s.bcap) {
-                uint8_t *dst = blk(qb);
-                reinterpret_cast<uint16_t *>(dst)[pend[i] & cmask] = (uint16_t)ptag[i];
-                if constexpr (HAS_VAL) reinterpret_cast<TV *>(dst + s.o_bval)[pend[i] & cmask] = pv[i];
+            for (int i = 0; i < kBktPend; ++i) {
+                const bool h = pend[i] != kNoSlot;
+                const unsigned long long m = __ballot(h);
+                if (m == 0ull) continue;  // uniform
+                const uint32_t c = (uint32_t)__popcll(m);
+                if (n + c > 64u) { place(n); n = 0u; }
+                if (h) {
+                    const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                    wfl[n + r] = make_uint2(pend[i], ptag[i]);
+                }
+                n += c;
+                pend[i] = kNoSlot;
             }
-            pend[i] = kNoSlot;
-        }
+            if (n != 0u) place(n);
+        } else {
+            uint32_t bw[kBktPend], lw[kBktPend];
 #pragma unroll
-        for (int i = 0; i < kBktPend; ++i) {
-            if (pend[i] == kNoSlot) continue;
-            const uint32_t at = lds_at(ptag[i] >> 16, pend[i]);
-            boff[at] = (uint16_t)ptag[i];
-            if constexpr (HAS_VAL) bval[at] = pv[i];
-            pend[i] = kNoSlot;
+            for (int i = 0; i < kBktPend; ++i) {
+                const uint32_t idx = pend[i] != kNoSlot ? ptag[i] >> 16 : spare;
+                bw[i] = bbase[idx];
+                lw[i] = lo[idx];
+            }
+            // (two loops: one store site that could write either memory would be a flat
+            // store, and the LDS waits after it would also wait for its write to global memory)
+#pragma unroll
+            for (int i = 0; i < kBktPend; ++i) {
+                if (pend[i] == kNoSlot || pend[i] >= lw[i]) continue;
+                const uint32_t qb = bw[i] + (pend[i] >> cs);
+                if (qb < s.bcap) {
+                    uint8_t *dst = blk(qb);
+                    reinterpret_cast<uint16_t *>(dst)[pend[i] & cmask] = (uint16_t)ptag[i];
+                    reinterpret_cast<TV *>(dst + s.o_bval)[pend[i] & cmask] = pv[i];
+                }
+                pend[i] = kNoSlot;
+            }
+#pragma unroll
+            for (int i = 0; i < kBktPend; ++i) {
+                if (pend[i] == kNoSlot) continue;
+                const uint32_t at = lds_at(ptag[i] >> 16, pend[i]);
+                boff[at] = (uint16_t)ptag[i];
+                bval[at] = pv[i];
+                pend[i] = kNoSlot;
+            }
         }
         // no barrier: the next chunks' rows insert at slots past every one just placed, and
         // one that reads the old lo holds its row (slot - old lo >= k C), so no LDS position
@@ -1460,20 +1506,20 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
             ll[it] = lo[tt[it]];  // current: lo changes only inside a flush
             ss[it] = atomicAdd(&cnt[tt[it]], 1u);
         }
+        // without branches: a row that does not go to its LDS block (dropped, or held:
+        // its slot is past the block) writes its lane's spare position instead
 #pragma unroll
         for (int it = 0; it < kItems; ++it) {
             const uint32_t t = tt[it];
-            if (t >= (uint32_t)s.nb) continue;  // dropped
-            if (ss[it] - ll[it] < C) {
-                const uint32_t at = lds_at(t, ss[it]);
-                boff[at] = oo[it];
-                if constexpr (HAS_VAL) bval[at] = cur.v[it];
-            } else {
-                pend[pb + it] = ss[it];  // the slot itself: lo may move before the flush
-                ptag[pb + it] = (t << 16) | oo[it];
-                if constexpr (HAS_VAL) pv[pb + it] = cur.v[it];
-                over = true;
-            }
+            const bool keep = t < (uint32_t)s.nb;
+            const bool inl = keep & (ss[it] - ll[it] < C);
+            const bool held = keep & !inl;
+            boff[inl ? lds_at(t, ss[it]) : off_spare] = oo[it];
+            if constexpr (HAS_VAL) bval[inl ? lds_at(t, ss[it]) : val_spare] = cur.v[it];
+            pend[pb + it] = held ? ss[it] : kNoSlot;  // the slot itself: lo may move before the flush
+            ptag[pb + it] = (t << 16) | oo[it];
+            if constexpr (HAS_VAL) pv[pb + it] = cur.v[it];
+            over |= held;
         }
         ck += gridDim.x;
     };
@@ -1824,7 +1870,9 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
             P.bo_lo = (uint32_t)q; q = align_up(q + 4 * (nb + kBktSpare), 16);
             P.bo_tblk = (uint32_t)q; q = align_up(q + 4 * nb, 16);
             P.bo_base = (uint32_t)q; q = align_up(q + 4 * (nb + kBktSpare), 16);
-            P.bo_bk = (uint32_t)q; q = align_up(q + 4 * kBktSpare, 16);  // >= 17 words
+            // (the end's scan scratch, >= 17 words; during the scan, a spare offset and
+            // value position for each lane of a wave)
+            P.bo_bk = (uint32_t)q; q = align_up(q + 8 * 64, 16);
             // (the end-of-scan list; during the scan, the flushes' per-wave lists)
             P.bo_fl = (uint32_t)q; q = align_up(q + 8 * (nb > (size_t)kScanThreads ? nb : (size_t)kScanThreads), 16);
             P.bo_ctrl = (uint32_t)q; q += 32;
