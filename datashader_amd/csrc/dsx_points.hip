@@ -1283,6 +1283,12 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     const uint32_t omask = (1u << s.tile_shift) - 1u;
     const uint32_t spare = (uint32_t)s.nb + (uint32_t)lane;
     const int ncat = a.cat.ncat;
+    const uint32_t wn = (uint32_t)a.W * (uint32_t)(CK == CK_NONE ? 1 : ncat);  // < 2^24 (host)
+    // a one-byte code b (zero-extended) is category b - (b >= c8_at ? c8_sub : 0), dropped
+    // unless that is below ncat: for uint8 codes c8_at = 256 (never); for int8 codes the
+    // bytes >= 128 are k = b - 256 < 0, which the reference wraps once to k + ncat
+    const uint32_t c8_at = a.cat.dtype == DSX_I8 ? 128u : 256u;
+    const uint32_t c8_sub = 256u - (uint32_t)ncat;
     for (int b = tid; b < s.nb + kBktSpare; b += kScanThreads) {
         cnt[b] = 0u; lo[b] = 0u; bbase[b] = 0u;
         if (b < s.nb) tblk[b] = 0u;
@@ -1490,14 +1496,20 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
             } else {
                 keep = j < rem && locate_xy<TC>(a, cur.x[it], cur.y[it], px, py);
             }
-            if constexpr (CK != CK_NONE) {
+            if constexpr (CK == CK_CODE8) {
+                const uint32_t b8 = (uint32_t)cur.c[it];
+                const uint32_t k = b8 - (b8 >= c8_at ? c8_sub : 0u);
+                keep = keep & (k < (uint32_t)ncat);
+                pc = (int32_t)k;
+            } else if constexpr (CK != CK_NONE) {
                 pc = scan_category<CK>(a.cat, cur.c[it]);
                 keep = keep && pc >= 0;
             }
-            // cells < 2^32 (checked on the host), so 32-bit arithmetic is exact; py and W
-            // are < 2^24, so the pixel is a 24-bit multiply-add
-            const uint32_t pix = __umul24((uint32_t)py, (uint32_t)a.W) + (uint32_t)px;
-            const uint32_t cell = CK == CK_NONE ? pix : mul_add32(pix, (uint32_t)ncat, (uint32_t)pc);
+            // cells < 2^32 (checked on the host), so 32-bit arithmetic is exact; py, W ncat,
+            // px and ncat are < 2^24, so the cell is 24-bit multiply-adds
+            const uint32_t cell = CK == CK_NONE
+                ? __umul24((uint32_t)py, wn) + (uint32_t)px
+                : __umul24((uint32_t)py, wn) + __umul24((uint32_t)px, (uint32_t)ncat) + (uint32_t)pc;
             tt[it] = keep ? cell >> s.tile_shift : spare;
             oo[it] = (uint16_t)(cell & omask);
         }
@@ -1858,7 +1870,8 @@ static int make_plan(const dsx_points_desc *d, const dsx_view *v, const dsx_plan
     // nb buckets fit the scan's LDS next to seven words per bucket
     P.tiles_ok = false;
     P.batch = kBktBatch;
-    if (tcap <= (int64_t)kBktMaxTileCells) {
+    // (its scan forms a cell with 24-bit multiplies: py * (W ncat) + px * ncat + category)
+    if (tcap <= (int64_t)kBktMaxTileCells && (int64_t)v->width * ncat < (1ll << 24)) {
         const size_t vb = (size_t)P.vbytes, nb = (size_t)P.ntiles;
         for (int cs = 8; (1 << cs) >= kBktMinC && !P.tiles_ok; --cs) {
             if ((1 << cs) > kBktMaxC) continue;

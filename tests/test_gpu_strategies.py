@@ -115,6 +115,32 @@ def test_rows_in_one_tile_by_category(strategy):
     assert agg.values[500].sum() == n
 
 
+@pytest.mark.parametrize("dt", [np.int8, np.uint8])
+def test_byte_codes_out_of_range_vs_oracle(strategy, dt):
+    """One-byte category codes outside [0, ncat): an int8 code k < 0 counts as k + ncat
+    (once; still negative is dropped), codes >= ncat are dropped, for int8 and uint8
+    columns on a 1024 x 256 x 16 canvas, bit-exact vs the oracle."""
+    rng = np.random.default_rng(11)
+    n = 1_500_003
+    W, H, ncat = 1024, 256, 16
+    x = rng.random(n).astype(np.float32)
+    y = rng.random(n).astype(np.float32)
+    lo_, hi_ = (-40, 40) if dt is np.int8 else (0, 60)
+    cat = rng.integers(lo_, hi_, n).astype(dt)
+    cats = [f"c{k}" for k in range(ncat)]
+    f = dsa.DeviceFrame({"x": torch.from_numpy(x).cuda(), "y": torch.from_numpy(y).cuda(),
+                         "cat": torch.from_numpy(cat).cuda()}, categories={"cat": cats})
+    agg = dsa.Canvas(W, H, x_range=(0, 1), y_range=(0, 1)).points(
+        f, "x", "y", dsa.by("cat", dsa.count()))
+    view = G.view_for(W, H, (0, 1), (0, 1))
+    exp = oracle.points(x, y, view, W, H, ("count",), cat=cat.astype(np.int64), ncat=ncat,
+                        squeeze=False)["count"]
+    np.testing.assert_array_equal(agg.values, exp)
+    kept = cat.astype(np.int64)
+    kept = np.where(kept < 0, kept + ncat, kept)
+    assert agg.values.sum() == int(((kept >= 0) & (kept < ncat)).sum())
+
+
 def test_auto_picks_by_the_sample_and_agrees():
     """AUTO on a resident frame: spread ridge rows by 16 categories pick the tile
     strategy, a 0.01-sigma blob picks the window; the record keeps the pick, later calls
