@@ -1240,16 +1240,16 @@ struct BktArgs {
     uint32_t batch;                // block ids the tile pass stages at a time
 };
 
-// The tile strategy's scan.  Bucket b's slots are numbered from the start of the scan:
-// its LDS block holds slots [lo[b], lo[b] + C) at LDS position slot % C.  Each row takes
-// the next slot of its tile's bucket (an LDS atomic); a row whose slot is past the LDS
-// block keeps it in registers.  After every kSteps chunks (three for float32 coordinates
-// without a value column, else one: the rows held cost registers) one barrier reads
-// whether any row did, and if so the workgroup flushes: every bucket holding C slots or
-// more past lo writes its LDS block and the full blocks its overflowing rows fill (those
-// rows write theirs directly), lo moves past them, and the rows left over take their LDS
-// positions (the next chunks' rows insert at later slots, whose positions cannot collide
-// with theirs).  A flush is one more barrier and a few LDS round trips during which no
+// The tile strategy's scan.  Bucket b's slots are numbered from its LDS block: slots
+// [0, C) are the block's positions, and a flush that writes k blocks out takes k C off the
+// count.  Each row takes the next slot of its tile's bucket (an LDS atomic); a row whose
+// slot is past the LDS block keeps it in registers.  After every kSteps chunks (three
+// for float32 coordinates without a value column, else one: the rows held cost registers)
+// one barrier reads whether any row did, and if so the workgroup flushes: every bucket
+// holding C slots or more writes its LDS block and the full blocks its overflowing rows
+// fill (those rows write theirs directly), its count drops by the slots written, and the
+// rows left over take their LDS positions (the next chunks' rows insert at later slots,
+// whose positions cannot collide with theirs).  A flush is one more barrier and a few LDS round trips during which no
 // row is binned, so they are batched: few checks, every LDS read of a phase issued
 // together, the full blocks of a wave's buckets copied by all its lanes.  (A
 // barrier-free form, in which the lane completing a block writes it out alone, measured
@@ -1267,7 +1267,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     extern __shared__ __align__(16) uint8_t smem[];
     // per bucket (+ kBktSpare spare words, which dropped rows and idle lanes address)
     uint32_t *cnt = reinterpret_cast<uint32_t *>(smem + s.o_cnt);     // slots taken
-    uint32_t *lo = reinterpret_cast<uint32_t *>(smem + s.o_lo);       // first slot in LDS
+    uint32_t *lo = reinterpret_cast<uint32_t *>(smem + s.o_lo);       // a flush: slots written
     uint32_t *bbase = reinterpret_cast<uint32_t *>(smem + s.o_base);  // a flush: block of slot v
                                                                       // is bbase + v / C
     uint32_t *xs = reinterpret_cast<uint32_t *>(smem + s.o_bk);       // end: a scan's scratch
@@ -1320,10 +1320,9 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
         for (int o = 0; o < kBktOwn; ++o) {
             if (o * kScanThreads >= s.nb) break;  // uniform
             const int b = tid + o * kScanThreads;
-            uint32_t l = 0u, k = 0u;
+            uint32_t n = 0u, k = 0u;
             if (b < s.nb) {
-                l = lo[b];
-                const uint32_t n = cnt[b] - l;
+                n = cnt[b];
                 k = n >= C ? n >> cs : 0u;
             }
             // blocks: the lanes with one block take consecutive ones after one atomic for
@@ -1342,9 +1341,10 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
             const unsigned long long mf = __ballot(k != 0u);
             if (k != 0u) {
                 const uint32_t r0 = tblk[b];
-                bbase[b] = q0 - (l >> cs);  // the block of slot v is bbase + v / C
+                bbase[b] = q0;          // the block of a held row's slot v is bbase + v / C
                 tblk[b] = r0 + k;
-                lo[b] = l + (k << cs);  // only owners read lo during a flush
+                lo[b] = k << cs;        // held rows below this slot go to those blocks
+                cnt[b] = n - (k << cs); // (no row inserts during this phase)
                 for (uint32_t q = 0; q < k; ++q)
                     if (q0 + q < s.bcap)  // always: bcap bounds a workgroup's blocks
                         s.hdr[g0 + q0 + q] = (unsigned long long)b | ((unsigned long long)(r0 + q) << 32);
@@ -1379,7 +1379,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
         }
         __syncthreads();
         // B. a row held goes to the full block of its slot (straight to the stream) when
-        //    the slot is below the new lo, else to the front of the bucket's next LDS block
+        //    the slot is below k C, else to the front of the bucket's next LDS block
         //    (position slot % C)
         if constexpr (!HAS_VAL) {
             // a wave gathers its held rows (about one in ten) into its 64-entry list and
@@ -1451,9 +1451,9 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
                 pend[i] = kNoSlot;
             }
         }
-        // no barrier: the next chunks' rows insert at slots past every one just placed, and
-        // one that reads the old lo holds its row (slot - old lo >= k C), so no LDS position
-        // collides; the next check's barrier orders these writes before the next flush
+        // no barrier: the next chunks' rows take slots past every one just placed (phase A
+        // lowered the counts before the barrier), so no LDS position collides; the next
+        // check's barrier orders these writes before the next flush
     };
     const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
     using R = Rows<TC, TV, HAS_VAL, CK>;
@@ -1477,7 +1477,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
         // three passes over the chunk's rows, so that the LDS round trips of all its rows
         // are in flight together: bin every row (a dropped row takes its lane's spare
         // bucket, whose slots are never stored), take every slot, then store
-        uint32_t tt[kItems], ss[kItems], ll[kItems];
+        uint32_t tt[kItems], ss[kItems];
         uint16_t oo[kItems];
 #pragma unroll
         for (int it = 0; it < kItems; ++it) {
@@ -1514,21 +1514,18 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
             oo[it] = (uint16_t)(cell & omask);
         }
 #pragma unroll
-        for (int it = 0; it < kItems; ++it) {
-            ll[it] = lo[tt[it]];  // current: lo changes only inside a flush
-            ss[it] = atomicAdd(&cnt[tt[it]], 1u);
-        }
+        for (int it = 0; it < kItems; ++it) ss[it] = atomicAdd(&cnt[tt[it]], 1u);
         // without branches: a row that does not go to its LDS block (dropped, or held:
         // its slot is past the block) writes its lane's spare position instead
 #pragma unroll
         for (int it = 0; it < kItems; ++it) {
             const uint32_t t = tt[it];
             const bool keep = t < (uint32_t)s.nb;
-            const bool inl = keep & (ss[it] - ll[it] < C);
+            const bool inl = keep & (ss[it] < C);
             const bool held = keep & !inl;
             boff[inl ? lds_at(t, ss[it]) : off_spare] = oo[it];
             if constexpr (HAS_VAL) bval[inl ? lds_at(t, ss[it]) : val_spare] = cur.v[it];
-            pend[pb + it] = held ? ss[it] : kNoSlot;  // the slot itself: lo may move before the flush
+            pend[pb + it] = held ? ss[it] : kNoSlot;
             ptag[pb + it] = (t << 16) | oo[it];
             if constexpr (HAS_VAL) pv[pb + it] = cur.v[it];
             over |= held;
@@ -1566,7 +1563,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     __syncthreads();
     // the buckets left: one partial block each (every bucket holds < C slots past lo)
     for (int b = tid; b < s.nb; b += kScanThreads) {
-        const uint32_t n = cnt[b] - lo[b];
+        const uint32_t n = cnt[b];
         if (n > 0u) {
             const uint32_t q0 = atomicAdd(&ctrl[0], 1u);
             if (q0 < s.bcap) s.hdr[g0 + q0] = (unsigned long long)b | ((unsigned long long)tblk[b] << 32);
@@ -1578,7 +1575,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     const uint32_t nfl = ctrl[1];
     for (uint32_t d = tid; d < (nfl << cs); d += kScanThreads) {  // one entry per thread
         const uint2 e = fl[d >> cs];
-        const uint32_t w = d & cmask, n = cnt[e.x] - lo[e.x];
+        const uint32_t w = d & cmask, n = cnt[e.x];
         if (e.y >= s.bcap) continue;
         uint8_t *dst = blk(e.y);
         reinterpret_cast<uint16_t *>(dst)[w] = w < n ? boff[(e.x << cs) + w] : kBktUnused;
