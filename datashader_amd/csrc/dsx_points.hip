@@ -1696,16 +1696,24 @@ __global__ __launch_bounds__(kTileThreads) void btile_agg_kernel(PartArgs p, Bkt
     for (unsigned long long bb = b; bb < e; bb += s.batch) {
         const uint32_t nbb = (uint32_t)(e - bb < (unsigned long long)s.batch ? e - bb : s.batch);
         __syncthreads();  // the region and ex / lpg are written, the last batch's ids are read
-        for (uint32_t i = threadIdx.x; i < nbb; i += kTileThreads) {
-            const uint32_t v = (uint32_t)(bb - tb) + i;  // the block's index in the tile
-            int lo_ = 0, hi_ = G;  // the last workgroup whose run starts at or before v
-            while (hi_ - lo_ > 1) {
-                const int mid = (lo_ + hi_) >> 1;
-                if (ex[mid] <= v) lo_ = mid;
+        // each thread stages a run of consecutive ids: one binary search for the first
+        // (the last workgroup whose run starts at or before it), then a forward walk
+        const uint32_t per = (nbb + kTileThreads - 1) / kTileThreads;
+        const uint32_t i0 = threadIdx.x * per;
+        if (i0 < nbb) {
+            uint32_t v = (uint32_t)(bb - tb) + i0;  // the block's index in the tile
+            int g = 0, hi_ = G;
+            while (hi_ - g > 1) {
+                const int mid = (g + hi_) >> 1;
+                if (ex[mid] <= v) g = mid;
                 else hi_ = mid;
             }
-            const size_t g0 = (size_t)lo_ * s.bcap;
-            sblk[i] = (uint32_t)(g0 + s.wl[g0 + lpg[lo_] + (v - ex[lo_])]);
+            const uint32_t i1 = i0 + per < nbb ? i0 + per : nbb;
+            for (uint32_t i = i0; i < i1; ++i, ++v) {
+                while (g + 1 < G && ex[g + 1] <= v) ++g;  // (empty runs are skipped)
+                const size_t gb = (size_t)g * s.bcap;
+                sblk[i] = (uint32_t)(gb + s.wl[gb + lpg[g] + (v - ex[g])]);
+            }
         }
         __syncthreads();
         const uint32_t ngrp = nbb << (s.cshift - 2u);  // groups of four entries
