@@ -74,11 +74,12 @@ class Pass:
         return f"Pass(val={self.val!r}, cat={self.cat!r}, planes={sorted(self.planes)}, m2={self.m2})"
 
 
-def plan(agg):
+def plan(agg, packed=frozenset()):
     """Group the leaves of ``agg`` into passes; returns (passes, [(name, red, pass)]).
     Row-index leaves (first, last, where, the row-index selectors) get the key
     ``("rowindex", i)``: they run their own kernel (``row_index_leaf``), after the passes
-    (a where over max / min adds that plane to the pass of its selector's column)."""
+    (a where over max / min adds that plane to the pass of its selector's column, unless
+    the column is in ``packed``: float32 columns, whose where takes one packed pass)."""
     passes = {}
     leaves = []
     for i, (name, red) in enumerate(rd.leaves(agg)):
@@ -86,7 +87,7 @@ def plan(agg):
         inner = red.reduction if isinstance(red, rd.by) else red
         if getattr(inner, "row_index", False):
             sel = inner.selector if isinstance(inner, rd.where) else inner
-            if isinstance(sel, (rd.max, rd.min)):
+            if isinstance(sel, (rd.max, rd.min)) and sel.column not in packed:
                 key = (sel.column, cat)
                 p = passes.get(key)
                 if p is None:
@@ -449,6 +450,21 @@ def make_view(canvas, x_range, y_range):
                 xlog=canvas.x_axis.log, ylog=canvas.y_axis.log, x_st=x_st, y_st=y_st)
 
 
+def _packed_columns(frame, agg):
+    """The float32 columns of where(max / min) and the max / min row-index selectors: they
+    take dsx_points_rowindex's packed pass (no max / min plane)."""
+    t = dev.torch()
+    out = set()
+    for _, red in rd.leaves(agg):
+        inner = red.reduction if isinstance(red, rd.by) else red
+        if not getattr(inner, "row_index", False):
+            continue
+        sel = inner.selector if isinstance(inner, rd.where) else inner
+        if isinstance(sel, (rd.max, rd.min)) and frame[sel.column].dtype == t.float32:
+            out.add(sel.column)
+    return frozenset(out)
+
+
 def row_index_leaf(lib, frame, schema, glyph, view, red, passes, results, H, W):
     """The finalized array of a row-index leaf: the selected row of every cell
     (dsx_points_rowindex) and, with a lookup column, that row's value (dsx_take)."""
@@ -474,12 +490,15 @@ def row_index_leaf(lib, frame, schema, glyph, view, red, passes, results, H, W):
     r = _abi.RowIndexDesc()
     key = None
     valcol = sel.column
+    packed = False
     if isinstance(sel, (rd.max, rd.min)):
         r.mode = _abi.RI_ARGMAX if isinstance(sel, rd.max) else _abi.RI_ARGMIN
-        res = results[(sel.column, cat)]
-        key = res.planes["max" if isinstance(sel, rd.max) else "min"]
-        r.key = dev.ptr(key)
         r.nan_check = 1
+        packed = frame[sel.column].dtype == t.float32
+        if not packed:  # float64 values: the key plane of the max / min pass
+            res = results[(sel.column, cat)]
+            key = res.planes["max" if isinstance(sel, rd.max) else "min"]
+            r.key = dev.ptr(key)
     else:
         r.mode = sel.ri_mode
         r.nan_check = 1 if valcol is not None else 0
@@ -492,8 +511,9 @@ def row_index_leaf(lib, frame, schema, glyph, view, red, passes, results, H, W):
     vw = _view_struct(view)
     if key is None and getattr(frame, "persistent", False):
         # the LDS-window pass keeps its hot window per (frame, canvas, view) like the
-        # points pass (a window of one 32-bit row per cell)
-        wkey = ("rowindex", glyph.x, glyph.y, valcol, None if cat is None else cat._key(),
+        # points pass (a window of one 32-bit row per cell, or 64-bit packed cells)
+        wkey = ("rowindex_packed" if packed else "rowindex", glyph.x, glyph.y, valcol,
+                None if cat is None else cat._key(),
                 W, H, tuple(view[k] for k in ("xmin", "xmax", "ymin", "ymax")))
         slot, reuse, _ = _window_slot(frame, wkey, device)
         d.window, d.reuse_window = dev.ptr(slot.rec), reuse
@@ -571,7 +591,7 @@ def aggregate(frame, schema, canvas, glyph, agg):
     canvas.validate_ranges(x_range, y_range)
     view = make_view(canvas, x_range, y_range)
     H, W = view["H"], view["W"]
-    passes, leaves = plan(agg)
+    passes, leaves = plan(agg, _packed_columns(frame, agg))
     results = {key: run_pass(lib, frame, schema, glyph, view, p, H, W)
                for key, p in passes.items()}
     out = []

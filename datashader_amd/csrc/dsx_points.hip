@@ -2813,17 +2813,23 @@ static int rowindex_launch(const RiArgs &r, int val_dtype, hipStream_t st) {
 // skip test.  A merge then folds the workgroups' windows into the index plane (the global
 // atomics never touch a window cell).  Rows are < 2^32 (checked on the host), so a row fits
 // 32 bits: LAST keeps row + 1, so that 0 is the empty cell.
+// ARGMAX / ARGMIN over a float32 value column without a key plane (PK): one such pass over
+// 64-bit cells holding key(v) << 32 | row (ARGMIN, atomic min) or key(v) << 32 | ~row
+// (ARGMAX, atomic max): the extreme value wins and, among its rows, the smallest -- the
+// row the reference keeps (its strict > / < appends, reductions.py:1160-1260 and
+// 2236-2372) -- so the max / min pass and the key-matching pass become one.
 constexpr int kRiwThreads = 1024;
 constexpr int kRiwItems = 4;
 constexpr int kRiwChunk = kRiwThreads * kRiwItems;
 
 struct RiwArgs {
     PointArgs a;
-    int32_t last, nan_check;
+    int32_t last, nan_check;      // (packed: last = ARGMAX)
     unsigned long long *index;
     const int32_t *window;
     uint32_t cap;                 // window cells (pixels x categories)
-    uint32_t *partials;           // [G][cap]
+    uint32_t *partials;           // [G][cap] (packed: 64-bit cells)
+    int32_t packed;
 };
 
 struct RiwPlan {
@@ -2832,10 +2838,11 @@ struct RiwPlan {
     size_t lds, o_window, o_hist, o_partials, total;
 };
 
-static int riw_plan(const dsx_points_desc *d, const dsx_view *v, RiwPlan &P) {
+static int riw_plan(const dsx_points_desc *d, const dsx_view *v, RiwPlan &P, bool packed = false) {
     const int ncat = d->cat.kind == DSX_CAT_NONE ? 1 : d->cat.ncat;
+    const size_t cb = packed ? 8 : 4;  // bytes a window cell
     P.lds = (size_t)kLdsBytes - 1024;  // the window, less the kernel's static words
-    P.cap = (uint32_t)(P.lds / 4);
+    P.cap = (uint32_t)(P.lds / cb);
     if ((int64_t)P.cap / ncat < 1) return DSX_ERR_SHAPE;
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
@@ -2854,24 +2861,30 @@ static int riw_plan(const dsx_points_desc *d, const dsx_view *v, RiwPlan &P) {
     size_t o = 0;
     P.o_window = o; o = align_up(o + 4 * kRecWords, 256);
     P.o_hist = o; o = align_up(o + 4 * (size_t)kSelBins, 256);
-    P.o_partials = o; o = align_up(o + 4 * (size_t)P.cap * (size_t)P.groups, 256);
+    P.o_partials = o; o = align_up(o + cb * (size_t)P.cap * (size_t)P.groups, 256);
     P.total = o;
     return DSX_OK;
 }
 
 // NANCHK: rows whose value is NaN are skipped (the value column is read); CK, LIN: as
 // in scan_kernel (the category read with the row's other columns, the branch-free linear
-// projection).
-template <typename TC, typename TV, bool NANCHK, int CK, bool LIN>
+// projection); PK: the packed ARGMAX / ARGMIN form (float32 values).
+template <typename TC, typename TV, bool NANCHK, int CK, bool LIN, bool PK = false>
 __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r) {
     static_assert(kRiwThreads == kScanThreads && kRiwItems == kItems, "chunks as the scan's");
+    static_assert(!PK || (NANCHK && sizeof(TV) == 4), "packed: float32 values, NaN skipped");
     extern __shared__ __align__(16) uint32_t win[];
+    unsigned long long *const win64 = reinterpret_cast<unsigned long long *>(win);
     const PointArgs &a = r.a;
     const int ncat = a.cat.ncat;
     const int tid = threadIdx.x;
     const Window w = load_window(r.window, a.W, a.H, (int64_t)(r.cap / (uint32_t)ncat));
     const uint32_t empty = r.last ? 0u : 0xFFFFFFFFu;
-    for (uint32_t c = tid; c < r.cap; c += kRiwThreads) win[c] = empty;
+    const unsigned long long empty64 = r.last ? 0ull : ~0ull;
+    for (uint32_t c = tid; c < r.cap; c += kRiwThreads) {
+        if constexpr (PK) win64[c] = empty64;
+        else win[c] = empty;
+    }
     __syncthreads();
     const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
     // FIRST visits the chunks in increasing row order and LAST in decreasing order, so
@@ -2894,6 +2907,7 @@ __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r)
         const uint32_t rem = rem64 >= kChunk ? (uint32_t)kChunk : (rem64 > 0 ? (uint32_t)rem64 : 0u);
         int64_t cell[kRiwItems];
         long long cur[kRiwItems];
+        unsigned long long pkv[kRiwItems];  // PK: the row's packed value
         bool out[kRiwItems];
 #pragma unroll
         for (int it = 0; it < kRiwItems; ++it) {
@@ -2921,11 +2935,19 @@ __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r)
             cell[it] = 0;
             cur[it] = 0;
             const uint32_t row = (uint32_t)(b + j);  // rows < 2^32 (checked on the host)
+            pkv[it] = 0ull;
+            if constexpr (PK)
+                pkv[it] = ((unsigned long long)fkey32((float)S.v[it]) << 32) | (r.last ? ~row : row);
             if (!keep) continue;
             if (in_window(w, px, py)) {
                 const uint32_t lc = (uint32_t)(((py - w.y0) * w.w + (px - w.x0)) * ncat + pc);
-                if (r.last) atomicMax(&win[lc], row + 1u);
-                else atomicMin(&win[lc], row);
+                if constexpr (PK) {
+                    if (r.last) atomicMax(&win64[lc], pkv[it]);
+                    else atomicMin(&win64[lc], pkv[it]);
+                } else {
+                    if (r.last) atomicMax(&win[lc], row + 1u);
+                    else atomicMin(&win[lc], row);
+                }
                 continue;
             }
             out[it] = true;
@@ -2941,6 +2963,18 @@ __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r)
 #pragma unroll
         for (int it = 0; it < kRiwItems; ++it) {
             if (!out[it]) continue;
+            if constexpr (PK) {  // unsigned: the plane starts at 0 (ARGMAX) / all ones (ARGMIN)
+                const unsigned long long c_ = (unsigned long long)cur[it];
+                if (r.last) {
+                    if (pkv[it] > c_)
+                        __hip_atomic_fetch_max(index + cell[it], pkv[it], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                } else if (pkv[it] < c_) {
+                    __hip_atomic_fetch_min(index + cell[it], pkv[it], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                }
+                continue;
+            }
             const long long row = (long long)(b + it * kRiwThreads + tid);
             if (r.last) {
                 if (row > cur[it])
@@ -2963,9 +2997,15 @@ __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r)
         step(k + 2 * G, sc, sb);
     }
     __syncthreads();
-    uint32_t *mine = r.partials + (size_t)blockIdx.x * r.cap;
     const uint32_t used = (uint32_t)(w.w * w.h * ncat);
-    for (uint32_t c = tid; c < used; c += kRiwThreads) mine[c] = win[c];
+    if constexpr (PK) {
+        unsigned long long *mine = reinterpret_cast<unsigned long long *>(r.partials) +
+                                   (size_t)blockIdx.x * r.cap;
+        for (uint32_t c = tid; c < used; c += kRiwThreads) mine[c] = win64[c];
+    } else {
+        uint32_t *mine = r.partials + (size_t)blockIdx.x * r.cap;
+        for (uint32_t c = tid; c < used; c += kRiwThreads) mine[c] = win[c];
+    }
 }
 
 // The window cells of the index plane: min (FIRST) / max (LAST) over the workgroups.
@@ -2974,6 +3014,28 @@ __global__ __launch_bounds__(256) void rowindex_window_merge(RiwArgs r, int grou
     const int ncat = a.cat.ncat;
     const Window w = load_window(r.window, a.W, a.H, (int64_t)(r.cap / (uint32_t)ncat));
     const uint32_t used = (uint32_t)(w.w * w.h * ncat);
+    if (r.packed) {  // the packed plane's window cells: max (ARGMAX) / min (ARGMIN)
+        const unsigned long long *part = reinterpret_cast<const unsigned long long *>(r.partials);
+        for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < used; c += gridDim.x * blockDim.x) {
+            unsigned long long m = r.last ? 0ull : ~0ull;
+            int q = 0;
+            for (; q + 8 <= groups; q += 8) {
+                unsigned long long x[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) x[u] = part[(size_t)(q + u) * r.cap + c];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) m = r.last ? (x[u] > m ? x[u] : m) : (x[u] < m ? x[u] : m);
+            }
+            for (; q < groups; ++q) {
+                const unsigned long long x = part[(size_t)q * r.cap + c];
+                m = r.last ? (x > m ? x : m) : (x < m ? x : m);
+            }
+            const uint32_t pix = c / (uint32_t)ncat, cat = c - pix * (uint32_t)ncat;
+            const int32_t ly = (int32_t)(pix / (uint32_t)w.w), lx = (int32_t)(pix - (uint32_t)ly * w.w);
+            r.index[((int64_t)(w.y0 + ly) * a.W + (w.x0 + lx)) * ncat + cat] = m;
+        }
+        return;
+    }
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < used; c += gridDim.x * blockDim.x) {
         uint32_t m = r.last ? 0u : 0xFFFFFFFFu;
         auto fold = [&](uint32_t x) { m = r.last ? (x > m ? x : m) : (x < m ? x : m); };
@@ -2993,6 +3055,18 @@ __global__ __launch_bounds__(256) void rowindex_window_merge(RiwArgs r, int grou
         if (r.last) out = m == 0u ? -1ll : (long long)(m - 1u);
         else out = m == 0xFFFFFFFFu ? -1ll : (long long)m;
         r.index[g] = (unsigned long long)out;
+    }
+}
+
+// The packed plane to rows: the low word (ARGMIN) or its complement (ARGMAX), -1 where no
+// row was kept (the plane's start value).
+__global__ __launch_bounds__(256) void rowindex_unpack_kernel(unsigned long long *p, int64_t n, int amax) {
+    const unsigned long long empty = amax ? 0ull : ~0ull;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long v = p[i];
+        const uint32_t lo32 = (uint32_t)v;
+        p[i] = v == empty ? ~0ull : (unsigned long long)(amax ? ~lo32 : lo32);
     }
 }
 
@@ -3042,6 +3116,13 @@ static int riw_launch(const dsx_points_desc *d, const RiwArgs &r, const RiwPlan 
                            : rowindex_window_kernel<TC, double, true, CK_ANY, false>)
                    : rowindex_window_kernel<TC, double, false, CK_ANY, false>;
     }
+    if (r.packed) {  // float32 values (checked by the caller), NaN rows skipped
+        if (a.cat.kind == DSX_CAT_NONE)
+            kern = lin ? rowindex_window_kernel<TC, float, true, CK_NONE, true, true>
+                       : rowindex_window_kernel<TC, float, true, CK_NONE, false, true>;
+        else if (byte) kern = rowindex_window_kernel<TC, float, true, CK_CODE8, false, true>;
+        else kern = rowindex_window_kernel<TC, float, true, CK_ANY, false, true>;
+    }
     const hipError_t e = hipFuncSetAttribute((const void *)kern,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds);
     if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
@@ -3049,14 +3130,27 @@ static int riw_launch(const dsx_points_desc *d, const RiwArgs &r, const RiwPlan 
     if ((s = launch_status())) return s;
     hipLaunchKernelGGL(rowindex_window_merge, dim3(grid_for(P.cap, 256, 4)), dim3(256), 0, st, r,
                        P.groups);
+    if (r.packed) {
+        if ((s = launch_status())) return s;
+        const int64_t ncells = (int64_t)a.W * a.H * a.cat.ncat;
+        hipLaunchKernelGGL(rowindex_unpack_kernel, dim3(grid_for(ncells, 256, 4)), dim3(256), 0, st,
+                           r.index, ncells, r.last);
+    }
     return launch_status();
+}
+
+// ARGMAX / ARGMIN take the packed window pass when no key plane is given, with a
+// workspace, over a float32 value column.
+static bool ri_packed(const dsx_points_desc *d, const dsx_rowindex_desc *r) {
+    return (r->mode == DSX_RI_ARGMAX || r->mode == DSX_RI_ARGMIN) && !r->key && has_value(d) &&
+           d->val_dtype == DSX_F32;
 }
 
 extern "C" size_t dsx_points_rowindex_workspace_size(const dsx_points_desc *d, const dsx_view *v,
                                                      const dsx_rowindex_desc *r) {
     if (check_points(d, v) || !r) return 0;
     RiwPlan P;
-    if (riw_plan(d, v, P)) return 0;
+    if (riw_plan(d, v, P, ri_packed(d, r))) return 0;
     return P.total;
 }
 
@@ -3069,10 +3163,14 @@ extern "C" int dsx_points_rowindex(const dsx_points_desc *d, const dsx_view *v,
     if (r->mode < DSX_RI_FIRST || r->mode > DSX_RI_ARGMIN) return DSX_ERR_MODE;
     const bool arg = r->mode == DSX_RI_ARGMAX || r->mode == DSX_RI_ARGMIN;
     if ((r->nan_check || arg) && !has_value(d)) return DSX_ERR_ACCUM;
-    if (arg && !r->key) return DSX_ERR_NULL;
+    const bool packed = ri_packed(d, r) && workspace;
+    if (arg && !r->key && !packed) return DSX_ERR_NULL;
     const int64_t ncells = (int64_t)v->width * v->height * cat_of(d->cat).ncat;
     hipStream_t s = as_stream(stream);
-    if ((st = fill_async(r->index, DSX_FILL_ONES64, ncells, s))) return st;
+    // (the packed ARGMAX plane starts at 0, below every packed value)
+    const int fill = packed && r->mode == DSX_RI_ARGMAX ? DSX_FILL_ZERO64 : DSX_FILL_ONES64;
+    if ((st = fill_async(r->index, fill, ncells, s))) return st;
+    if (d->n == 0 && packed) return fill_async(r->index, DSX_FILL_ONES64, ncells, s);
     if (d->n == 0) return DSX_OK;
     RiArgs ra;
     ra.a = args_of(d, v);
@@ -3081,15 +3179,16 @@ extern "C" int dsx_points_rowindex(const dsx_points_desc *d, const dsx_view *v,
     ra.key = r->key;
     ra.index = reinterpret_cast<unsigned long long *>(r->index);
     const int vdt = has_value(d) ? d->val_dtype : DSX_F64;
-    if (!arg && workspace) {
-        // FIRST / LAST with a workspace: the LDS-window pass
+    if ((!arg && workspace) || packed) {
+        // FIRST / LAST with a workspace, or packed ARGMAX / ARGMIN: the LDS-window pass
         RiwPlan P;
-        if ((st = riw_plan(d, v, P))) return st;
+        if ((st = riw_plan(d, v, P, packed))) return st;
         if (workspace_bytes < P.total) return DSX_ERR_WORKSPACE;
         uint8_t *ws = reinterpret_cast<uint8_t *>(workspace);
         RiwArgs rw;
         rw.a = ra.a;
-        rw.last = r->mode == DSX_RI_LAST ? 1 : 0;
+        rw.last = (r->mode == DSX_RI_LAST || r->mode == DSX_RI_ARGMAX) ? 1 : 0;
+        rw.packed = packed ? 1 : 0;
         rw.nan_check = ra.nan_check;
         rw.index = ra.index;
         rw.window = reinterpret_cast<const int32_t *>(ws + P.o_window);

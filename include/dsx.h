@@ -244,14 +244,18 @@ typedef struct {
     int32_t nan_check;            /* FIRST / LAST: skip rows whose value is NaN           */
     const void *key;              /* ARGMAX / ARGMIN: the max / min key plane of the value
                                      column as dsx_points_agg wrote it (key width = value
-                                     width)                                               */
+                                     width); NULL with a float32 value column and a
+                                     workspace: one packed pass, no key plane needed      */
     int64_t *index;               /* out, H*W*ncat: the selected row, -1 where none       */
 } dsx_rowindex_desc;
 
 /* FIRST and LAST with a workspace of dsx_points_rowindex_workspace_size bytes take an LDS
  * pass: the rows in a hot window of the canvas (picked from a sample of the rows) keep
- * their cells' rows in LDS, the others take the global atomics; with workspace == NULL (and
- * always for ARGMAX / ARGMIN) one global-atomic pass.  Same results either way. */
+ * their cells' rows in LDS, the others take the global atomics; with workspace == NULL one
+ * global-atomic pass.  ARGMAX / ARGMIN with a key plane take one streaming pass that keeps
+ * the first row matching the key; without one (float32 values, a workspace) the LDS pass
+ * over 64-bit cells key(v) << 32 | row (ARGMIN) or | ~row (ARGMAX), whose atomic min / max
+ * keeps the extreme value's first row.  Same results every way. */
 size_t dsx_points_rowindex_workspace_size(const dsx_points_desc *d, const dsx_view *v,
                                           const dsx_rowindex_desc *r);
 int dsx_points_rowindex(const dsx_points_desc *d, const dsx_view *v,

@@ -206,3 +206,79 @@ def test_windowed_first_last_vs_global_pass_and_oracle(xdt, by):
             outs.append(idx.cpu().numpy().reshape(shape))
         np.testing.assert_array_equal(outs[0], want)
         np.testing.assert_array_equal(outs[1], want)
+
+
+@pytest.mark.parametrize("by", [False, True], ids=["plain", "by-cat"])
+def test_packed_where_max_min_vs_oracle(by):
+    """where(max / min) over a float32 value column on a canvas much larger than the LDS
+    window (1920 x 1080, a hot 0.01-sigma blob plus uniform rows): one packed pass (64-bit
+    cells key(v) << 32 | row, or | ~row for max) keeps the extreme value's first row, in
+    the window and through the global atomics.  Integer values give many ties per cell;
+    NaN rows are skipped; bit-exact vs the oracle's closed form, also through the C ABI
+    (no key plane, with a workspace) and with an empty input."""
+    import torch
+
+    from datashader_amd import _abi, backend
+    from datashader_amd import _device as dev
+    from datashader_amd.glyphs import Point
+
+    rng = np.random.default_rng(23)
+    n = 3_000_000
+    hot = rng.random(n) < 0.9
+    x = np.where(hot, rng.normal(0.5, 0.01, n), rng.uniform(0, 1, n)).astype(np.float32)
+    y = np.where(hot, rng.normal(0.5, 0.01, n), rng.uniform(0, 1, n)).astype(np.float32)
+    v = rng.integers(-20, 20, n).astype(np.float32)
+    v[rng.random(n) < 0.02] = np.nan
+    w = rng.normal(size=n)
+    cat = pd.Categorical(rng.choice(list("pqrs"), n), categories=list("pqrs"))
+    df = pd.DataFrame({"x": x, "y": y, "v": v, "w": w, "cat": cat})
+    W, H = 1920, 1080
+    cvs = dsa.Canvas(W, H, x_range=(0, 1), y_range=(0, 1))
+    src = dsa.DeviceFrame.from_pandas(df)
+    view = G.view_for(W, H, (0, 1), (0, 1))
+    bins = cf.point_bins(x.astype(np.float64), y.astype(np.float64), view, W, H)
+    ncat = 4 if by else 1
+    cell = ri.cells(bins, df["cat"].cat.codes.to_numpy() if by else None, ncat)
+    ncells = W * H * ncat
+    shape = (H, W, ncat) if by else (H, W)
+
+    def wrap(r):
+        return dsa.by("cat", r) if by else r
+
+    got = cvs.points(src, "x", "y", dsa.summary(
+        mx=wrap(dsa.where(dsa.max("v"))), mn=wrap(dsa.where(dsa.min("v"), "w"))))
+    vv = v.astype(np.float64)
+    want_mx = ri.selected_rows_closed_form(cell, ncells, "max", vv).reshape(shape)
+    want_mn = ri.selected_rows_closed_form(cell, ncells, "min", vv).reshape(shape)
+    np.testing.assert_array_equal(got["mx"].values, want_mx)
+    G.assert_same(got["mn"].values, ri.take(want_mn, w))
+    assert (want_mx >= 0).sum() > 10_000  # cells outside the window are covered
+    if by:
+        return
+    lib = _abi.load()
+    p = backend.Pass("v", None)
+    catc, _, _ = backend._categorizer(None, src, None)
+    mview = backend.make_view(cvs, (0, 1), (0, 1))
+    d, keep = backend._pass_desc(src, None, Point("x", "y"), mview, p, src.device, catc)
+    vw = backend._view_struct(mview)
+    for mode, want in ((_abi.RI_ARGMAX, want_mx), (_abi.RI_ARGMIN, want_mn)):
+        for rows in (n, 0):
+            d.n = rows
+            r = _abi.RowIndexDesc()
+            r.mode, r.nan_check = mode, 1
+            idx = torch.empty(ncells, dtype=torch.int64, device="cuda")
+            r.index = dev.ptr(idx)
+            ws = int(lib.dsx_points_rowindex_workspace_size(d, vw, r))
+            assert ws > 0
+            work = torch.empty(ws, dtype=torch.uint8, device="cuda")
+            print(f"packed rowindex mode {mode} rows {rows} ws {ws}", flush=True)
+            _abi.check(lib.dsx_points_rowindex(d, vw, r, dev.ptr(work), ws, dev.stream_handle()),
+                       "dsx_points_rowindex")
+            out = idx.cpu().numpy().reshape(shape)
+            np.testing.assert_array_equal(out, want if rows else np.full(shape, -1))
+        # no key and no workspace: the key plane is required
+        r = _abi.RowIndexDesc()
+        r.mode, r.nan_check = mode, 1
+        r.index = dev.ptr(idx)
+        assert lib.dsx_points_rowindex(d, vw, r, None, 0, dev.stream_handle()) == -1  # DSX_ERR_NULL
+    d.n = n
