@@ -1312,10 +1312,9 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
     const uint32_t val_spare = (s.o_bk - s.o_val) / (uint32_t)sizeof(TV) + (uint32_t)lane;
     const uint32_t lco = cs - 3u;
     const uint32_t lcv = HAS_VAL ? cs + (sizeof(TV) == 8 ? 3u : 2u) - 4u : 0u;
-    auto flush = [&]() {  // every thread calls it, after a check's barrier (lo, cnt stable)
-        // A. every bucket holding C slots or more past lo gets k = (count - lo) / C
-        //    consecutive blocks of the stream, its headers, and its owner lane's copy of its
-        //    LDS block, 16 bytes at a time
+    auto flush = [&]() {  // every thread calls it, after a check's barrier (counts stable)
+        // A. every bucket holding C slots or more gets k = count / C consecutive blocks of
+        //    the stream and their headers; its wave copies its LDS block out
 #pragma unroll
         for (int o = 0; o < kBktOwn; ++o) {
             if (o * kScanThreads >= s.nb) break;  // uniform
@@ -1561,7 +1560,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
         }
     }
     __syncthreads();
-    // the buckets left: one partial block each (every bucket holds < C slots past lo)
+    // the buckets left: one partial block each (every bucket holds < C slots)
     for (int b = tid; b < s.nb; b += kScanThreads) {
         const uint32_t n = cnt[b];
         if (n > 0u) {
