@@ -1,8 +1,11 @@
 // dsx_nselect.hip -- the n-value reductions of the points path: max_n, min_n, first_n,
 // last_n and their row-index forms (reductions.py:1457-1805, 2375-2562).
 //
-// The reference inserts every row into a per-pixel sorted list of n slots.  Here the
-// kept rows are sorted once on the device by (cell, order key) with rocPRIM's stable
+// The reference inserts every row into a per-pixel sorted list of n slots.  Here, for
+// n <= kNpassMax (float32 values for max_n / min_n), n streaming passes each keep one
+// slot of every cell: a row's packed 64-bit order key is distinct per row and orders as
+// the list does, and pass k keeps the best key past the one pass k - 1 kept.  Otherwise
+// the kept rows are sorted once on the device by (cell, order key) with rocPRIM's stable
 // radix sort -- rows that tie on the order key keep their row order, which is the order
 // the reference's strict comparisons leave them in -- and the first n rows of every
 // cell's run are written out.
@@ -104,6 +107,32 @@ __global__ __launch_bounds__(256) void take_runs_kernel(int64_t n, const uint64_
             index[(int64_t)c * nsel + k] = (int64_t)rows[j + k];
         }
     }
+}
+
+// ---- the n-pass form --------------------------------------------------------------------
+// ri_npass (the row-index window pass over packed 64-bit keys) keeps slot k of every cell
+// in pass k; index[cell * n + k] holds the slot's packed key -- MAX key(v) << 32 | ~row,
+// MIN key(v) << 32 | row, FIRST row, LAST ~row (32-bit); 0 (MAX) or all ones: empty --
+// until this turns it into the row.
+constexpr int kNpassMax = 8;
+
+__global__ __launch_bounds__(256) void npass_unpack_kernel(int64_t n, int mode, unsigned long long *index) {
+    const bool best_max = mode == DSX_NS_MAX;
+    const unsigned long long empty = best_max ? 0ull : ~0ull;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long v = index[i];
+        const uint32_t lo32 = (uint32_t)v;
+        const uint32_t row = (best_max || mode == DSX_NS_LAST) ? ~lo32 : lo32;
+        index[i] = v == empty ? ~0ull : (unsigned long long)row;
+    }
+}
+
+// n <= kNpassMax, and float32 values wherever values are read (max_n / min_n, a NaN check)
+bool np_eligible(const dsx_points_desc *d, const dsx_nselect_desc *r) {
+    const bool value = r->mode == DSX_NS_MAX || r->mode == DSX_NS_MIN;
+    const bool reads = value || r->nan_check;
+    return r->n <= kNpassMax && (!reads || (has_value(d) && d->val_dtype == DSX_F32));
 }
 
 struct NsPlan {
@@ -215,6 +244,7 @@ using namespace dsx;
 extern "C" size_t dsx_points_nselect_workspace_size(const dsx_points_desc *d, const dsx_view *v,
                                                     const dsx_nselect_desc *r) {
     if (ns_check(d, v, r)) return 0;
+    if (np_eligible(d, r)) return ri_npass_workspace_size(d, v);
     NsPlan P;
     if (ns_plan(d, v, r, P)) return 0;
     return P.total;
@@ -226,11 +256,21 @@ extern "C" int dsx_points_nselect(const dsx_points_desc *d, const dsx_view *v,
     int s = ns_check(d, v, r);
     if (s) return s;
     if (!r->index) return DSX_ERR_NULL;
+    hipStream_t st = as_stream(stream);
+    const int64_t cells = (int64_t)v->width * v->height * cat_of(d->cat).ncat * r->n;
+    if (np_eligible(d, r)) {
+        if ((s = fill_async(r->index, DSX_FILL_ONES64, cells, st))) return s;
+        if (d->n == 0) return DSX_OK;
+        if (r->mode == DSX_NS_MAX && (s = fill_async(r->index, DSX_FILL_ZERO64, cells, st))) return s;
+        if ((s = ri_npass(d, v, r->mode, r->n, r->nan_check, r->index, workspace, workspace_bytes, st)))
+            return s;
+        hipLaunchKernelGGL(npass_unpack_kernel, dim3(grid_for(cells, 256, 4)), dim3(256), 0, st, cells,
+                           r->mode, reinterpret_cast<unsigned long long *>(r->index));
+        return launch_status();
+    }
     NsPlan P;
     if ((s = ns_plan(d, v, r, P))) return s;
     if (!workspace || workspace_bytes < P.total) return DSX_ERR_WORKSPACE;
-    hipStream_t st = as_stream(stream);
-    const int64_t cells = (int64_t)v->width * v->height * cat_of(d->cat).ncat * r->n;
     if ((s = fill_async(r->index, DSX_FILL_ONES64, cells, st))) return s;
     if (d->n == 0) return DSX_OK;
     uint8_t *ws = reinterpret_cast<uint8_t *>(workspace);

@@ -2823,12 +2823,16 @@ constexpr int kRiwChunk = kRiwThreads * kRiwItems;
 
 struct RiwArgs {
     PointArgs a;
-    int32_t last, nan_check;      // (packed: last = ARGMAX)
-    unsigned long long *index;
+    int32_t last, nan_check;      // (packed: last = the largest packed key wins)
+    unsigned long long *index;    // cell c's slot: index[c * nsel + kslot]
     const int32_t *window;
     uint32_t cap;                 // window cells (pixels x categories)
     uint32_t *partials;           // [G][cap] (packed: 64-bit cells)
     int32_t packed;
+    int32_t pkmode;               // packed key: 0 key(v) << 32 | row (~row when last),
+                                  // 1 row, 2 ~row (32-bit)
+    int32_t nsel, kslot;
+    int32_t thr;                  // packed: keep only keys past the previous slot's (kslot > 0)
 };
 
 struct RiwPlan {
@@ -2837,9 +2841,11 @@ struct RiwPlan {
     size_t lds, o_window, o_hist, o_partials, total;
 };
 
-static int riw_plan(const dsx_points_desc *d, const dsx_view *v, RiwPlan &P, bool packed = false) {
+static int riw_plan(const dsx_points_desc *d, const dsx_view *v, RiwPlan &P, bool packed = false,
+                    bool thr = false) {
     const int ncat = d->cat.kind == DSX_CAT_NONE ? 1 : d->cat.ncat;
-    const size_t cb = packed ? 8 : 4;  // bytes a window cell
+    // bytes a window cell (packed with a threshold: the previous slot's key beside it)
+    const size_t cb = packed ? (thr ? 16 : 8) : 4;
     P.lds = (size_t)kLdsBytes - 1024;  // the window, less the kernel's static words
     P.cap = (uint32_t)(P.lds / cb);
     if ((int64_t)P.cap / ncat < 1) return DSX_ERR_SHAPE;
@@ -2871,18 +2877,32 @@ static int riw_plan(const dsx_points_desc *d, const dsx_view *v, RiwPlan &P, boo
 template <typename TC, typename TV, bool NANCHK, int CK, bool LIN, bool PK = false>
 __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r) {
     static_assert(kRiwThreads == kScanThreads && kRiwItems == kItems, "chunks as the scan's");
-    static_assert(!PK || (NANCHK && sizeof(TV) == 4), "packed: float32 values, NaN skipped");
+    static_assert(!PK || sizeof(TV) == 4, "packed: float32 values");
     extern __shared__ __align__(16) uint32_t win[];
     unsigned long long *const win64 = reinterpret_cast<unsigned long long *>(win);
+    unsigned long long *const thr64 = win64 + r.cap;  // packed with a threshold
     const PointArgs &a = r.a;
     const int ncat = a.cat.ncat;
     const int tid = threadIdx.x;
     const Window w = load_window(r.window, a.W, a.H, (int64_t)(r.cap / (uint32_t)ncat));
+    const uint32_t used = (uint32_t)(w.w * w.h * ncat);
     const uint32_t empty = r.last ? 0u : 0xFFFFFFFFu;
     const unsigned long long empty64 = r.last ? 0ull : ~0ull;
+    const bool thr = PK && r.thr && r.kslot > 0;
+    // packed: a is kept over b (the larger key wins when last, else the smaller)
+    auto better = [&](unsigned long long x, unsigned long long y) { return r.last ? x > y : x < y; };
     for (uint32_t c = tid; c < r.cap; c += kRiwThreads) {
-        if constexpr (PK) win64[c] = empty64;
-        else win[c] = empty;
+        if constexpr (PK) {
+            win64[c] = empty64;
+            if (thr && c < used) {  // the window cell's previous slot
+                const uint32_t pix = c / (uint32_t)ncat, cat = c - pix * (uint32_t)ncat;
+                const int32_t ly = (int32_t)(pix / (uint32_t)w.w), lx = (int32_t)(pix - (uint32_t)ly * w.w);
+                const int64_t g = ((int64_t)(w.y0 + ly) * a.W + (w.x0 + lx)) * ncat + cat;
+                thr64[c] = r.index[g * r.nsel + r.kslot - 1];
+            }
+        } else {
+            win[c] = empty;
+        }
     }
     __syncthreads();
     const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
@@ -2907,6 +2927,7 @@ __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r)
         int64_t cell[kRiwItems];
         long long cur[kRiwItems];
         unsigned long long pkv[kRiwItems];  // PK: the row's packed value
+        unsigned long long pth[kRiwItems];  // PK with a threshold: the previous slot's key
         bool out[kRiwItems];
 #pragma unroll
         for (int it = 0; it < kRiwItems; ++it) {
@@ -2935,12 +2956,15 @@ __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r)
             cur[it] = 0;
             const uint32_t row = (uint32_t)(b + j);  // rows < 2^32 (checked on the host)
             pkv[it] = 0ull;
+            pth[it] = 0ull;
             if constexpr (PK)
-                pkv[it] = ((unsigned long long)fkey32((float)S.v[it]) << 32) | (r.last ? ~row : row);
+                pkv[it] = r.pkmode == 0 ? ((unsigned long long)fkey32((float)S.v[it]) << 32) | (r.last ? ~row : row)
+                                        : (unsigned long long)(r.pkmode == 1 ? row : ~row);
             if (!keep) continue;
             if (in_window(w, px, py)) {
                 const uint32_t lc = (uint32_t)(((py - w.y0) * w.w + (px - w.x0)) * ncat + pc);
                 if constexpr (PK) {
+                    if (thr && !better(thr64[lc], pkv[it])) continue;  // an earlier slot's
                     if (r.last) atomicMax(&win64[lc], pkv[it]);
                     else atomicMin(&win64[lc], pkv[it]);
                 } else {
@@ -2950,20 +2974,24 @@ __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r)
                 continue;
             }
             out[it] = true;
-            cell[it] = ((int64_t)py * a.W + px) * ncat + pc;
+            cell[it] = (((int64_t)py * a.W + px) * ncat + pc) * r.nsel + r.kslot;  // its slot
         }
         // (global-memory pointers: through a generic one these would be flat operations,
         // whose waits also wait for LDS)
 #pragma unroll
-        for (int it = 0; it < kRiwItems; ++it)
+        for (int it = 0; it < kRiwItems; ++it) {
             if (out[it]) cur[it] = (long long)__hip_atomic_load(index + cell[it], __ATOMIC_RELAXED,
                                                                 __HIP_MEMORY_SCOPE_AGENT);
+            if (thr && out[it]) pth[it] = __hip_atomic_load(index + cell[it] - 1, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT);
+        }
         far.load(a, chunk_of(k + 2 * (int64_t)gridDim.x), tid);
 #pragma unroll
         for (int it = 0; it < kRiwItems; ++it) {
             if (!out[it]) continue;
             if constexpr (PK) {  // unsigned: the plane starts at 0 (ARGMAX) / all ones (ARGMIN)
                 const unsigned long long c_ = (unsigned long long)cur[it];
+                if (thr && !better(pth[it], pkv[it])) continue;  // an earlier slot's
                 if (r.last) {
                     if (pkv[it] > c_)
                         __hip_atomic_fetch_max(index + cell[it], pkv[it], __ATOMIC_RELAXED,
@@ -2996,7 +3024,6 @@ __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r)
         step(k + 2 * G, sc, sb);
     }
     __syncthreads();
-    const uint32_t used = (uint32_t)(w.w * w.h * ncat);
     if constexpr (PK) {
         unsigned long long *mine = reinterpret_cast<unsigned long long *>(r.partials) +
                                    (size_t)blockIdx.x * r.cap;
@@ -3031,7 +3058,7 @@ __global__ __launch_bounds__(256) void rowindex_window_merge(RiwArgs r, int grou
             }
             const uint32_t pix = c / (uint32_t)ncat, cat = c - pix * (uint32_t)ncat;
             const int32_t ly = (int32_t)(pix / (uint32_t)w.w), lx = (int32_t)(pix - (uint32_t)ly * w.w);
-            r.index[((int64_t)(w.y0 + ly) * a.W + (w.x0 + lx)) * ncat + cat] = m;
+            r.index[(((int64_t)(w.y0 + ly) * a.W + (w.x0 + lx)) * ncat + cat) * r.nsel + r.kslot] = m;
         }
         return;
     }
@@ -3069,13 +3096,16 @@ __global__ __launch_bounds__(256) void rowindex_unpack_kernel(unsigned long long
     }
 }
 
+// pick: set the call's window record first (else it is already in the workspace);
+// unpack: turn a packed plane into rows afterwards.
 template <typename TC>
 static int riw_launch(const dsx_points_desc *d, const RiwArgs &r, const RiwPlan &P, uint8_t *ws,
-                      hipStream_t st) {
+                      hipStream_t st, bool pick = true, bool unpack = true) {
     int32_t *window = const_cast<int32_t *>(r.window);
     uint32_t *hist_sel = reinterpret_cast<uint32_t *>(ws + P.o_hist);
     int s;
-    if (d->window && d->reuse_window) {
+    if (!pick) {
+    } else if (d->window && d->reuse_window) {
         // the caller's stored window (sanitised by every kernel that reads it)
         hipLaunchKernelGGL(copy_window_kernel, dim3(1), dim3(64), 0, st,
                            (const int32_t *)d->window, window, (int32_t)DSX_STRATEGY_WINDOW);
@@ -3115,12 +3145,18 @@ static int riw_launch(const dsx_points_desc *d, const RiwArgs &r, const RiwPlan 
                            : rowindex_window_kernel<TC, double, true, CK_ANY, false>)
                    : rowindex_window_kernel<TC, double, false, CK_ANY, false>;
     }
-    if (r.packed) {  // float32 values (checked by the caller), NaN rows skipped
+    if (r.packed && nan) {  // float32 values (checked by the caller), NaN rows skipped
         if (a.cat.kind == DSX_CAT_NONE)
             kern = lin ? rowindex_window_kernel<TC, float, true, CK_NONE, true, true>
                        : rowindex_window_kernel<TC, float, true, CK_NONE, false, true>;
         else if (byte) kern = rowindex_window_kernel<TC, float, true, CK_CODE8, false, true>;
         else kern = rowindex_window_kernel<TC, float, true, CK_ANY, false, true>;
+    } else if (r.packed) {  // row keys, no value read
+        if (a.cat.kind == DSX_CAT_NONE)
+            kern = lin ? rowindex_window_kernel<TC, float, false, CK_NONE, true, true>
+                       : rowindex_window_kernel<TC, float, false, CK_NONE, false, true>;
+        else if (byte) kern = rowindex_window_kernel<TC, float, false, CK_CODE8, false, true>;
+        else kern = rowindex_window_kernel<TC, float, false, CK_ANY, false, true>;
     }
     const hipError_t e = hipFuncSetAttribute((const void *)kern,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds);
@@ -3129,7 +3165,7 @@ static int riw_launch(const dsx_points_desc *d, const RiwArgs &r, const RiwPlan 
     if ((s = launch_status())) return s;
     hipLaunchKernelGGL(rowindex_window_merge, dim3(grid_for(P.cap, 256, 4)), dim3(256), 0, st, r,
                        P.groups);
-    if (r.packed) {
+    if (r.packed && unpack) {
         if ((s = launch_status())) return s;
         const int64_t ncells = (int64_t)a.W * a.H * a.cat.ncat;
         hipLaunchKernelGGL(rowindex_unpack_kernel, dim3(grid_for(ncells, 256, 4)), dim3(256), 0, st,
@@ -3188,6 +3224,10 @@ extern "C" int dsx_points_rowindex(const dsx_points_desc *d, const dsx_view *v,
         rw.a = ra.a;
         rw.last = (r->mode == DSX_RI_LAST || r->mode == DSX_RI_ARGMAX) ? 1 : 0;
         rw.packed = packed ? 1 : 0;
+        rw.pkmode = 0;
+        rw.nsel = 1;
+        rw.kslot = 0;
+        rw.thr = 0;
         rw.nan_check = ra.nan_check;
         rw.index = ra.index;
         rw.window = reinterpret_cast<const int32_t *>(ws + P.o_window);
@@ -3202,6 +3242,53 @@ extern "C" int dsx_points_rowindex(const dsx_points_desc *d, const dsx_view *v,
     if (d->xy_dtype == DSX_F32) return rowindex_launch<float>(ra, vdt, s);
     return rowindex_launch<double>(ra, vdt, s);
 }
+
+// The n-pass form of the n-value reductions (called by dsx_points_nselect): pass k keeps
+// slot k of every cell with the packed window pass, the previous slot's key the bound a
+// row's key must pass.  Packed keys: MAX key(v) << 32 | ~row (largest wins), MIN
+// key(v) << 32 | row, FIRST row, LAST ~row (smallest wins).  The slots keep their packed
+// keys (the caller turns them into rows).  Float32 values for MAX / MIN and for a NaN
+// check; the caller fills index with the empty key (0 for MAX, else all ones).
+namespace dsx {
+size_t ri_npass_workspace_size(const dsx_points_desc *d, const dsx_view *v) {
+    RiwPlan P;
+    if (riw_plan(d, v, P, true, true)) return 0;
+    return P.total;
+}
+
+int ri_npass(const dsx_points_desc *d, const dsx_view *v, int mode, int nsel, int nan_check,
+             int64_t *index, void *workspace, size_t workspace_bytes, hipStream_t st) {
+    RiwPlan P;
+    int s;
+    if ((s = riw_plan(d, v, P, true, true))) return s;
+    if (!workspace || workspace_bytes < P.total) return DSX_ERR_WORKSPACE;
+    uint8_t *ws = reinterpret_cast<uint8_t *>(workspace);
+    const bool value = mode == DSX_NS_MAX || mode == DSX_NS_MIN;
+    RiwArgs rw;
+    rw.a = args_of(d, v);
+    rw.last = mode == DSX_NS_MAX ? 1 : 0;
+    rw.nan_check = (value || nan_check) ? 1 : 0;
+    rw.index = reinterpret_cast<unsigned long long *>(index);
+    rw.window = reinterpret_cast<const int32_t *>(ws + P.o_window);
+    rw.cap = P.cap;
+    rw.partials = reinterpret_cast<uint32_t *>(ws + P.o_partials);
+    rw.packed = 1;
+    rw.pkmode = value ? 0 : (mode == DSX_NS_FIRST ? 1 : 2);
+    rw.nsel = nsel;
+    rw.thr = 1;
+    if (!rw.nan_check) rw.a.val = rw.a.x;  // (no value read)
+    dsx_points_desc dd = *d;
+    dd.window = nullptr;  // the pass picks its own window, once
+    dd.reuse_window = 0;
+    for (int k = 0; k < nsel; ++k) {
+        rw.kslot = k;
+        s = d->xy_dtype == DSX_F32 ? riw_launch<float>(&dd, rw, P, ws, st, k == 0, false)
+                                   : riw_launch<double>(&dd, rw, P, ws, st, k == 0, false);
+        if (s) return s;
+    }
+    return DSX_OK;
+}
+}  // namespace dsx
 
 extern "C" size_t dsx_points_indexed_workspace_size(const dsx_points_desc *d, const dsx_view *v,
                                                     const dsx_planes *p, const dsx_grid_ref *g) {

@@ -61,6 +61,15 @@ static int check_points(const dsx_points_desc *d, const dsx_view *v) {
     return check_cat(d->cat);
 }
 
+// The n-pass form of the n-value reductions (defined with the row-index window pass in
+// dsx_points.hip, run by dsx_points_nselect).
+__attribute__((visibility("hidden"))) size_t ri_npass_workspace_size(const dsx_points_desc *d,
+                                                                     const dsx_view *v);
+__attribute__((visibility("hidden"))) int ri_npass(const dsx_points_desc *d, const dsx_view *v,
+                                                   int mode, int nsel, int nan_check,
+                                                   int64_t *index, void *workspace,
+                                                   size_t workspace_bytes, hipStream_t st);
+
 static PointArgs args_of(const dsx_points_desc *d, const dsx_view *v) {
     PointArgs a;
     a.n = d->n; a.x = d->x; a.y = d->y; a.val = d->val; a.cat = cat_of(d->cat);

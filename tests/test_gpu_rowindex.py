@@ -91,7 +91,8 @@ def test_row_index_lines_not_supported():
 N_MODES = {"max_n": "max", "min_n": "min", "first_n": "first", "last_n": "last"}
 
 
-@pytest.mark.parametrize("n", [1, 4])
+# n <= 8 takes the n-pass form (float32 values for max_n / min_n), n = 12 the sort
+@pytest.mark.parametrize("n", [1, 4, 12])
 @pytest.mark.parametrize("by", [False, True], ids=["plain", "by-cat"])
 @pytest.mark.parametrize("vdt", ["f4", "f8"])
 def test_n_reductions_vs_oracle(vdt, by, n):
@@ -282,3 +283,45 @@ def test_packed_where_max_min_vs_oracle(by):
         r.index = dev.ptr(idx)
         assert lib.dsx_points_rowindex(d, vw, r, None, 0, dev.stream_handle()) == -1  # DSX_ERR_NULL
     d.n = n
+
+
+@pytest.mark.parametrize("by", [False, True], ids=["plain", "by-cat"])
+def test_n_pass_on_a_large_canvas_vs_oracle(by):
+    """max_n / min_n / first_n / last_n (n = 3, float32 values) on a canvas much larger
+    than the LDS window (1920 x 1080, a hot 0.01-sigma blob plus uniform rows): the
+    n-pass form keeps slot k of every cell in pass k, in the window and through the
+    global atomics with the previous slot's key as the bound.  Integer values give many
+    ties per cell; NaN rows are skipped; bit-exact vs the oracle's closed form."""
+    rng = np.random.default_rng(31)
+    n = 3_000_000
+    hot = rng.random(n) < 0.9
+    x = np.where(hot, rng.normal(0.5, 0.01, n), rng.uniform(0, 1, n)).astype(np.float32)
+    y = np.where(hot, rng.normal(0.5, 0.01, n), rng.uniform(0, 1, n)).astype(np.float32)
+    v = rng.integers(1, 40, n).astype(np.float32)
+    v[rng.random(n) < 0.02] = np.nan
+    cat = pd.Categorical(rng.choice(list("pqrs"), n), categories=list("pqrs"))
+    df = pd.DataFrame({"x": x, "y": y, "v": v, "cat": cat})
+    W, H = 1920, 1080
+    cvs = dsa.Canvas(W, H, x_range=(0, 1), y_range=(0, 1))
+    src = dsa.DeviceFrame.from_pandas(df)
+    view = G.view_for(W, H, (0, 1), (0, 1))
+    bins = cf.point_bins(x.astype(np.float64), y.astype(np.float64), view, W, H)
+    ncat = 4 if by else 1
+    cell = ri.cells(bins, df["cat"].cat.codes.to_numpy() if by else None, ncat)
+    ncells = W * H * ncat
+    k = 3
+    shape = (H, W, ncat, k) if by else (H, W, k)
+
+    def wrap(r):
+        return dsa.by("cat", r) if by else r
+
+    got = cvs.points(src, "x", "y", dsa.summary(
+        mx=wrap(dsa.where(dsa.max_n("v", n=k))), mn=wrap(dsa.where(dsa.min_n("v", n=k))),
+        fi=wrap(dsa.where(dsa.first_n("v", n=k))), la=wrap(dsa._max_n_row_index(n=k))))
+    vv = v.astype(np.float64)
+    for name, mode, val in (("mx", "max", vv), ("mn", "min", vv), ("fi", "first", vv),
+                            ("la", "last", None)):
+        want = ri.select_n_closed_form(cell, ncells, k, mode, val).reshape(shape)
+        np.testing.assert_array_equal(got[name].values, want, err_msg=name)
+    # second slots of cells outside the window (the blob holds ~7.5k cells)
+    assert (got["mx"].values[..., 1] >= 0).sum() > 20_000
