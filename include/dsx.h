@@ -268,7 +268,9 @@ int dsx_points_rowindex(const dsx_points_desc *d, const dsx_view *v,
  * last n, latest first (last_n, _max_n_row_index), MAX / MIN the rows of the n largest /
  * smallest values, rows that tie keeping row order (max_n / min_n insert only before a
  * strictly smaller / larger value).  index[cell * n + k]; -1 pads cells with fewer rows.
- * Rows with a NaN value are skipped when nan_check is set (always for MAX / MIN).        */
+ * Rows with a NaN value are skipped when nan_check is set (always for MAX / MIN).  n <= 8
+ * (float32 values where values are read) takes n window passes, one per slot; otherwise
+ * one radix sort of the kept rows.  The workspace size depends on which.               */
 typedef enum { DSX_NS_FIRST = 1, DSX_NS_LAST = 2, DSX_NS_MAX = 3, DSX_NS_MIN = 4 }
     dsx_ns_mode;
 
