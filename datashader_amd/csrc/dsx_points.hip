@@ -415,13 +415,46 @@ enum : int { CK_NONE = 0, CK_BYTE = 1, CK_ANY = 2, CK_CODE8 = 3 };
 // past the end reads the last row and is dropped when binned), so the loads of a chunk
 // are a fixed-length run and the compiler waits for exactly the chunk it bins while the
 // chunks prefetched behind it stay in flight.  Loaded values are not touched here.
-template <typename TC, typename TV, bool HAS_VAL, int CK>
+// WIDE (float32 columns, 16-byte aligned, n a multiple of 4 -- checked on the host): the
+// thread's rows are b + 4 tid .. b + 4 tid + 3, one 16-byte load per column (the streams
+// then run ~10 % faster than with a 4-byte load per row: 6.95 against 6.24 TB/s for three
+// columns on one MI355X).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename TC, typename TV, bool HAS_VAL, int CK, bool WIDE = false>
 struct Rows {
     TC x[kItems], y[kItems];
     TV v[kItems];
     int64_t c[kItems];
 
     __device__ __forceinline__ void load(const PointArgs &a, int64_t ck, int tid) {
+        if constexpr (WIDE) {
+            static_assert(sizeof(TC) == 4 && (!HAS_VAL || sizeof(TV) == 4) && CK == CK_NONE &&
+                          kItems == 4, "wide rows: float32 columns, four rows a thread");
+            int64_t b = ck * kChunk;
+            uint32_t lastq;  // the last readable group of four rows
+            if (b >= a.n) {  // a prefetch past the end: re-read the last four rows
+                b = a.n - 4;
+                lastq = 0;
+            } else {
+                const int64_t rem = a.n - b;
+                lastq = rem >= kChunk ? (uint32_t)(kChunk / 4 - 1) : (uint32_t)(rem / 4 - 1);
+            }
+            const uint32_t q = (uint32_t)tid < lastq ? (uint32_t)tid : lastq;
+            const f32x4 X = stream_load(reinterpret_cast<const f32x4 *>(reinterpret_cast<const float *>(a.x) + b) + q);
+            const f32x4 Y = stream_load(reinterpret_cast<const f32x4 *>(reinterpret_cast<const float *>(a.y) + b) + q);
+            f32x4 V = {0.f, 0.f, 0.f, 0.f};
+            if constexpr (HAS_VAL)
+                V = stream_load(reinterpret_cast<const f32x4 *>(reinterpret_cast<const float *>(a.val) + b) + q);
+#pragma unroll
+            for (int it = 0; it < kItems; ++it) {
+                x[it] = X[it];
+                y[it] = Y[it];
+                v[it] = HAS_VAL ? (TV)V[it] : TV(0);
+                c[it] = 0;
+            }
+            return;
+        }
         // chunk base and last readable in-chunk row are uniform; each thread adds a
         // 32-bit offset (scalar base + vector offset addressing)
         int64_t b = ck * kChunk;
@@ -478,7 +511,7 @@ __device__ __forceinline__ int32_t scan_category(const Cat &c, int64_t raw) {
 // LIN: linear axes (no log10 call in the loop); WF: the window's plane set when known at
 // compile time (0: read from the layout).
 template <typename TC, typename TV, bool HAS_VAL, int KEYB, bool SQDEV, int CK, bool LIN = false,
-          uint32_t WF = 0>
+          uint32_t WF = 0, bool WIDE = false>
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArgs s) {
     if (tiles_run(s.window)) return;  // uniform: the tile strategy's scan runs instead
     extern __shared__ __align__(16) uint8_t smem[];
@@ -502,7 +535,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
     // software pipeline, two chunks deep: while chunk ck is binned, the loads of chunks
     // ck + G and ck + 2G are in flight.  Three register sets rotate roles (binned / in
     // flight / being loaded), so no set is ever copied: a copy would wait for its loads.
-    using R = Rows<TC, TV, HAS_VAL, CK>;
+    using R = Rows<TC, TV, HAS_VAL, CK, WIDE>;
     R ra, rb, rc;
     ra.load(a, blockIdx.x, tid);
     rb.load(a, (int64_t)blockIdx.x + gridDim.x, tid);
@@ -514,7 +547,8 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
                                              : (rem64 > 0 ? (uint32_t)rem64 : 0u);
 #pragma unroll
         for (int it = 0; it < kItems; ++it) {
-            const uint32_t j = (uint32_t)(it * kScanThreads + tid);
+            // the row's place in the chunk (rows past rem are dropped)
+            const uint32_t j = WIDE ? (uint32_t)(4 * tid + it) : (uint32_t)(it * kScanThreads + tid);
             int32_t px = 0, py = 0, pc = 0;
             bool keep;
             if constexpr (LIN) {
@@ -2046,6 +2080,22 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
                 scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE, true, PL_CNTV | PL_SUM>;
             else if (wf == PL_CNT)
                 scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE, true, PL_CNT>;
+        }
+        // float32 columns, 16-byte aligned, whole groups of four rows: the wide loads
+        const bool wide = sizeof(TC) == 4 && (!HAS_VAL || sizeof(TV) == 4) && d->n >= 4 &&
+                          d->n % 4 == 0 && ((uintptr_t)a.x % 16) == 0 && ((uintptr_t)a.y % 16) == 0 &&
+                          (!HAS_VAL || ((uintptr_t)a.val % 16) == 0);
+        if constexpr (sizeof(TC) == 4 && (!HAS_VAL || sizeof(TV) == 4)) {
+            if (wide) {
+                if constexpr (!HAS_VAL) {
+                    if (wf == PL_CNT) scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE, true, PL_CNT, true>;
+                } else if constexpr (!SQDEV) {
+                    if (wf == (PL_CNT | PL_SUM))
+                        scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE, true, PL_CNT | PL_SUM, true>;
+                    else if (wf == (PL_CNTV | PL_SUM))
+                        scan = scan_kernel<TC, TV, HAS_VAL, KEYB, SQDEV, CK_NONE, true, PL_CNTV | PL_SUM, true>;
+                }
+            }
         }
     }
     hipError_t e;

@@ -248,6 +248,32 @@ def test_skewed_hot_bins_compensated_sum():
     assert _rel_err(ds["s"].values, cf.sum_nan(bins, v, W, H)) <= 1e-6
 
 
+@pytest.mark.parametrize("n,offset", [(4, 0), (8, 0), (4100, 0), (1_000_004, 0), (5, 0),
+                                      (1_000_003, 0), (1_000_004, 1)])
+def test_count_mean_row_counts_and_alignment(n, offset):
+    """count + mean over linear axes take 16-byte loads of four rows when the float32
+    columns are 16-byte aligned and n is a multiple of 4 (a partial last chunk and the
+    prefetch past the end included), else 4-byte loads: both equal the closed forms.
+    offset 1 starts the columns one element into a larger tensor (misaligned)."""
+    import torch
+    rng = np.random.default_rng(n + offset)
+    m = n + offset
+    hot = rng.random(m) < 0.8
+    x = np.where(hot, rng.normal(0.5, 0.01, m), rng.uniform(0, 1, m)).astype(np.float32)
+    y = np.where(hot, rng.normal(0.5, 0.01, m), rng.uniform(0, 1, m)).astype(np.float32)
+    v = rng.lognormal(0, 1, m).astype(np.float32)
+    v[rng.random(m) < 0.03] = np.nan
+    cols = {k: torch.from_numpy(a).cuda()[offset:] for k, a in (("x", x), ("y", y), ("v", v))}
+    x, y, v = x[offset:], y[offset:], v[offset:]
+    W, H = 1920, 1080
+    ds = dsa.Canvas(W, H, x_range=(0, 1), y_range=(0, 1)).points(
+        dsa.DeviceFrame(cols), "x", "y", dsa.summary(c=dsa.count(), m=dsa.mean("v")))
+    view = G.view_for(W, H, (0, 1), (0, 1))
+    bins = cf.point_bins(x, y, view, W, H)
+    np.testing.assert_array_equal(ds["c"].values, cf.count(bins, W, H))
+    assert _rel_err(ds["m"].values, cf.mean(bins, v, W, H)) <= 1e-6
+
+
 def test_by_category_vs_oracle():
     from datashader_amd import synthetic
     n = 2_000_000
