@@ -429,8 +429,9 @@ struct Rows {
 
     __device__ __forceinline__ void load(const PointArgs &a, int64_t ck, int tid) {
         if constexpr (WIDE) {
-            static_assert(sizeof(TC) == 4 && (!HAS_VAL || sizeof(TV) == 4) && CK == CK_NONE &&
-                          kItems == 4, "wide rows: float32 columns, four rows a thread");
+            static_assert(sizeof(TC) == 4 && (!HAS_VAL || sizeof(TV) == 4) &&
+                          (CK == CK_NONE || CK == CK_CODE8) && kItems == 4,
+                          "wide rows: float32 columns (one-byte codes), four rows a thread");
             int64_t b = ck * kChunk;
             uint32_t lastq;  // the last readable group of four rows
             if (b >= a.n) {  // a prefetch past the end: re-read the last four rows
@@ -446,12 +447,15 @@ struct Rows {
             f32x4 V = {0.f, 0.f, 0.f, 0.f};
             if constexpr (HAS_VAL)
                 V = stream_load(reinterpret_cast<const f32x4 *>(reinterpret_cast<const float *>(a.val) + b) + q);
+            uint32_t C4 = 0u;  // the four rows' one-byte codes (a 4-byte aligned column)
+            if constexpr (CK == CK_CODE8)
+                C4 = reinterpret_cast<const uint32_t *>(reinterpret_cast<const uint8_t *>(a.cat.col) + b)[q];
 #pragma unroll
             for (int it = 0; it < kItems; ++it) {
                 x[it] = X[it];
                 y[it] = Y[it];
                 v[it] = HAS_VAL ? (TV)V[it] : TV(0);
-                c[it] = 0;
+                c[it] = CK == CK_CODE8 ? (int64_t)((C4 >> (8 * it)) & 0xFFu) : 0;
             }
             return;
         }
@@ -1293,7 +1297,7 @@ constexpr int kBktOwn = 4;                      // buckets a thread owns (nb <= 
 static_assert(kBktOwn * kScanThreads >= kMaxTiles, "every bucket has an owner");
 constexpr int kBktSpare = 64;                   // spare bucket words (one per lane)
 
-template <typename TC, typename TV, bool HAS_VAL, int CK, bool LIN>
+template <typename TC, typename TV, bool HAS_VAL, int CK, bool LIN, bool WIDE = false>
 __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArgs s) {
     if (!tiles_run(s.rec)) return;  // uniform: the window strategy's scan runs instead
     constexpr int kSteps = (HAS_VAL || sizeof(TC) == 8) ? 1 : 3;  // chunks between checks
@@ -1489,7 +1493,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
         // check's barrier orders these writes before the next flush
     };
     const int64_t nchunks = (a.n + kChunk - 1) / kChunk;
-    using R = Rows<TC, TV, HAS_VAL, CK>;
+    using R = Rows<TC, TV, HAS_VAL, CK, WIDE>;
     // chunks loaded ahead: three where the checks come every three chunks (four register
     // sets: the loads in flight cover a flush), else two; the generic category column (a
     // 64-bit raw value a row) keeps two, as a fourth set would spill registers
@@ -1514,7 +1518,7 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
         uint16_t oo[kItems];
 #pragma unroll
         for (int it = 0; it < kItems; ++it) {
-            const uint32_t j = (uint32_t)(it * kScanThreads + tid);
+            const uint32_t j = WIDE ? (uint32_t)(4 * tid + it) : (uint32_t)(it * kScanThreads + tid);
             int32_t px = 0, py = 0, pc = 0;
             bool keep;
             if constexpr (LIN) {  // as in scan_kernel
@@ -2140,6 +2144,18 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
         else if ((a.cat.dtype == DSX_I8 || a.cat.dtype == DSX_U8) && a.cat.kind == DSX_CAT_CODES)
             bscan = lin ? bscan_kernel<TC, TV, HAS_VAL, CK_CODE8, true>
                         : bscan_kernel<TC, TV, HAS_VAL, CK_CODE8, false>;
+        // float32 columns without values, 16-byte aligned (the code column 4-byte aligned),
+        // whole groups of four rows: the wide loads
+        if constexpr (sizeof(TC) == 4 && !HAS_VAL) {
+            const bool codes = (a.cat.dtype == DSX_I8 || a.cat.dtype == DSX_U8) &&
+                               a.cat.kind == DSX_CAT_CODES;
+            const bool wide = lin && d->n >= 4 && d->n % 4 == 0 && ((uintptr_t)a.x % 16) == 0 &&
+                              ((uintptr_t)a.y % 16) == 0 &&
+                              (a.cat.kind == DSX_CAT_NONE || (codes && ((uintptr_t)a.cat.col % 4) == 0));
+            if (wide)
+                bscan = a.cat.kind == DSX_CAT_NONE ? bscan_kernel<TC, TV, HAS_VAL, CK_NONE, true, true>
+                                                   : bscan_kernel<TC, TV, HAS_VAL, CK_CODE8, true, true>;
+        }
         else if (a.cat.dtype == DSX_I8 || a.cat.dtype == DSX_U8)
             bscan = bscan_kernel<TC, TV, HAS_VAL, CK_BYTE, false>;
         e = hipFuncSetAttribute((const void *)bscan, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -115,13 +115,14 @@ def test_rows_in_one_tile_by_category(strategy):
     assert agg.values[500].sum() == n
 
 
+@pytest.mark.parametrize("n", [1_500_003, 1_500_004])
 @pytest.mark.parametrize("dt", [np.int8, np.uint8])
-def test_byte_codes_out_of_range_vs_oracle(strategy, dt):
+def test_byte_codes_out_of_range_vs_oracle(strategy, dt, n):
     """One-byte category codes outside [0, ncat): an int8 code k < 0 counts as k + ncat
     (once; still negative is dropped), codes >= ncat are dropped, for int8 and uint8
-    columns on a 1024 x 256 x 16 canvas, bit-exact vs the oracle."""
+    columns on a 1024 x 256 x 16 canvas, bit-exact vs the oracle (n a multiple of 4: the
+    tile scan's 16-byte row loads, four codes in one 4-byte load)."""
     rng = np.random.default_rng(11)
-    n = 1_500_003
     W, H, ncat = 1024, 256, 16
     x = rng.random(n).astype(np.float32)
     y = rng.random(n).astype(np.float32)
