@@ -2940,7 +2940,7 @@ static int riw_plan(const dsx_points_desc *d, const dsx_view *v, RiwPlan &P, boo
 // NANCHK: rows whose value is NaN are skipped (the value column is read); CK, LIN: as
 // in scan_kernel (the category read with the row's other columns, the branch-free linear
 // projection); PK: the packed ARGMAX / ARGMIN form (float32 values).
-template <typename TC, typename TV, bool NANCHK, int CK, bool LIN, bool PK = false>
+template <typename TC, typename TV, bool NANCHK, int CK, bool LIN, bool PK = false, bool WIDE = false>
 __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r) {
     static_assert(kRiwThreads == kScanThreads && kRiwItems == kItems, "chunks as the scan's");
     static_assert(!PK || sizeof(TV) == 4, "packed: float32 values");
@@ -2976,7 +2976,7 @@ __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r)
     // that after the first sweep most rows fail the global skip test; a step past the
     // last chunk reads chunk nchunks (rows past the end: re-reads the last row, dropped)
     auto chunk_of = [&](int64_t k) { return k >= nchunks ? nchunks : (r.last ? nchunks - 1 - k : k); };
-    using R = Rows<TC, TV, NANCHK, CK>;
+    using R = Rows<TC, TV, NANCHK, CK, WIDE>;
     auto *const index = global_ptr(r.index);
     auto *const sindex = global_ptr(reinterpret_cast<long long *>(r.index));
     // Three register sets rotate roles (binned / in flight / being loaded), as in the
@@ -2997,7 +2997,8 @@ __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r)
         bool out[kRiwItems];
 #pragma unroll
         for (int it = 0; it < kRiwItems; ++it) {
-            const uint32_t j = (uint32_t)(it * kRiwThreads + tid);
+            // the row's place in the chunk: as the loader placed it
+            const uint32_t j = WIDE ? (uint32_t)(4 * tid + it) : (uint32_t)(it * kRiwThreads + tid);
             int32_t px = 0, py = 0, pc = 0;
             bool keep;
             if constexpr (LIN) {  // as in scan_kernel
@@ -3068,7 +3069,8 @@ __global__ __launch_bounds__(kRiwThreads) void rowindex_window_kernel(RiwArgs r)
                 }
                 continue;
             }
-            const long long row = (long long)(b + it * kRiwThreads + tid);
+            const long long row =
+                (long long)(b + (WIDE ? 4 * tid + it : it * kRiwThreads + tid));
             if (r.last) {
                 if (row > cur[it])
                     __hip_atomic_fetch_max(sindex + cell[it], row, __ATOMIC_RELAXED,
@@ -3223,6 +3225,21 @@ static int riw_launch(const dsx_points_desc *d, const RiwArgs &r, const RiwPlan 
                        : rowindex_window_kernel<TC, float, false, CK_NONE, false, true>;
         else if (byte) kern = rowindex_window_kernel<TC, float, false, CK_CODE8, false, true>;
         else kern = rowindex_window_kernel<TC, float, false, CK_ANY, false, true>;
+    }
+    // float32 columns, 16-byte aligned, whole groups of four rows, linear axes, no
+    // categorizer: the wide row loads
+    if constexpr (sizeof(TC) == 4) {
+        const bool wide = lin && a.cat.kind == DSX_CAT_NONE && d->n >= 4 && d->n % 4 == 0 &&
+                          ((uintptr_t)a.x % 16) == 0 && ((uintptr_t)a.y % 16) == 0 &&
+                          (!nan || (f32v && ((uintptr_t)a.val % 16) == 0));
+        if (wide) {
+            if (r.packed)
+                kern = nan ? rowindex_window_kernel<TC, float, true, CK_NONE, true, true, true>
+                           : rowindex_window_kernel<TC, float, false, CK_NONE, true, true, true>;
+            else
+                kern = nan ? rowindex_window_kernel<TC, float, true, CK_NONE, true, false, true>
+                           : rowindex_window_kernel<TC, double, false, CK_NONE, true, false, true>;
+        }
     }
     const hipError_t e = hipFuncSetAttribute((const void *)kern,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds);
