@@ -1089,8 +1089,10 @@ __device__ void tile_store(const uint8_t *smem, const Lay &L, const PartArgs &p,
     int64_t c1 = c0 + p.tile_cells;
     if (c1 > ncells) c1 = ncells;
     for (int64_t gcell = c0 + threadIdx.x; gcell < c1; gcell += kTileThreads) {
-        const int64_t pix = gcell / ncat;
-        const int32_t py = (int32_t)(pix / W), px = (int32_t)(pix - (int64_t)py * W);
+        // cells < 2^32 (checked on the host): 32-bit divisions, not 64-bit ones
+        const uint32_t gc = (uint32_t)gcell;
+        const uint32_t pix = ncat == 1 ? gc : gc / (uint32_t)ncat;
+        const int32_t py = (int32_t)(pix / (uint32_t)W), px = (int32_t)(pix - (uint32_t)py * (uint32_t)W);
         if (in_window(w, px, py)) {
             if (L.flags & PL_NAN) region_add_to<KEYB>(smem, L, (uint32_t)(gcell - c0), out, gcell);
             continue;
@@ -1099,7 +1101,8 @@ __device__ void tile_store(const uint8_t *smem, const Lay &L, const PartArgs &p,
     }
 }
 
-template <typename TV, bool HAS_VAL, int KEYB, bool SQDEV>
+// WF: the plane set when known at compile time (0: read from the layout).
+template <typename TV, bool HAS_VAL, int KEYB, bool SQDEV, uint32_t WF = 0>
 __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay L,
                                                                const int32_t *window, int32_t W,
                                                                int32_t H, int64_t wcap_px,
@@ -1147,7 +1150,7 @@ __global__ __launch_bounds__(kTileThreads) void tile_agg_kernel(PartArgs p, Lay 
                     const double d = v - mean[cur.gc[u]];
                     add = d * d;
                 }
-                region_add<KEYB>(smem, L, lc, vnan, v, add);
+                region_add<KEYB, WF>(smem, L, lc, vnan, v, add);
             }
             j0 += kStep;
         };
@@ -2201,6 +2204,15 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
     if ((s = launch_status())) return s;
     if (P.run_window) {
         auto tile = tile_agg_kernel<TV, HAS_VAL, KEYB, SQDEV>;
+        if constexpr (!SQDEV) {  // the common plane sets: compile-time planes
+            if (P.tl.flags == PL_CNT) tile = tile_agg_kernel<TV, HAS_VAL, KEYB, SQDEV, PL_CNT>;
+            else if (P.tl.flags == (PL_CNT | PL_SUM))
+                tile = tile_agg_kernel<TV, HAS_VAL, KEYB, SQDEV, PL_CNT | PL_SUM>;
+            else if (P.tl.flags == (PL_CNTV | PL_SUM))
+                tile = tile_agg_kernel<TV, HAS_VAL, KEYB, SQDEV, PL_CNTV | PL_SUM>;
+            else if (P.tl.flags == (PL_CNT | PL_SUM | PL_NAN))  // count + mean / sum
+                tile = tile_agg_kernel<TV, HAS_VAL, KEYB, SQDEV, PL_CNT | PL_SUM | PL_NAN>;
+        }
         e = hipFuncSetAttribute((const void *)tile, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)P.tile_lds);
         if (e != hipSuccess) { record_hip_error(e); return DSX_ERR_HIP; }
