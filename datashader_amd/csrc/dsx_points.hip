@@ -672,7 +672,9 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(PointArgs a, ScanArg
 constexpr int kMergeCells = 64;
 constexpr int kMergeWays = 16;
 
-template <int KEYB>
+// WF: the plane set when known at compile time (0: read from the layout); known, the
+// loop's loads carry no branches and four partials' loads are in flight at a time.
+template <int KEYB, uint32_t WF = 0>
 __global__ __launch_bounds__(kMergeCells * kMergeWays) void merge_window_kernel(
     Lay L, const uint8_t *partials, int nparts, const int32_t *window, int32_t W, int32_t H,
     int32_t ncat, Planes out) {
@@ -683,6 +685,7 @@ __global__ __launch_bounds__(kMergeCells * kMergeWays) void merge_window_kernel(
     __shared__ uint32_t s_cnt[kMergeWays][kMergeCells];
     __shared__ uint32_t s_cntv[kMergeWays][kMergeCells];
     __shared__ uint32_t s_nan[kMergeWays][kMergeCells];
+    const uint32_t f = WF ? WF : L.flags;
     const Window w = load_window(window, W, H, (int64_t)(L.cap / (uint32_t)ncat));
     const uint32_t ncells = (uint32_t)(w.w * w.h * ncat);
     const int cl = threadIdx.x % kMergeCells, way = threadIdx.x / kMergeCells;
@@ -692,20 +695,21 @@ __global__ __launch_bounds__(kMergeCells * kMergeWays) void merge_window_kernel(
     typename K::T mn = K::kMinEmpty, mx = 0;
     uint32_t cnt = 0u, cntv = 0u, nan = 0u;
     if (live) {
+#pragma unroll 4
         for (int g = way; g < nparts; g += kMergeWays) {
             const uint8_t *r = partials + (size_t)g * L.bytes;
-            if (L.flags & PL_SUM) sum += reinterpret_cast<const double *>(r + L.o_sum)[c];
-            if (L.flags & PL_MIN) {
+            if (f & PL_SUM) sum += reinterpret_cast<const double *>(r + L.o_sum)[c];
+            if (f & PL_MIN) {
                 const typename K::T k = reinterpret_cast<const typename K::T *>(r + L.o_min)[c];
                 mn = k < mn ? k : mn;
             }
-            if (L.flags & PL_MAX) {
+            if (f & PL_MAX) {
                 const typename K::T k = reinterpret_cast<const typename K::T *>(r + L.o_max)[c];
                 mx = k > mx ? k : mx;
             }
-            if (L.flags & PL_CNT) cnt += reinterpret_cast<const uint32_t *>(r + L.o_cnt)[c];
-            if (L.flags & PL_CNTV) cntv += reinterpret_cast<const uint32_t *>(r + L.o_cntv)[c];
-            if (L.flags & PL_NAN) nan += reinterpret_cast<const uint32_t *>(r + L.o_nan)[c];
+            if (f & PL_CNT) cnt += reinterpret_cast<const uint32_t *>(r + L.o_cnt)[c];
+            if (f & PL_CNTV) cntv += reinterpret_cast<const uint32_t *>(r + L.o_cntv)[c];
+            if (f & PL_NAN) nan += reinterpret_cast<const uint32_t *>(r + L.o_nan)[c];
         }
     }
     s_sum[way][cl] = sum; s_min[way][cl] = mn; s_max[way][cl] = mx;
@@ -726,19 +730,19 @@ __global__ __launch_bounds__(kMergeCells * kMergeWays) void merge_window_kernel(
     const uint32_t pix = c / (uint32_t)ncat, cat = c - pix * (uint32_t)ncat;
     const int32_t ly = (int32_t)(pix / (uint32_t)w.w), lx = (int32_t)(pix - (uint32_t)ly * w.w);
     const int64_t g = ((int64_t)(w.y0 + ly) * W + (w.x0 + lx)) * ncat + cat;
-    if (L.flags & PL_MIN) reinterpret_cast<typename K::T *>(out.min)[g] = s_min[0][cl];
-    if (L.flags & PL_MAX) reinterpret_cast<typename K::T *>(out.max)[g] = s_max[0][cl];
+    if (f & PL_MIN) reinterpret_cast<typename K::T *>(out.min)[g] = s_min[0][cl];
+    if (f & PL_MAX) reinterpret_cast<typename K::T *>(out.max)[g] = s_max[0][cl];
     uint32_t nv = 0u;
-    if (L.flags & PL_CNT) {
+    if (f & PL_CNT) {
         if (out.count) out.count[g] = s_cnt[0][cl];
         // count() with count(v): the window holds the rows with a value only (rows with
         // a NaN value are spilled), so its count is also its count(v); the tile adds the
         // spilled rows of the cell afterwards
-        nv = (L.flags & PL_NAN) ? s_cnt[0][cl] - s_nan[0][cl] : s_cnt[0][cl];
+        nv = (f & PL_NAN) ? s_cnt[0][cl] - s_nan[0][cl] : s_cnt[0][cl];
     }
-    if (L.flags & PL_CNTV) nv = s_cntv[0][cl];
+    if (f & PL_CNTV) nv = s_cntv[0][cl];
     if (out.countv) out.countv[g] = nv;
-    if (L.flags & PL_SUM) {
+    if (f & PL_SUM) {
         if (out.sum) out.sum[g] = s_sum[0][cl];
         if (out.mean) out.mean[g] = mean_of(s_sum[0][cl], nv);
     }
@@ -2168,7 +2172,11 @@ static int run_points(const dsx_points_desc *d, const PointArgs &a, const Plan &
         if ((s = launch_status())) return s;
     }
     if (P.run_window) {
-        hipLaunchKernelGGL((merge_window_kernel<KEYB>), dim3((P.wl.cap + kMergeCells - 1) / kMergeCells),
+        auto merge_w = merge_window_kernel<KEYB>;  // the common plane sets: compile-time planes
+        if (P.wl.flags == PL_CNT) merge_w = merge_window_kernel<KEYB, PL_CNT>;
+        else if (P.wl.flags == (PL_CNT | PL_SUM)) merge_w = merge_window_kernel<KEYB, PL_CNT | PL_SUM>;
+        else if (P.wl.flags == (PL_CNTV | PL_SUM)) merge_w = merge_window_kernel<KEYB, PL_CNTV | PL_SUM>;
+        hipLaunchKernelGGL(merge_w, dim3((P.wl.cap + kMergeCells - 1) / kMergeCells),
                            dim3(kMergeCells * kMergeWays), 0, st, P.wl, (const uint8_t *)sa.partials,
                            P.scan_groups, (const int32_t *)window, a.W, a.H, ncat, out);
         if ((s = launch_status())) return s;
