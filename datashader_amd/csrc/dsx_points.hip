@@ -1254,9 +1254,9 @@ __global__ __launch_bounds__(kTileThreads) void tile_merge_kernel(PartArgs p, La
 // entries (a 16-bit tile-local cell offset, and the value) and writes a bucket that fills
 // as one block of its workgroup's own block stream (nothing is reserved across
 // workgroups); rows that arrive at a full bucket go straight into the next blocks of
-// their tile.  A list kernel orders the blocks by tile, and the tile pass aggregates each
-// tile's blocks in LDS as it does the spill list.  A row costs one write and one read of
-// 2 + value bytes.
+// their tile.  At its end each scan workgroup lists its blocks in tile order, and the tile
+// pass aggregates each tile's blocks in LDS as it does the spill list.  A row costs one
+// write and one read of 2 + value bytes.
 constexpr int kBktMinC = 16;              // smallest bucket worth a block
 constexpr int kBktMaxC = 256;
 constexpr uint32_t kBktMaxTileCells = 32768;  // tile offsets < 2^15: 0xFFFF marks unused slots
