@@ -490,6 +490,15 @@ struct Rows {
     }
 };
 
+// float64 to int32 by the hardware conversion (truncation; out-of-range values saturate,
+// NaN gives 0): defined for every input, so a row outside the view needs no select before
+// it (its cell is dropped).  (A C++ conversion of an out-of-range value is undefined.)
+__device__ __forceinline__ int32_t cvt_sat_i32(double d) {
+    int32_t r;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(d));
+    return r;
+}
+
 // a * b + c in 32 bits, the multiply kept apart from the add: fused, they become a 64-bit
 // multiply-add whose unused high addend may share a register with a load still in flight,
 // and the wait for that load drains the scan's prefetch.
@@ -1528,12 +1537,12 @@ __global__ __launch_bounds__(kScanThreads) void bscan_kernel(PointArgs a, BktArg
             const uint32_t j = WIDE ? (uint32_t)(4 * tid + it) : (uint32_t)(it * kScanThreads + tid);
             int32_t px = 0, py = 0, pc = 0;
             bool keep;
-            if constexpr (LIN) {  // as in scan_kernel
+            if constexpr (LIN) {  // as in scan_kernel, without its selects (cvt_sat_i32)
                 const double x = (double)cur.x[it], y = (double)cur.y[it];
                 const bool inr = (a.xmin <= x) & (x <= a.xmax) & (a.ymin <= y) & (y <= a.ymax);
-                const double fx = inr ? __dadd_rn(__dmul_rn(x, a.sx), a.tx) : 0.0;
-                const double fy = inr ? __dadd_rn(__dmul_rn(y, a.sy), a.ty) : 0.0;
-                const int32_t xx = (int32_t)fx, yy = (int32_t)fy;
+                const double fx = __dadd_rn(__dmul_rn(x, a.sx), a.tx);
+                const double fy = __dadd_rn(__dmul_rn(y, a.sy), a.ty);
+                const int32_t xx = cvt_sat_i32(fx), yy = cvt_sat_i32(fy);
                 px = xx < a.W - 1 ? xx : a.W - 1;
                 py = yy < a.H - 1 ? yy : a.H - 1;
                 keep = (j < rem) & inr;
